@@ -1,0 +1,23 @@
+/* PLUTO compile-time configuration for the CPU oracle build: cart3d_plm_default_rk3 */
+#define  PHYSICS                        HD
+#define  DIMENSIONS                     3
+#define  GEOMETRY                       CARTESIAN
+#define  BODY_FORCE                     NO
+#define  COOLING                        NO
+#define  RECONSTRUCTION                 LINEAR
+#define  TIME_STEPPING                  RK3
+#define  NTRACER                        0
+#define  PARTICLES                      NO
+#define  USER_DEF_PARAMETERS            1
+
+#define  DUST_FLUID                     NO
+#define  EOS                            IDEAL
+#define  ENTROPY_SWITCH                 NO
+#define  INCLUDE_LES                    NO
+#define  THERMAL_CONDUCTION             NO
+#define  VISCOSITY                      NO
+#define  ROTATING_FRAME                 NO
+
+#define  GAMMA_EOS                      0
+
+#define  LIMITER                        DEFAULT

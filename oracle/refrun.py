@@ -1,0 +1,156 @@
+"""Test infrastructure: drive the CPU oracles.
+
+* ``run_reference`` runs the unmodified PLUTO reference build
+  (oracle/_ref/<problem>/pluto, compiled by oracle/ref/Makefile from the
+  reference sources) on a configuration and an initial condition, and
+  returns the state it dumps after the requested number of steps.
+* ``run_oracle`` runs the plain-C restatement (oracle/_build/liboracle.so).
+
+Only tests, the package smoke test and bench.py's cpu_baseline leg use
+this module; the product never does.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import pathlib
+import struct
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = pathlib.Path(__file__).resolve().parent
+REF_DIR = HERE / "_ref"
+LIB_PATH = HERE / "_build" / "liboracle.so"
+
+
+def build(with_reference: bool | None = None) -> None:
+    """Compile the oracle (and, when the reference sources are present,
+    the reference CPU builds)."""
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    if with_reference is None:
+        with_reference = pathlib.Path("/root/reference/Src").is_dir()
+    if with_reference:
+        subprocess.run(["make", "-s", "-j8", "-C", str(HERE / "ref"), "all"],
+                       check=True, stdout=subprocess.DEVNULL)
+
+
+def reference_binary(problem: str) -> pathlib.Path:
+    return REF_DIR / problem / "pluto"
+
+
+def read_dump(path: pathlib.Path, shape) -> dict:
+    raw = path.read_bytes()
+    magic, step, nx1, nx2, nx3, nvar = struct.unpack_from("<6i", raw, 0)
+    assert magic == 0x504D5844, "bad dump magic"
+    t, dt, wall = struct.unpack_from("<3d", raw, 24)
+    n = nvar * nx1 * nx2 * nx3
+    arr = np.frombuffer(raw, dtype="<f8", count=2 * n, offset=48)
+    V = arr[:n].reshape(nvar, nx3, nx2, nx1).copy()
+    U = arr[n:].reshape(nvar, nx3, nx2, nx1).copy()
+    assert V.shape == tuple(shape)
+    return {"V": V, "U": U, "step": step, "t": t, "dt": dt, "wall": wall}
+
+
+def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
+                  timeout: float = 3600.0) -> dict:
+    """Run the reference for ``nsteps`` (>= 2) steps from ``V0``.
+
+    PLUTO's ``-maxsteps N`` integrates N+1 steps (Src/main.c:233-240), so
+    N = nsteps - 1 is passed.  Returns V, U (Uc), t, next dt, the dt
+    sequence, and (time_only) the wall time of steps 2..nsteps."""
+    if nsteps < 2:
+        raise ValueError("the reference driver needs nsteps >= 2")
+    binary = reference_binary(cfg.reference_problem())
+    if not binary.exists():
+        raise FileNotFoundError(f"{binary} not built (make -C oracle/ref)")
+    with tempfile.TemporaryDirectory(prefix="pmxref_") as tmp:
+        tmp = pathlib.Path(tmp)
+        (tmp / "pluto.ini").write_text(cfg.pluto_ini())
+        np.ascontiguousarray(V0, dtype="<f8").tofile(tmp / "ic.bin")
+        env = dict(os.environ)
+        env["PMX_IC"] = str(tmp / "ic.bin")
+        env["PMX_DUMP"] = str(tmp / "dump.bin")
+        env.pop("PMX_TIME_ONLY", None)
+        if time_only:
+            env["PMX_TIME_ONLY"] = "1"
+        proc = subprocess.run([str(binary), "-i", "pluto.ini", "-maxsteps",
+                               str(nsteps - 1)], cwd=tmp, env=env,
+                              capture_output=True, text=True, timeout=timeout)
+        if proc.returncode != 0:
+            raise RuntimeError(f"reference failed:\n{proc.stdout[-2000:]}\n{proc.stderr[-2000:]}")
+        if time_only:
+            walls = [float(l.split("wall=")[1]) for l in proc.stdout.splitlines()
+                     if l.startswith("PMX_TIMING")]
+            return {"wall": walls[-1] - walls[0], "steps_timed": len(walls) - 1}
+        out = read_dump(tmp / "dump.bin", cfg.shape)
+        dts = np.loadtxt(tmp / "dump.bin.dt", ndmin=2)
+        out["dt_seq"] = dts[:, 2]
+        return out
+
+
+# ----------------------------------------------------------------------
+# plain-C restatement
+# ----------------------------------------------------------------------
+class _OrcCfg(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int), ("nx", ctypes.c_int * 3),
+                ("recon", ctypes.c_int), ("limiter", ctypes.c_int),
+                ("solver", ctypes.c_int), ("rk", ctypes.c_int),
+                ("bc", ctypes.c_int * 6), ("gamma", ctypes.c_double),
+                ("cfl", ctypes.c_double), ("cfl_max_var", ctypes.c_double),
+                ("xbeg", ctypes.c_double * 3), ("xend", ctypes.c_double * 3)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            build(with_reference=False)
+        _lib = ctypes.CDLL(str(LIB_PATH))
+        dp = ctypes.POINTER(ctypes.c_double)
+        _lib.orc_run.argtypes = [ctypes.POINTER(_OrcCfg), dp, ctypes.c_int, dp, dp, dp, dp]
+        _lib.orc_run.restype = ctypes.c_int
+        _lib.orc_riemann.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                     dp, dp, dp, dp, dp]
+        _lib.orc_cons_to_prim.argtypes = [ctypes.c_double, dp, dp]
+        _lib.orc_cons_to_prim.restype = ctypes.c_int
+        _lib.orc_prim_to_cons.argtypes = [ctypes.c_double, dp, dp]
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def orc_cfg(cfg) -> _OrcCfg:
+    c = _OrcCfg()
+    c.dims = cfg.dims
+    for d in range(3):
+        c.nx[d] = cfg.nx[d] if d < cfg.dims else 1
+        c.xbeg[d] = cfg.xbeg[d]
+        c.xend[d] = cfg.xend[d]
+    c.recon, c.limiter, c.solver, c.rk = cfg.recon, cfg.limiter, cfg.solver, cfg.rk
+    for s in range(6):
+        c.bc[s] = cfg.bc[s]
+    c.gamma, c.cfl, c.cfl_max_var = cfg.gamma, cfg.cfl, cfg.cfl_max_var
+    return c
+
+
+def run_oracle(cfg, V0: np.ndarray, nsteps: int, dt0: float | None = None,
+               t0: float = 0.0) -> dict:
+    V = np.ascontiguousarray(V0, dtype=np.float64).copy()
+    U = np.empty_like(V)
+    dt = np.array([cfg.first_dt if dt0 is None else dt0])
+    t = np.array([t0])
+    hist = np.empty(max(nsteps, 1))
+    c = orc_cfg(cfg)
+    rc = lib().orc_run(ctypes.byref(c), _ptr(V), nsteps, _ptr(dt), _ptr(t),
+                       _ptr(U), _ptr(hist))
+    if rc != 0:
+        raise RuntimeError(f"orc_run failed ({rc})")
+    return {"V": V, "U": U, "t": float(t[0]), "dt": float(dt[0]),
+            "dt_seq": hist[:nsteps].copy()}

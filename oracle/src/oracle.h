@@ -1,0 +1,61 @@
+/* Oracle: a plain-C CPU restatement of PLUTO-sirocco's finite-volume hydro
+ * step, used ONLY as the checker of the GPU library in tests/, in the
+ * package smoke test and in bench.py's cpu_baseline leg.  It is never
+ * linked into, loaded by or called from the product path.
+ *
+ * Each routine cites the reference file:line whose arithmetic it follows;
+ * the operation order matches the reference so that results agree bit for
+ * bit with the compiled reference (oracle/_ref) on the same inputs.
+ *
+ * Array layout (all interior-only, float64):
+ *     V[nv][k][j][i]   primitive variables  (rho, vx1, vx2, vx3, p)
+ *     U[nv][k][j][i]   conservative variables (rho, m1, m2, m3, E)
+ */
+#ifndef PMX_ORACLE_H
+#define PMX_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_PLM = 0, ORC_PPM = 1 };
+enum { ORC_LIM_DEFAULT = 0, ORC_LIM_VANLEER = 1, ORC_LIM_MC = 2,
+       ORC_LIM_MINMOD = 3, ORC_LIM_FLAT = 4 };
+enum { ORC_HLLC = 0, ORC_ROE = 1 };
+enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW = 1, ORC_BC_REFLECTIVE = 2 };
+
+typedef struct {
+  int    dims;          /* 2 or 3                                        */
+  int    nx[3];         /* interior cells per direction (nx[2]=1 in 2-D) */
+  int    recon;         /* ORC_PLM / ORC_PPM                              */
+  int    limiter;       /* PLM limiter (ignored by PPM)                   */
+  int    solver;        /* ORC_HLLC / ORC_ROE                             */
+  int    rk;            /* 2 or 3                                         */
+  int    bc[6];         /* x1beg, x1end, x2beg, x2end, x3beg, x3end       */
+  double gamma, cfl, cfl_max_var;
+  double xbeg[3], xend[3];
+} orc_cfg;
+
+/* Advance V (interior, in/out) by nsteps time steps exactly as PLUTO's
+ * main loop does (Integrate + NextTimeStep, Src/main.c:215-337).
+ *   *dt   in: dt of the first step; out: dt of the next step
+ *   *t    in/out: simulation time
+ *   U_out optional: Uc after the last step (interior, [nv][k][j][i])
+ *   dt_hist optional: nsteps entries, dt used at each step
+ * Returns 0, or a negative value on bad configuration. */
+int orc_run(const orc_cfg *cfg, double *V, int nsteps, double *dt, double *t,
+            double *U_out, double *dt_hist);
+
+/* Pencil-level entry points (used by tests to pin single functions). */
+int  orc_nghost(const orc_cfg *cfg);
+void orc_prim_to_cons(double gamma, const double *v, double *u);
+int  orc_cons_to_prim(double gamma, double *u, double *v);
+/* One HLLC / Roe interface: vL,vR primitive states (5), dir 0..2.
+ * Outputs flux[5], *press, *cmax. */
+void orc_riemann(int solver, double gamma, int dir, const double *vL,
+                 const double *vR, double *flux, double *press, double *cmax);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
