@@ -85,8 +85,11 @@ def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
                      if l.startswith("PMX_TIMING")]
             return {"wall": walls[-1] - walls[0], "steps_timed": len(walls) - 1}
         out = read_dump(tmp / "dump.bin", cfg.shape)
+        # Analysis() runs at start-up (main.c:145-150), before steps 1..N-1
+        # and after the last step: its records hold the dt of steps
+        # 0..N-1 followed by the next dt.
         dts = np.loadtxt(tmp / "dump.bin.dt", ndmin=2)
-        out["dt_seq"] = dts[:, 2]
+        out["dt_seq"] = dts[:-1, 2]
         return out
 
 

@@ -1,0 +1,464 @@
+// libplutomx: hydro context, step driver and C ABI (see include/plutomx.h).
+#include <hip/hip_runtime.h>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/plutomx.h"
+#include "pmx_common.h"
+#include "pmx_hydro_kernels.h"
+
+namespace pmx {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+constexpr int TILE_X = 32;
+constexpr int TILE_Y = 8;
+
+struct DevScalars {
+  double t;                      // g_time
+  double dt;                     // g_dt of the step being taken
+  unsigned long long cdt_bits;   // max C_dt of stage 1
+  double inv_dt_hyp;             // Dts->invDt_hyp of the last step
+  int err;                       // cons->prim flags of the last step
+  int pad;
+};
+
+// dt/dx for every cell and direction, and reset of the step reductions.
+__global__ void prep_kernel(DevScalars* s, const double* dx, double* dtdx, int ntot,
+                            const double* dt_override) {
+  double dt = dt_override ? *dt_override : s->dt;
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int q = t; q < ntot; q += gridDim.x * blockDim.x) dtdx[q] = dt / dx[q];
+  if (t == 0) {
+    if (dt_override) s->dt = dt;
+    s->cdt_bits = 0ull;
+    s->err = 0;
+  }
+}
+
+// Src/update_stage.c:389-394 then Src/main.c:596-697 (NextTimeStep, no
+// parabolic terms, no cooling) and the time update of main.c:296.
+__global__ void finish_kernel(DevScalars* s, int ndim, double cfl, double cfl_max_var) {
+  double inv = 0.0;
+  double c = __longlong_as_double((long long)s->cdt_bits);
+  inv = dmax(inv, c);
+  inv /= (double)ndim;
+  s->inv_dt_hyp = inv;
+  double dt = s->dt;
+  s->t += dt;
+  double dt_hyp = 1.0 / inv;
+  dt_hyp *= cfl;
+  double dtnext = dt_hyp;
+  dtnext = dmin(dtnext, cfl_max_var * dt);
+  if (cfl_max_var != 1.0) s->dt = dtnext;
+}
+
+}  // namespace pmx
+
+using namespace pmx;
+
+struct pmx_hydro {
+  pmx_hydro_desc d;
+  int nx, ny, nz;
+  long long ncell;
+  double* V[3];
+  double* U;
+  double* dx;       // [nx | ny | nz]
+  double* dtdx;
+  double* inv_dl;
+  DevScalars* s;
+  double* dt_in;    // device slot for an externally imposed dt
+  hipStream_t stream;
+  hipGraphExec_t graph;
+  bool have_graph;
+  bool timing;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
+  double timed_ms;
+  int timed_launches;
+  int cz;
+};
+
+namespace {
+
+template <int DIMS, int REC, int SOLVER, int MODE>
+void launch_stage(pmx_hydro* h, const StageArgs& a, int nblocks) {
+  hipLaunchKernelGGL((stage_kernel<DIMS, REC, SOLVER, MODE, TILE_X, TILE_Y>), dim3(nblocks),
+                     dim3(TILE_X * TILE_Y), 0, h->stream, a);
+}
+
+template <int DIMS, int REC, int SOLVER>
+void dispatch_mode(pmx_hydro* h, const StageArgs& a, int nblocks, int mode) {
+  if (mode == MODE_FIRST) launch_stage<DIMS, REC, SOLVER, MODE_FIRST>(h, a, nblocks);
+  else if (mode == MODE_WEIGHTED) launch_stage<DIMS, REC, SOLVER, MODE_WEIGHTED>(h, a, nblocks);
+  else launch_stage<DIMS, REC, SOLVER, MODE_THIRD>(h, a, nblocks);
+}
+
+template <int DIMS, int REC>
+void dispatch_solver(pmx_hydro* h, const StageArgs& a, int nblocks, int mode) {
+  if (h->d.solver == PMX_SOLVER_ROE) dispatch_mode<DIMS, REC, SOL_ROE>(h, a, nblocks, mode);
+  else dispatch_mode<DIMS, REC, SOL_HLLC>(h, a, nblocks, mode);
+}
+
+template <int DIMS>
+void dispatch_rec(pmx_hydro* h, const StageArgs& a, int nblocks, int mode) {
+  switch (h->d.recon) {
+    case PMX_REC_PLM_VANLEER: dispatch_solver<DIMS, REC_PLM_VL>(h, a, nblocks, mode); break;
+    case PMX_REC_PLM_MC: dispatch_solver<DIMS, REC_PLM_MC>(h, a, nblocks, mode); break;
+    case PMX_REC_PLM_MINMOD: dispatch_solver<DIMS, REC_PLM_MM>(h, a, nblocks, mode); break;
+    case PMX_REC_PLM_DEFAULT: dispatch_solver<DIMS, REC_PLM_DEFAULT>(h, a, nblocks, mode); break;
+    default: dispatch_solver<DIMS, REC_PPM>(h, a, nblocks, mode); break;
+  }
+}
+
+int stage(pmx_hydro* h, const double* Vin, const double* V0, double* Vout, int mode, double w0,
+          double wc, int write_u) {
+  StageArgs a;
+  a.Vin = Vin;
+  a.V0 = V0;
+  a.U = h->U;
+  a.Vout = Vout;
+  a.nx = h->nx; a.ny = h->ny; a.nz = h->nz;
+  a.sv = h->ncell;
+  for (int q = 0; q < 6; q++) a.bc[q] = h->d.bc[q];
+  a.gamma = h->d.gamma;
+  a.gmm1 = h->d.gamma - 1.0;
+  a.dtdx[0] = h->dtdx; a.dtdx[1] = h->dtdx + h->nx; a.dtdx[2] = h->dtdx + h->nx + h->ny;
+  a.inv_dl[0] = h->inv_dl; a.inv_dl[1] = h->inv_dl + h->nx; a.inv_dl[2] = h->inv_dl + h->nx + h->ny;
+  a.w0 = w0; a.wc = wc;
+  a.cz = h->cz;
+  a.tiles_x = (h->nx + TILE_X - 1) / TILE_X;
+  a.tiles_y = (h->ny + TILE_Y - 1) / TILE_Y;
+  a.tiles_z = (h->d.dims == 3) ? (h->nz + h->cz - 1) / h->cz : 1;
+  a.write_u = write_u;
+  a.cdt_bits = &h->s->cdt_bits;
+  a.err = &h->s->err;
+  int nblocks = a.tiles_x * a.tiles_y * a.tiles_z;
+
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (h->timing) {
+    if (h->ev_pool.size() < 2) {
+      for (int q = 0; q < 64; q++) {
+        hipEvent_t e;
+        PMX_HIP_CHECK(hipEventCreate(&e));
+        h->ev_pool.push_back(e);
+      }
+    }
+    e0 = h->ev_pool.back(); h->ev_pool.pop_back();
+    e1 = h->ev_pool.back(); h->ev_pool.pop_back();
+    PMX_HIP_CHECK(hipEventRecord(e0, h->stream));
+  }
+  if (h->d.dims == 3) dispatch_rec<3>(h, a, nblocks, mode);
+  else dispatch_rec<2>(h, a, nblocks, mode);
+  PMX_HIP_CHECK(hipGetLastError());
+  if (h->timing) {
+    PMX_HIP_CHECK(hipEventRecord(e1, h->stream));
+    h->ev_pending.push_back({e0, e1});
+  }
+  return 0;
+}
+
+// One AdvanceStep: Src/Time_Stepping/rk_step.c:117-330.  V[0] holds V^n on
+// entry and V^{n+1} on exit.
+int advance(pmx_hydro* h, const double* dt_override) {
+  int ntot = h->nx + h->ny + h->nz;
+  hipLaunchKernelGGL(prep_kernel, dim3((ntot + 255) / 256), dim3(256), 0, h->stream, h->s,
+                     h->dx, h->dtdx, ntot, dt_override);
+  PMX_HIP_CHECK(hipGetLastError());
+  if (h->d.rk == 3) {
+    if (stage(h, h->V[0], h->V[0], h->V[1], MODE_FIRST, 0, 0, 1)) return -1;
+    if (stage(h, h->V[1], h->V[0], h->V[2], MODE_WEIGHTED, 0.75, 0.25, 1)) return -1;
+    if (stage(h, h->V[2], h->V[0], h->V[0], MODE_THIRD, 0, 0, 1)) return -1;
+  } else {
+    if (stage(h, h->V[0], h->V[0], h->V[1], MODE_FIRST, 0, 0, 1)) return -1;
+    if (stage(h, h->V[1], h->V[0], h->V[0], MODE_WEIGHTED, 0.5, 0.5, 1)) return -1;
+  }
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, h->stream, h->s, h->d.dims,
+                     h->d.cfl, h->d.cfl_max_var);
+  PMX_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int collect_timing(pmx_hydro* h) {
+  for (auto& p : h->ev_pending) {
+    float ms = 0.f;
+    PMX_HIP_CHECK(hipEventSynchronize(p.second));
+    PMX_HIP_CHECK(hipEventElapsedTime(&ms, p.first, p.second));
+    h->timed_ms += ms;
+    h->timed_launches++;
+    h->ev_pool.push_back(p.first);
+    h->ev_pool.push_back(p.second);
+  }
+  h->ev_pending.clear();
+  return 0;
+}
+
+// strided copy helpers between PLUTO's ghosted Vc and the interior layout
+void copy_interior(const pmx_hydro* h, const double* src, double* dst, bool src_ghost, int ng) {
+  int gx = ng, gy = ng, gz = (h->d.dims == 3) ? ng : 0;
+  long long tx = h->nx + 2 * gx, ty = h->ny + 2 * gy, tz = h->nz + 2 * gz;
+  for (int nv = 0; nv < NV; nv++)
+    for (int k = 0; k < h->nz; k++)
+      for (int j = 0; j < h->ny; j++) {
+        long long oi = ((long long)nv * h->nz + k) * h->ny * h->nx + (long long)j * h->nx;
+        long long og = ((nv * tz + k + gz) * ty + j + gy) * tx + gx;
+        if (src_ghost) memcpy(dst + oi, src + og, sizeof(double) * h->nx);
+        else memcpy(dst + og, src + oi, sizeof(double) * h->nx);
+      }
+}
+
+int host_map(int g, int n, int bl, int br, bool& flip) {
+  if (g < 0) {
+    if (bl == BC_PERIODIC) g += n;
+    else if (bl == BC_OUTFLOW) g = 0;
+    else { g = -1 - g; flip = !flip; }
+  } else if (g >= n) {
+    if (br == BC_PERIODIC) g -= n;
+    else if (br == BC_OUTFLOW) g = n - 1;
+    else { g = 2 * n - 1 - g; flip = !flip; }
+  }
+  return g;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* pmx_last_error(void) { return g_err.c_str(); }
+
+int pmx_hydro_create(const pmx_hydro_desc* desc, pmx_hydro** out) {
+  if (!desc || !out) { set_error("null argument"); return -1; }
+  if (desc->dims != 2 && desc->dims != 3) { set_error("dims must be 2 or 3"); return -1; }
+  if (desc->rk != 2 && desc->rk != 3) { set_error("rk must be 2 or 3"); return -1; }
+  if (desc->recon != PMX_REC_PLM_VANLEER && desc->recon != PMX_REC_PLM_MC &&
+      desc->recon != PMX_REC_PLM_MINMOD && desc->recon != PMX_REC_PLM_DEFAULT &&
+      desc->recon != PMX_REC_PPM) {
+    set_error("unknown reconstruction"); return -1;
+  }
+  if (desc->solver != PMX_SOLVER_HLLC && desc->solver != PMX_SOLVER_ROE) {
+    set_error("unknown Riemann solver"); return -1;
+  }
+  for (int q = 0; q < 6; q++)
+    if (desc->bc[q] < 0 || desc->bc[q] > 2) { set_error("unknown boundary type"); return -1; }
+  int nz = desc->dims == 3 ? desc->nx3 : 1;
+  int H = desc->recon == PMX_REC_PPM ? 3 : 2;
+  if (desc->nx1 < H || desc->nx2 < H || (desc->dims == 3 && desc->nx3 < H)) {
+    set_error("grid smaller than the ghost width"); return -1;
+  }
+  if (!desc->dx1 || !desc->dx2 || (desc->dims == 3 && !desc->dx3)) {
+    set_error("missing cell widths"); return -1;
+  }
+  pmx_hydro* h = new pmx_hydro();
+  h->d = *desc;
+  h->nx = desc->nx1; h->ny = desc->nx2; h->nz = nz;
+  h->ncell = (long long)h->nx * h->ny * h->nz;
+  h->cz = 16;
+  if (h->nz < 64) h->cz = 8;
+  size_t vbytes = sizeof(double) * NV * h->ncell;
+  int ntot = h->nx + h->ny + h->nz;
+#define CK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { \
+    set_error(std::string(#x) + ": " + hipGetErrorString(e_)); pmx_hydro_destroy(h); return -1; } } while (0)
+  CK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  for (int q = 0; q < 3; q++) CK(hipMalloc(&h->V[q], vbytes));
+  CK(hipMalloc(&h->U, vbytes));
+  CK(hipMalloc(&h->dx, sizeof(double) * ntot));
+  CK(hipMalloc(&h->dtdx, sizeof(double) * ntot));
+  CK(hipMalloc(&h->inv_dl, sizeof(double) * ntot));
+  CK(hipMalloc(&h->s, sizeof(DevScalars)));
+  CK(hipMalloc(&h->dt_in, sizeof(double)));
+  {
+    std::vector<double> dx(ntot), inv(ntot);
+    for (int i = 0; i < h->nx; i++) dx[i] = desc->dx1[i];
+    for (int j = 0; j < h->ny; j++) dx[h->nx + j] = desc->dx2[j];
+    for (int k = 0; k < h->nz; k++) dx[h->nx + h->ny + k] = desc->dims == 3 ? desc->dx3[k] : 1.0;
+    for (int q = 0; q < ntot; q++) inv[q] = 1.0 / dx[q];   // Src/set_geometry.c:305
+    CK(hipMemcpy(h->dx, dx.data(), sizeof(double) * ntot, hipMemcpyHostToDevice));
+    CK(hipMemcpy(h->inv_dl, inv.data(), sizeof(double) * ntot, hipMemcpyHostToDevice));
+    DevScalars s0;
+    memset(&s0, 0, sizeof(s0));
+    CK(hipMemcpy(h->s, &s0, sizeof(s0), hipMemcpyHostToDevice));
+  }
+  for (int q = 0; q < 3; q++) CK(hipMemset(h->V[q], 0, vbytes));
+  CK(hipMemset(h->U, 0, vbytes));
+#undef CK
+  *out = h;
+  return 0;
+}
+
+int pmx_hydro_destroy(pmx_hydro* h) {
+  if (!h) return 0;
+  if (h->stream) hipStreamSynchronize(h->stream);
+  if (h->have_graph) hipGraphExecDestroy(h->graph);
+  for (auto& p : h->ev_pending) { h->ev_pool.push_back(p.first); h->ev_pool.push_back(p.second); }
+  for (auto e : h->ev_pool) hipEventDestroy(e);
+  for (int q = 0; q < 3; q++) if (h->V[q]) hipFree(h->V[q]);
+  if (h->U) hipFree(h->U);
+  if (h->dx) hipFree(h->dx);
+  if (h->dtdx) hipFree(h->dtdx);
+  if (h->inv_dl) hipFree(h->inv_dl);
+  if (h->s) hipFree(h->s);
+  if (h->dt_in) hipFree(h->dt_in);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return 0;
+}
+
+int pmx_hydro_set_prim(pmx_hydro* h, const double* V, int ghosts) {
+  if (!h || !V) { set_error("null argument"); return -1; }
+  size_t n = (size_t)NV * h->ncell;
+  if (ghosts) {
+    std::vector<double> tmp(n);
+    copy_interior(h, V, tmp.data(), true, h->d.recon == PMX_REC_PPM ? 3 : 2);
+    PMX_HIP_CHECK(hipMemcpyAsync(h->V[0], tmp.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                                 h->stream));
+    PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
+  } else {
+    PMX_HIP_CHECK(hipMemcpyAsync(h->V[0], V, n * sizeof(double), hipMemcpyHostToDevice,
+                                 h->stream));
+    PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
+  }
+  return 0;
+}
+
+int pmx_hydro_get_prim(pmx_hydro* h, double* V, int ghosts) {
+  if (!h || !V) { set_error("null argument"); return -1; }
+  size_t n = (size_t)NV * h->ncell;
+  if (!ghosts) {
+    PMX_HIP_CHECK(hipMemcpyAsync(V, h->V[0], n * sizeof(double), hipMemcpyDeviceToHost,
+                                 h->stream));
+    PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
+    return 0;
+  }
+  std::vector<double> tmp(n);
+  PMX_HIP_CHECK(hipMemcpyAsync(tmp.data(), h->V[0], n * sizeof(double), hipMemcpyDeviceToHost,
+                               h->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
+  int ng = h->d.recon == PMX_REC_PPM ? 3 : 2;
+  int gz = h->d.dims == 3 ? ng : 0;
+  long long tx = h->nx + 2 * ng, ty = h->ny + 2 * ng, tz = h->nz + 2 * gz;
+  for (int nv = 0; nv < NV; nv++)
+    for (long long k = 0; k < tz; k++)
+      for (long long j = 0; j < ty; j++)
+        for (long long i = 0; i < tx; i++) {
+          bool flip = false;
+          int ii = host_map((int)i - ng, h->nx, h->d.bc[0], h->d.bc[1], flip);
+          int jj = host_map((int)j - ng, h->ny, h->d.bc[2], h->d.bc[3], flip);
+          bool fx = false, fy = false, fz = false;
+          ii = host_map((int)i - ng, h->nx, h->d.bc[0], h->d.bc[1], fx);
+          jj = host_map((int)j - ng, h->ny, h->d.bc[2], h->d.bc[3], fy);
+          int kk = h->d.dims == 3 ? host_map((int)k - gz, h->nz, h->d.bc[4], h->d.bc[5], fz) : 0;
+          double val = tmp[(((long long)nv * h->nz + kk) * h->ny + jj) * h->nx + ii];
+          if ((nv == 1 && fx) || (nv == 2 && fy) || (nv == 3 && fz)) val = -val;
+          V[((nv * tz + k) * ty + j) * tx + i] = val;
+          (void)flip;
+        }
+  return 0;
+}
+
+int pmx_hydro_get_cons(pmx_hydro* h, double* U, int pluto_layout) {
+  if (!h || !U) { set_error("null argument"); return -1; }
+  size_t n = (size_t)NV * h->ncell;
+  if (!pluto_layout) {
+    PMX_HIP_CHECK(hipMemcpyAsync(U, h->U, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
+    return 0;
+  }
+  std::vector<double> tmp(n);
+  PMX_HIP_CHECK(hipMemcpyAsync(tmp.data(), h->U, n * sizeof(double), hipMemcpyDeviceToHost,
+                               h->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
+  for (long long c = 0; c < h->ncell; c++)
+    for (int nv = 0; nv < NV; nv++) U[c * NV + nv] = tmp[nv * h->ncell + c];
+  return 0;
+}
+
+int pmx_hydro_set_time(pmx_hydro* h, double t, double dt) {
+  if (!h) { set_error("null argument"); return -1; }
+  DevScalars s0;
+  PMX_HIP_CHECK(hipMemcpyAsync(&s0, h->s, sizeof(s0), hipMemcpyDeviceToHost, h->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
+  s0.t = t;
+  s0.dt = dt;
+  PMX_HIP_CHECK(hipMemcpyAsync(h->s, &s0, sizeof(s0), hipMemcpyHostToDevice, h->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int pmx_hydro_get_time(pmx_hydro* h, double* t, double* dt, double* inv_dt_hyp, int* flags) {
+  if (!h) { set_error("null argument"); return -1; }
+  DevScalars s0;
+  PMX_HIP_CHECK(hipMemcpyAsync(&s0, h->s, sizeof(s0), hipMemcpyDeviceToHost, h->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
+  if (t) *t = s0.t;
+  if (dt) *dt = s0.dt;
+  if (inv_dt_hyp) *inv_dt_hyp = s0.inv_dt_hyp;
+  if (flags) *flags = s0.err;
+  return 0;
+}
+
+int pmx_hydro_advance_step(pmx_hydro* h, double dt, double* inv_dt_hyp) {
+  if (!h) { set_error("null argument"); return -1; }
+  PMX_HIP_CHECK(hipMemcpyAsync(h->dt_in, &dt, sizeof(double), hipMemcpyHostToDevice, h->stream));
+  if (advance(h, h->dt_in)) return -1;
+  if (h->timing) collect_timing(h);
+  DevScalars s0;
+  PMX_HIP_CHECK(hipMemcpyAsync(&s0, h->s, sizeof(s0), hipMemcpyDeviceToHost, h->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
+  if (inv_dt_hyp) *inv_dt_hyp = s0.inv_dt_hyp;
+  return 0;
+}
+
+int pmx_hydro_run(pmx_hydro* h, int nsteps) {
+  if (!h || nsteps < 0) { set_error("bad argument"); return -1; }
+  for (int n = 0; n < nsteps; n++)
+    if (advance(h, nullptr)) return -1;
+  return 0;
+}
+
+int pmx_hydro_run_graph(pmx_hydro* h, int nsteps) {
+  if (!h || nsteps < 0) { set_error("bad argument"); return -1; }
+  if (h->timing) return pmx_hydro_run(h, nsteps);
+  if (!h->have_graph) {
+    hipGraph_t g;
+    PMX_HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    int rc = advance(h, nullptr);
+    hipError_t e = hipStreamEndCapture(h->stream, &g);
+    if (rc) return -1;
+    if (e != hipSuccess) { set_error(std::string("capture: ") + hipGetErrorString(e)); return -1; }
+    PMX_HIP_CHECK(hipGraphInstantiate(&h->graph, g, nullptr, nullptr, 0));
+    PMX_HIP_CHECK(hipGraphDestroy(g));
+    h->have_graph = true;
+  }
+  for (int n = 0; n < nsteps; n++) PMX_HIP_CHECK(hipGraphLaunch(h->graph, h->stream));
+  return 0;
+}
+
+int pmx_hydro_sync(pmx_hydro* h) {
+  if (!h) { set_error("null argument"); return -1; }
+  PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
+  if (h->timing) collect_timing(h);
+  return 0;
+}
+
+int pmx_hydro_stage_timing(pmx_hydro* h, int enable, int reset, double* total_ms, int* launches) {
+  if (!h) { set_error("null argument"); return -1; }
+  PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
+  collect_timing(h);
+  if (total_ms) *total_ms = h->timed_ms;
+  if (launches) *launches = h->timed_launches;
+  if (reset) { h->timed_ms = 0.0; h->timed_launches = 0; }
+  h->timing = enable != 0;
+  return 0;
+}
+
+int pmx_hydro_device_state(pmx_hydro* h, double** V, double** U) {
+  if (!h) { set_error("null argument"); return -1; }
+  if (V) *V = h->V[0];
+  if (U) *U = h->U;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,397 @@
+// Fused method-of-lines stage kernel for HD on uniform Cartesian grids.
+//
+// One launch performs one RK stage of the reference's UpdateStage
+// (Src/Time_Stepping/update_stage.c:35-410) for every direction at once,
+// followed by the RK combination (rk_step.c:214-237, 281-305) and the
+// cons->prim map (mappers3D.c:16-141):
+//
+//   U_s = U_in + rhs_x(V) + rhs_y(V) + rhs_z(V)        (added in that order)
+//   U   = combine(U0, U_s);   V_out = C2P(U)            (U floored in place)
+//
+// Work decomposition (2.5-D): a workgroup owns a TX x TY tile of the x-y
+// plane and marches along z over a chunk of CZ planes.
+//   * z: each thread keeps a register window of its column, the PPM
+//     interface value and left state of the previous cell, and the flux of
+//     the previous z-face, so every z-face and every z-state is computed
+//     exactly once.
+//   * x, y: the plane (with H ghost cells in x and y) is staged in LDS;
+//     per-cell left/right states are computed once into LDS, then the
+//     face fluxes, then the update.  Only the one-cell ring around the
+//     tile is computed twice (by the neighbouring tile).
+// Ghost cells are never stored: halo reads map out-of-domain indices with
+// the boundary rule (map_bc), so no separate boundary kernel runs.
+//
+// Layout in HBM: V, U are SoA [nv][nz][ny][nx] float64, interior only.
+#pragma once
+#include "pmx_common.h"
+#include "pmx_hd_math.h"
+
+namespace pmx {
+
+enum { MODE_FIRST = 0, MODE_WEIGHTED = 1, MODE_THIRD = 2 };
+
+struct StageArgs {
+  const double* Vin;   // stage input, halo source
+  const double* V0;    // V at the start of the step (U0 = P2C(V0)); may alias Vout
+  double* U;           // U of the previous stage (in) / this stage (out)
+  double* Vout;        // V after this stage
+  int nx, ny, nz;
+  long long sv;        // stride between variables = nx*ny*nz
+  int bc[6];
+  double gamma, gmm1;
+  const double* dtdx[3];    // dt/dx per cell and direction (this step)
+  const double* inv_dl[3];  // 1/dx per cell and direction
+  double w0, wc;            // RK weights for MODE_WEIGHTED
+  int cz;                   // z planes per workgroup
+  int tiles_x, tiles_y, tiles_z;
+  int write_u;              // store U (always needed except after the last stage)
+  unsigned long long* cdt_bits;  // MODE_FIRST: max over cells of C_dt (bit pattern)
+  int* err;                      // OR of cons->prim floor flags
+};
+
+template <int REC> struct RecTraits {
+  static constexpr int H = (REC == REC_PPM) ? 3 : 2;  // ghost width, Src/get_nghost.c:19-42
+};
+
+// Load the 5 primitive variables of global cell (i, j, k) with boundary
+// mapping; flip the sign of the normal velocity of a reflected direction.
+template <int DIMS>
+__device__ __forceinline__ void load_cell(const StageArgs& a, const double* __restrict__ V,
+                                          int i, int j, int k, double* q) {
+  bool fx = false, fy = false, fz = false;
+  int ii = map_bc(i, a.nx, a.bc[0], a.bc[1], fx);
+  int jj = map_bc(j, a.ny, a.bc[2], a.bc[3], fy);
+  int kk = (DIMS == 3) ? map_bc(k, a.nz, a.bc[4], a.bc[5], fz) : 0;
+  long long o = ((long long)kk * a.ny + jj) * a.nx + ii;
+#pragma unroll
+  for (int nv = 0; nv < NV; nv++) q[nv] = V[o + nv * a.sv];
+  if (fx) q[VX1] = -q[VX1];
+  if (fy) q[VX2] = -q[VX2];
+  if (fz) q[VX3] = -q[VX3];
+}
+
+// Per-cell states from a 5-point (PPM) or 3-point (PLM) stencil s[0..4]
+// centred at s[2]; vp = right-face state, vm = left-face state.
+template <int REC>
+__device__ __forceinline__ void cell_states(const double (*s)[NV], double* vp, double* vm) {
+  if (REC == REC_PPM) {
+#pragma unroll
+    for (int nv = 0; nv < NV; nv++) {
+      double Ip = ppm_iface(s[1][nv], s[2][nv], s[3][nv], s[4][nv]);
+      double Im = ppm_iface(s[0][nv], s[1][nv], s[2][nv], s[3][nv]);
+      ppm_limit(s[2][nv], Ip, Im, vp[nv], vm[nv]);
+    }
+  } else {
+    plm_states<REC>(s[1], s[2], s[3], vp, vm);
+  }
+}
+
+template <int DIMS, int REC, int SOLVER, int MODE, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
+  constexpr int H = RecTraits<REC>::H;
+  constexpr int NT = TX * TY;
+  constexpr int LW = TX + 2 * H;
+  constexpr int LH = TY + 2 * H;
+  constexpr int NSX = (TX + 2) * TY;   // x-states per tile plane
+  constexpr int NSY = TX * (TY + 2);   // y-states
+  constexpr int NFX = (TX + 1) * TY;   // x-faces
+  constexpr int NFY = TX * (TY + 1);   // y-faces
+  constexpr int NST = NSX > NSY ? NSX : NSY;
+  constexpr int NPL = NV * LH * LW;
+  constexpr int NFYD = 7 * NFY;
+  constexpr int NPLANE = NPL > NFYD ? NPL : NFYD;
+
+  __shared__ double sPlane[NPLANE];    // V plane [nv][LH][LW]; later y-fluxes [NFY][7]
+  __shared__ double sSt[NST * 10];     // states [item][vp(5) vm(5)]
+  __shared__ double sFx[NFX * 7];      // x-fluxes [face][flux(5) press cmax]
+  __shared__ double sRed[NT / 64];
+
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  const int nblocks = a.tiles_x * a.tiles_y * a.tiles_z;
+  const int lb = xcd_remap(blockIdx.x, nblocks);
+  const int bx = lb % a.tiles_x;
+  const int by = (lb / a.tiles_x) % a.tiles_y;
+  const int bz = lb / (a.tiles_x * a.tiles_y);
+  const int i0 = bx * TX, j0 = by * TY;
+  const int i = i0 + tx, j = j0 + ty;
+  const bool valid = (i < a.nx) && (j < a.ny);
+  const int k0 = (DIMS == 3) ? bz * a.cz : 0;
+  const int k1 = (DIMS == 3) ? min(k0 + a.cz, a.nz) : 1;
+  const double gamma = a.gamma, gmm1 = a.gmm1;
+
+  double cdt_local = 0.0, cdt_max = 0.0;
+  int fail = 0;
+
+  // ---- z-direction register state -----------------------------------
+  // PPM window w[0..3] = v[k..k+3]; PLM window w[0..2] = v[k..k+2]
+  constexpr int WZ = (REC == REC_PPM) ? 4 : 3;
+  double w[WZ][NV];
+  double Iz[NV];          // PPM interface value at k+1/2
+  double vpz[NV];         // right-face state of cell k
+  double Fz[7];           // flux at k-1/2 (5), press, cmax
+
+  if (DIMS == 3) {
+    // prologue: states of cells k0-1 and k0, flux at k0-1/2
+    if (REC == REC_PPM) {
+      double p[6][NV];  // v[k0-3 .. k0+2]
+#pragma unroll
+      for (int s = 0; s < 6; s++) load_cell<DIMS>(a, a.Vin, i, j, k0 - 3 + s, p[s]);
+      double Ia[NV], Ib[NV], vm0[NV], vpm[NV], vmm[NV];
+#pragma unroll
+      for (int nv = 0; nv < NV; nv++) {
+        Ia[nv] = ppm_iface(p[0][nv], p[1][nv], p[2][nv], p[3][nv]);  // I(k0-3/2)
+        Ib[nv] = ppm_iface(p[1][nv], p[2][nv], p[3][nv], p[4][nv]);  // I(k0-1/2)
+        Iz[nv] = ppm_iface(p[2][nv], p[3][nv], p[4][nv], p[5][nv]);  // I(k0+1/2)
+        ppm_limit(p[2][nv], Ib[nv], Ia[nv], vpm[nv], vmm[nv]);       // cell k0-1
+        ppm_limit(p[3][nv], Iz[nv], Ib[nv], vpz[nv], vm0[nv]);       // cell k0
+      }
+      riemann<SOLVER, 2, DIMS>(vpm, vm0, gamma, gmm1, Fz, Fz[5], Fz[6]);
+#pragma unroll
+      for (int s = 0; s < 3; s++)
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) w[s][nv] = p[s + 3][nv];  // v[k0..k0+2]
+    } else {
+      double p[4][NV];  // v[k0-2 .. k0+1]
+#pragma unroll
+      for (int s = 0; s < 4; s++) load_cell<DIMS>(a, a.Vin, i, j, k0 - 2 + s, p[s]);
+      double vpm[NV], vmm[NV], vm0[NV];
+      plm_states<REC>(p[0], p[1], p[2], vpm, vmm);   // cell k0-1
+      plm_states<REC>(p[1], p[2], p[3], vpz, vm0);   // cell k0
+      riemann<SOLVER, 2, DIMS>(vpm, vm0, gamma, gmm1, Fz, Fz[5], Fz[6]);
+#pragma unroll
+      for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) w[s][nv] = p[s + 2][nv];  // v[k0..k0+1]
+    }
+  }
+
+  for (int k = k0; k < k1; k++) {
+    double vc[NV];  // own cell value at plane k
+    if (DIMS == 3) {
+      load_cell<DIMS>(a, a.Vin, i, j, k + WZ - 1, w[WZ - 1]);
+#pragma unroll
+      for (int nv = 0; nv < NV; nv++) vc[nv] = w[0][nv];
+    } else {
+      load_cell<DIMS>(a, a.Vin, i, j, 0, vc);
+    }
+
+    // ---- P0: stage the plane with its x/y halo in LDS ----------------
+#pragma unroll
+    for (int nv = 0; nv < NV; nv++) sPlane[(nv * LH + H + ty) * LW + H + tx] = vc[nv];
+    {
+      // x-halo: H columns left/right of each tile row; y-halo: H rows
+      // below/above each tile column.  (r, c) are LDS coordinates.
+      constexpr int NHX = 2 * H * TY, NHY = 2 * H * TX;
+      for (int it = tid; it < NHX + NHY; it += NT) {
+        int r, c;
+        if (it < NHX) {
+          int cc = it % (2 * H);
+          r = H + it / (2 * H);
+          c = cc < H ? cc : TX + cc;
+        } else {
+          int jt = it - NHX;
+          int rr = jt / TX;
+          c = H + jt % TX;
+          r = rr < H ? rr : TY + rr;
+        }
+        double q[NV];
+        load_cell<DIMS>(a, a.Vin, i0 + c - H, j0 + r - H, k, q);
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) sPlane[(nv * LH + r) * LW + c] = q[nv];
+      }
+    }
+    __syncthreads();
+
+    // ---- P1: x-states of cells -1 .. TX (per row) ----------------------
+    for (int it = tid; it < NSX; it += NT) {
+      int r = it / (TX + 2), c = it % (TX + 2) - 1;
+      double s[5][NV];
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        int cc = H + c - 2 + q;
+        cc = cc < 0 ? 0 : (cc >= LW ? LW - 1 : cc);  // PLM (H=2) never reads the outer pair
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) s[q][nv] = sPlane[(nv * LH + H + r) * LW + cc];
+      }
+      double vp[NV], vm[NV];
+      cell_states<REC>(s, vp, vm);
+#pragma unroll
+      for (int nv = 0; nv < NV; nv++) {
+        sSt[it * 10 + nv] = vp[nv];
+        sSt[it * 10 + 5 + nv] = vm[nv];
+      }
+    }
+    __syncthreads();
+
+    // ---- P2: x-faces ----------------------------------------------------
+    for (int f = tid; f < NFX; f += NT) {
+      int r = f / (TX + 1), fi = f % (TX + 1);  // face between cells fi-1 and fi
+      int sl = r * (TX + 2) + fi, sr = sl + 1;
+      double vL[NV], vR[NV], F[7];
+#pragma unroll
+      for (int nv = 0; nv < NV; nv++) {
+        vL[nv] = sSt[sl * 10 + nv];
+        vR[nv] = sSt[sr * 10 + 5 + nv];
+      }
+      riemann<SOLVER, 0, DIMS>(vL, vR, gamma, gmm1, F, F[5], F[6]);
+#pragma unroll
+      for (int q = 0; q < 7; q++) sFx[f * 7 + q] = F[q];
+    }
+    __syncthreads();
+
+    // ---- P3: y-states of rows -1 .. TY (per column) ---------------------
+    for (int it = tid; it < NSY; it += NT) {
+      int c = it % TX, r = it / TX - 1;
+      double s[5][NV];
+#pragma unroll
+      for (int q = 0; q < 5; q++) {
+        int rr = H + r - 2 + q;
+        rr = rr < 0 ? 0 : (rr >= LH ? LH - 1 : rr);
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) s[q][nv] = sPlane[(nv * LH + rr) * LW + H + c];
+      }
+      double vp[NV], vm[NV];
+      cell_states<REC>(s, vp, vm);
+#pragma unroll
+      for (int nv = 0; nv < NV; nv++) {
+        sSt[it * 10 + nv] = vp[nv];
+        sSt[it * 10 + 5 + nv] = vm[nv];
+      }
+    }
+    __syncthreads();
+
+    // ---- P4: y-faces (fluxes overwrite the plane buffer) ---------------
+    for (int f = tid; f < NFY; f += NT) {
+      int c = f % TX, fj = f / TX;  // face between rows fj-1 and fj
+      int sl = fj * TX + c, sr = sl + TX;
+      double vL[NV], vR[NV], F[7];
+#pragma unroll
+      for (int nv = 0; nv < NV; nv++) {
+        vL[nv] = sSt[sl * 10 + nv];
+        vR[nv] = sSt[sr * 10 + 5 + nv];
+      }
+      riemann<SOLVER, 1, DIMS>(vL, vR, gamma, gmm1, F, F[5], F[6]);
+#pragma unroll
+      for (int q = 0; q < 7; q++) sPlane[f * 7 + q] = F[q];
+    }
+    __syncthreads();
+
+    // ---- P5: z-face, update, RK combination, cons->prim ---------------
+    double Fzn[7];
+    if (DIMS == 3) {
+      double vmn[NV], vpn[NV];
+      if (REC == REC_PPM) {
+        double In[NV];
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) {
+          In[nv] = ppm_iface(w[0][nv], w[1][nv], w[2][nv], w[3][nv]);  // I(k+3/2)
+          ppm_limit(w[1][nv], In[nv], Iz[nv], vpn[nv], vmn[nv]);      // cell k+1
+          Iz[nv] = In[nv];
+        }
+      } else {
+        plm_states<REC>(w[0], w[1], w[2], vpn, vmn);                  // cell k+1
+      }
+      riemann<SOLVER, 2, DIMS>(vpz, vmn, gamma, gmm1, Fzn, Fzn[5], Fzn[6]);
+#pragma unroll
+      for (int nv = 0; nv < NV; nv++) vpz[nv] = vpn[nv];
+    }
+
+    if (valid) {
+      const long long o = ((long long)k * a.ny + j) * a.nx + i;
+      double u[NV];
+      if (MODE == MODE_FIRST) {
+        prim_to_cons(vc, u, gmm1);
+      } else {
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) u[nv] = a.U[o + nv * a.sv];
+      }
+      // x
+      {
+        const double* Fm = &sFx[(ty * (TX + 1) + tx) * 7];
+        const double* Fp = Fm + 7;
+        double scrh = a.dtdx[0][i];
+        double rhs[NV];
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) rhs[nv] = -scrh * (Fp[nv] - Fm[nv]);
+        rhs[VX1] -= scrh * (Fp[5] - Fm[5]);
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) u[nv] += rhs[nv];
+        if (MODE == MODE_FIRST) cdt_local = 0.0 + 0.5 * (Fm[6] + Fp[6]) * a.inv_dl[0][i] * 1.0;
+      }
+      // y
+      {
+        const double* Fm = &sPlane[(ty * TX + tx) * 7];
+        const double* Fp = Fm + 7 * TX;
+        double scrh = a.dtdx[1][j];
+        double rhs[NV];
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) rhs[nv] = -scrh * (Fp[nv] - Fm[nv]);
+        rhs[VX2] -= scrh * (Fp[5] - Fm[5]);
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) u[nv] += rhs[nv];
+        if (MODE == MODE_FIRST) cdt_local += 0.5 * (Fm[6] + Fp[6]) * a.inv_dl[1][j] * 1.0;
+      }
+      // z
+      if (DIMS == 3) {
+        double scrh = a.dtdx[2][k];
+        double rhs[NV];
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) rhs[nv] = -scrh * (Fzn[nv] - Fz[nv]);
+        rhs[VX3] -= scrh * (Fzn[5] - Fz[5]);
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) u[nv] += rhs[nv];
+        if (MODE == MODE_FIRST) cdt_local += 0.5 * (Fz[6] + Fzn[6]) * a.inv_dl[2][k] * 1.0;
+      }
+      if (MODE != MODE_FIRST) {
+        double v0[NV], u0[NV];
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) v0[nv] = a.V0[o + nv * a.sv];
+        prim_to_cons(v0, u0, gmm1);
+        if (MODE == MODE_WEIGHTED) {
+#pragma unroll
+          for (int nv = 0; nv < NV; nv++) u[nv] = a.w0 * u0[nv] + a.wc * u[nv];
+        } else {
+          const double one_third = 1.0 / 3.0;
+#pragma unroll
+          for (int nv = 0; nv < NV; nv++) u[nv] = one_third * (u0[nv] + 2.0 * u[nv]);
+        }
+      }
+      double vn[NV];
+      fail |= cons_to_prim(u, vn, gmm1);
+#pragma unroll
+      for (int nv = 0; nv < NV; nv++) a.Vout[o + nv * a.sv] = vn[nv];
+      if (a.write_u) {
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) a.U[o + nv * a.sv] = u[nv];
+      }
+      if (MODE == MODE_FIRST) cdt_max = dmax(cdt_max, cdt_local);
+    }
+    if (DIMS == 3) {
+#pragma unroll
+      for (int q = 0; q < 7; q++) Fz[q] = Fzn[q];
+#pragma unroll
+      for (int s = 0; s < WZ - 1; s++)
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) w[s][nv] = w[s + 1][nv];
+    }
+    __syncthreads();
+  }
+
+  // ---- block reductions: max C_dt (stage 1) and cons->prim flags -------
+  if (MODE == MODE_FIRST) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) cdt_max = dmax(cdt_max, __shfl_xor(cdt_max, off));
+    if ((tid & 63) == 0) sRed[tid >> 6] = cdt_max;
+    __syncthreads();
+    if (tid == 0) {
+      double m = sRed[0];
+      for (int q = 1; q < NT / 64; q++) m = dmax(m, sRed[q]);
+      // non-negative doubles order like their bit patterns
+      atomicMax(a.cdt_bits, (unsigned long long)__double_as_longlong(m));
+    }
+  }
+  if (fail) atomicOr(a.err, fail);
+}
+
+}  // namespace pmx

@@ -1,0 +1,191 @@
+"""Host-side Python binding of libplutomx (ctypes over include/plutomx.h).
+
+This mirrors the reference's driver interface for the hot path:
+``Hydro.advance_step(dt)`` is PLUTO's ``AdvanceStep`` (returning
+``Dts->invDt_hyp``), ``Hydro.run(n)`` is ``n`` iterations of the main loop
+(``Integrate`` + ``NextTimeStep``) kept on the device.  There is no CPU
+fallback: if the HIP library is missing or no GPU is visible, every call
+raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import pathlib
+
+import numpy as np
+
+import setups
+
+HERE = pathlib.Path(__file__).resolve().parent
+LIB_PATH = HERE / "libplutomx.so"
+
+REC_CODE = {  # (recon, limiter) -> PMX_REC_*
+    (setups.PLM, setups.LIM_VANLEER): 0,
+    (setups.PLM, setups.LIM_MC): 1,
+    (setups.PLM, setups.LIM_MINMOD): 2,
+    (setups.PLM, setups.LIM_DEFAULT): 3,
+}
+PMX_REC_PPM = 5
+
+EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro_set_prim",
+           "pmx_hydro_get_prim", "pmx_hydro_get_cons", "pmx_hydro_set_time",
+           "pmx_hydro_get_time", "pmx_hydro_advance_step", "pmx_hydro_run",
+           "pmx_hydro_sync", "pmx_hydro_run_graph", "pmx_hydro_stage_timing",
+           "pmx_hydro_device_state"]
+
+
+class PmxError(RuntimeError):
+    pass
+
+
+class HydroDesc(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int), ("nx1", ctypes.c_int), ("nx2", ctypes.c_int),
+                ("nx3", ctypes.c_int), ("recon", ctypes.c_int), ("solver", ctypes.c_int),
+                ("rk", ctypes.c_int), ("bc", ctypes.c_int * 6), ("gamma", ctypes.c_double),
+                ("cfl", ctypes.c_double), ("cfl_max_var", ctypes.c_double),
+                ("dx1", ctypes.POINTER(ctypes.c_double)),
+                ("dx2", ctypes.POINTER(ctypes.c_double)),
+                ("dx3", ctypes.POINTER(ctypes.c_double))]
+
+
+_lib = None
+
+
+def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
+    """Load libplutomx.so (built by build.py); raises if it is missing."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = pathlib.Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise PmxError(f"{p} is not built: run `python pluto-sirocco_amd/build.py`")
+    lib = ctypes.CDLL(str(p))
+    dp = ctypes.POINTER(ctypes.c_double)
+    ip = ctypes.POINTER(ctypes.c_int)
+    vp = ctypes.c_void_p
+    lib.pmx_last_error.restype = ctypes.c_char_p
+    lib.pmx_hydro_create.argtypes = [ctypes.POINTER(HydroDesc), ctypes.POINTER(vp)]
+    lib.pmx_hydro_destroy.argtypes = [vp]
+    lib.pmx_hydro_set_prim.argtypes = [vp, dp, ctypes.c_int]
+    lib.pmx_hydro_get_prim.argtypes = [vp, dp, ctypes.c_int]
+    lib.pmx_hydro_get_cons.argtypes = [vp, dp, ctypes.c_int]
+    lib.pmx_hydro_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+    lib.pmx_hydro_get_time.argtypes = [vp, dp, dp, dp, ip]
+    lib.pmx_hydro_advance_step.argtypes = [vp, ctypes.c_double, dp]
+    lib.pmx_hydro_run.argtypes = [vp, ctypes.c_int]
+    lib.pmx_hydro_run_graph.argtypes = [vp, ctypes.c_int]
+    lib.pmx_hydro_sync.argtypes = [vp]
+    lib.pmx_hydro_stage_timing.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ip]
+    lib.pmx_hydro_device_state.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    for name in EXPORTS:
+        if name != "pmx_last_error":
+            getattr(lib, name).restype = ctypes.c_int
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(rc: int, what: str) -> None:
+    if rc != 0:
+        raise PmxError(f"{what}: {load_library().pmx_last_error().decode()}")
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def cell_widths(cfg: setups.HydroConfig, d: int) -> np.ndarray:
+    """grid->dx[d][IBEG..IEND] of a uniform grid (Src/set_grid.c:396-403)."""
+    n = cfg.nx[d] if d < cfg.dims else 1
+    if d >= cfg.dims:
+        return np.ones(1)
+    return np.full(n, (cfg.xend[d] - cfg.xbeg[d]) / float(n))
+
+
+class Hydro:
+    """One grid advanced on the GPU by libplutomx."""
+
+    def __init__(self, cfg: setups.HydroConfig):
+        self.cfg = cfg
+        lib = load_library()
+        d = HydroDesc()
+        d.dims = cfg.dims
+        d.nx1, d.nx2 = cfg.nx[0], cfg.nx[1]
+        d.nx3 = cfg.nx[2] if cfg.dims == 3 else 1
+        d.recon = PMX_REC_PPM if cfg.recon == setups.PPM else REC_CODE[(cfg.recon, cfg.limiter)]
+        d.solver, d.rk = cfg.solver, cfg.rk
+        for s in range(6):
+            d.bc[s] = cfg.bc[s]
+        d.gamma, d.cfl, d.cfl_max_var = cfg.gamma, cfg.cfl, cfg.cfl_max_var
+        self._dx = [cell_widths(cfg, q) for q in range(3)]
+        d.dx1, d.dx2, d.dx3 = (_dptr(x) for x in self._dx)
+        h = ctypes.c_void_p()
+        _check(lib.pmx_hydro_create(ctypes.byref(d), ctypes.byref(h)), "pmx_hydro_create")
+        self._h = h
+        self._lib = lib
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.pmx_hydro_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # --- state -----------------------------------------------------------
+    def set_prim(self, V: np.ndarray, ghosts: bool = False) -> None:
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        _check(self._lib.pmx_hydro_set_prim(self._h, _dptr(V), int(ghosts)), "set_prim")
+
+    def get_prim(self, ghosts: bool = False) -> np.ndarray:
+        shape = self.cfg.shape
+        if ghosts:
+            g = self.cfg.nghost
+            shape = (5, shape[1] + (2 * g if self.cfg.dims == 3 else 0), shape[2] + 2 * g,
+                     shape[3] + 2 * g)
+        out = np.empty(shape)
+        _check(self._lib.pmx_hydro_get_prim(self._h, _dptr(out), int(ghosts)), "get_prim")
+        return out
+
+    def get_cons(self) -> np.ndarray:
+        out = np.empty(self.cfg.shape)
+        _check(self._lib.pmx_hydro_get_cons(self._h, _dptr(out), 0), "get_cons")
+        return out
+
+    def set_time(self, t: float, dt: float) -> None:
+        _check(self._lib.pmx_hydro_set_time(self._h, t, dt), "set_time")
+
+    def get_time(self):
+        t, dt, inv = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        fl = ctypes.c_int()
+        _check(self._lib.pmx_hydro_get_time(self._h, ctypes.byref(t), ctypes.byref(dt),
+                                            ctypes.byref(inv), ctypes.byref(fl)), "get_time")
+        return t.value, dt.value, inv.value, fl.value
+
+    # --- stepping --------------------------------------------------------
+    def advance_step(self, dt: float) -> float:
+        inv = ctypes.c_double()
+        _check(self._lib.pmx_hydro_advance_step(self._h, dt, ctypes.byref(inv)), "advance_step")
+        return inv.value
+
+    def run(self, nsteps: int, graph: bool = False) -> None:
+        fn = self._lib.pmx_hydro_run_graph if graph else self._lib.pmx_hydro_run
+        _check(fn(self._h, nsteps), "run")
+
+    def sync(self) -> None:
+        _check(self._lib.pmx_hydro_sync(self._h), "sync")
+
+    def stage_timing(self, enable: bool, reset: bool = True):
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        _check(self._lib.pmx_hydro_stage_timing(self._h, int(enable), int(reset),
+                                                ctypes.byref(ms), ctypes.byref(n)), "timing")
+        return ms.value, n.value

@@ -1,0 +1,73 @@
+"""Write the golden fixtures of the hydro step from the compiled reference.
+
+Run in a container that holds the reference sources (/root/reference):
+
+    python tests/golden/make_golden.py
+
+For every case below the unmodified PLUTO reference (oracle/_ref/<problem>/
+pluto, built by oracle/ref/Makefile) is run from a seeded initial condition;
+the initial state, the state after N steps (V and Uc) and the dt sequence
+are stored as tests/golden/<case>.npz (float64, no pickles).
+"""
+from __future__ import annotations
+
+import dataclasses
+import json
+import pathlib
+import sys
+
+import numpy as np
+
+ROOT = pathlib.Path(__file__).resolve().parents[2]
+sys.path.insert(0, str(ROOT / "pluto-sirocco_amd"))
+sys.path.insert(0, str(ROOT))
+
+import setups  # noqa: E402
+from oracle import refrun  # noqa: E402
+
+OUT = pathlib.Path(__file__).resolve().parent
+
+
+def cases():
+    kh = setups.c1_kh(32)
+    kh = dataclasses.replace(kh, first_dt=1.e-3)
+    yield "kh_plm_vl_hllc_rk2_32", kh, setups.kelvin_helmholtz(kh, seed=7), 30
+
+    sd = dataclasses.replace(setups.c2_sedov(12), first_dt=3.e-3)
+    yield "sedov_ppm_hllc_rk3_12", sd, setups.sedov(sd, seed=11), 20
+
+    sr = dataclasses.replace(setups.c2_sedov(12), first_dt=3.e-3, solver=setups.ROE,
+                             bc=(setups.BC_REFLECTIVE, setups.BC_OUTFLOW) * 3)
+    yield "sedov_ppm_roe_rk3_reflective_12", sr, setups.sedov(sr, seed=12), 20
+
+    pd = dataclasses.replace(setups.c2_sedov(10), recon=setups.PLM,
+                             limiter=setups.LIM_DEFAULT, first_dt=3.e-3,
+                             bc=(setups.BC_OUTFLOW, setups.BC_REFLECTIVE,
+                                 setups.BC_PERIODIC, setups.BC_PERIODIC,
+                                 setups.BC_REFLECTIVE, setups.BC_OUTFLOW))
+    yield "mixed_plm_default_hllc_rk3_10", pd, setups.sedov(pd, seed=13), 20
+
+    mc = dataclasses.replace(setups.c2_sedov(10), recon=setups.PLM, limiter=setups.LIM_MC,
+                             rk=2, solver=setups.ROE, first_dt=3.e-3,
+                             bc=(setups.BC_PERIODIC,) * 6)
+    yield "periodic_plm_mc_roe_rk2_10", mc, setups.sedov(mc, seed=14), 20
+
+    p2 = dataclasses.replace(setups.c1_kh(24), recon=setups.PPM, limiter=setups.LIM_DEFAULT,
+                             first_dt=1.e-3, bc=(setups.BC_OUTFLOW, setups.BC_REFLECTIVE,
+                                                 setups.BC_PERIODIC, setups.BC_PERIODIC, 0, 0))
+    yield "kh_ppm_hllc_rk2_mixed_24", p2, setups.kelvin_helmholtz(p2, seed=15), 30
+
+
+def main():
+    refrun.build(with_reference=True)
+    for name, cfg, V0, n in cases():
+        r = refrun.run_reference(cfg, V0, n)
+        meta = dataclasses.asdict(cfg)
+        np.savez_compressed(OUT / f"{name}.npz", V0=V0, V=r["V"], U=r["U"],
+                            dt_seq=r["dt_seq"], t=np.array(r["t"]), dt=np.array(r["dt"]),
+                            nsteps=np.array(n), config=np.array(json.dumps(meta)))
+        print(f"{name}: {n} steps, t={r['t']:.6e}, next dt={r['dt']:.6e}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,201 @@
+"""GPU parity tests of the hydro step (libplutomx through its C ABI).
+
+Bar: bit-identical to the reference for every conserved and primitive
+variable, the time and the next dt (integer-exact comparison of float64).
+
+* the golden fixtures written by the compiled reference;
+* the oracle (plain-C restatement, itself pinned to the reference by
+  tests/test_oracle.py) on grids whose sizes are not multiples of the tile
+  shape, all reconstructions, both solvers and all boundary types;
+* at the full 256^3 size of the Sedov workload: determinism, axis
+  permutation symmetry (1e-12 relative) and exact-to-round-off
+  conservation with periodic boundaries.
+"""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+import golden_io
+import plutomx
+import setups
+from oracle import refrun
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run_gpu(cfg, V0, nsteps, graph=False):
+    with plutomx.Hydro(cfg) as h:
+        h.set_prim(V0)
+        h.set_time(0.0, cfg.first_dt)
+        h.run(nsteps, graph=graph)
+        h.sync()
+        t, dt, inv, flags = h.get_time()
+        return {"V": h.get_prim(), "U": h.get_cons(), "t": t, "dt": dt, "inv": inv,
+                "flags": flags}
+
+
+def assert_bitwise(a, b, what):
+    if not np.array_equal(a, b):
+        d = np.abs(a - b)
+        idx = np.unravel_index(np.argmax(d), d.shape)
+        raise AssertionError(f"{what}: max |diff| {d.max():.3e} at {idx}")
+
+
+@pytest.mark.parametrize("name", golden_io.names())
+def test_gpu_matches_reference_fixture(name):
+    cfg, g = golden_io.load(name)
+    say(f"[gpu] fixture {name}")
+    r = run_gpu(cfg, g["V0"], int(g["nsteps"]))
+    assert_bitwise(r["V"], g["V"], "V")
+    assert_bitwise(r["U"], g["U"], "U")
+    assert r["t"] == float(g["t"]) and r["dt"] == float(g["dt"])
+
+
+ODD = [
+    setups.HydroConfig(dims=3, nx=(37, 21, 19), recon=setups.PPM, solver=setups.HLLC, rk=3,
+                       bc=(setups.BC_OUTFLOW,) * 6, cfl=0.3, first_dt=2e-3),
+    setups.HydroConfig(dims=3, nx=(20, 30, 17), recon=setups.PLM, limiter=setups.LIM_DEFAULT,
+                       solver=setups.ROE, rk=3, bc=(setups.BC_PERIODIC,) * 6, first_dt=2e-3),
+    setups.HydroConfig(dims=3, nx=(33, 17, 40), recon=setups.PPM, solver=setups.ROE, rk=2,
+                       bc=(setups.BC_REFLECTIVE, setups.BC_OUTFLOW, setups.BC_OUTFLOW,
+                           setups.BC_REFLECTIVE, setups.BC_PERIODIC, setups.BC_PERIODIC),
+                       first_dt=2e-3),
+    setups.HydroConfig(dims=3, nx=(16, 9, 12), recon=setups.PLM, limiter=setups.LIM_MC,
+                       solver=setups.HLLC, rk=2, bc=(setups.BC_REFLECTIVE,) * 6, first_dt=2e-3),
+    setups.HydroConfig(dims=2, nx=(45, 33, 1), recon=setups.PLM, limiter=setups.LIM_VANLEER,
+                       solver=setups.HLLC, rk=2, bc=(setups.BC_PERIODIC,) * 6, first_dt=1e-3),
+    setups.HydroConfig(dims=2, nx=(70, 19, 1), recon=setups.PPM, solver=setups.HLLC, rk=3,
+                       bc=(setups.BC_OUTFLOW, setups.BC_REFLECTIVE, setups.BC_REFLECTIVE,
+                           setups.BC_OUTFLOW, 0, 0), first_dt=1e-3),
+    setups.HydroConfig(dims=2, nx=(31, 64, 1), recon=setups.PLM, limiter=setups.LIM_MINMOD,
+                       solver=setups.ROE, rk=3, bc=(setups.BC_PERIODIC,) * 6, first_dt=1e-3),
+]
+
+
+def _ic(cfg, seed):
+    if cfg.dims == 3:
+        V = setups.sedov(cfg, seed=seed)
+    else:
+        V = setups.kelvin_helmholtz(cfg, seed=seed)
+    rng = np.random.default_rng(seed)
+    V[1:4] += 0.05 * rng.standard_normal(V[1:4].shape)
+    return V
+
+
+@pytest.mark.parametrize("idx", range(len(ODD)))
+def test_gpu_matches_oracle_odd_sizes(idx):
+    cfg = ODD[idx]
+    say(f"[gpu] oracle case {idx}: {cfg}")
+    V0 = _ic(cfg, 100 + idx)
+    n = 12
+    o = refrun.run_oracle(cfg, V0, n)
+    r = run_gpu(cfg, V0, n)
+    assert_bitwise(r["V"], o["V"], "V")
+    assert_bitwise(r["U"], o["U"], "U")
+    assert r["t"] == o["t"] and r["dt"] == o["dt"]
+
+
+def test_advance_step_interface():
+    """AdvanceStep-style call: the host supplies dt, gets invDt_hyp back."""
+    cfg = ODD[0]
+    say("[gpu] advance_step")
+    V0 = _ic(cfg, 7)
+    o = refrun.run_oracle(cfg, V0, 3)
+    with plutomx.Hydro(cfg) as h:
+        h.set_prim(V0)
+        dt = cfg.first_dt
+        for step in range(3):
+            assert dt == o["dt_seq"][step]
+            inv = h.advance_step(dt)
+            dtn = min(1.0 / inv * cfg.cfl, cfg.cfl_max_var * dt)   # NextTimeStep
+            dt = dtn
+        assert dt == o["dt"]
+        assert_bitwise(h.get_prim(), o["V"], "V")
+
+
+def test_ghost_download_follows_boundaries():
+    cfg = ODD[2]
+    say("[gpu] ghost download")
+    V0 = _ic(cfg, 9)
+    with plutomx.Hydro(cfg) as h:
+        h.set_prim(V0)
+        Vg = h.get_prim(ghosts=True)
+        g = cfg.nghost
+        assert_bitwise(Vg[:, g:-g, g:-g, g:-g], V0, "interior")
+        # x1-beg reflective: mirror with vx flipped
+        np.testing.assert_array_equal(Vg[0, g:-g, g:-g, g - 1], V0[0, :, :, 0])
+        np.testing.assert_array_equal(Vg[1, g:-g, g:-g, g - 1], -V0[1, :, :, 0])
+        # x3 periodic
+        np.testing.assert_array_equal(Vg[:, 0, g:-g, g:-g], V0[:, -g])
+        # upload from the ghosted layout round-trips
+        h.set_prim(Vg, ghosts=True)
+        assert_bitwise(h.get_prim(), V0, "round trip")
+
+
+def test_graph_replay_matches_eager():
+    cfg = ODD[1]
+    say("[gpu] graph replay")
+    V0 = _ic(cfg, 11)
+    ref = run_gpu(cfg, V0, 4)
+    with plutomx.Hydro(cfg) as h:
+        h.set_prim(V0)
+        h.set_time(0.0, cfg.first_dt)
+        for _ in range(4):           # replays separated by host work and syncs
+            h.run(1, graph=True)
+            h.sync()
+            _ = h.get_time()
+        assert_bitwise(h.get_prim(), ref["V"], "V after graph replays")
+        assert h.get_time()[:2] == (ref["t"], ref["dt"])
+
+
+# ---------------------------------------------------------------------
+# full size: 256^3 Sedov, PPM + HLLC + RK3
+# ---------------------------------------------------------------------
+@pytest.fixture(scope="module")
+def sedov256():
+    cfg = setups.c2_sedov(256)
+    return cfg, setups.sedov(cfg, seed=1234)
+
+
+def test_full_size_determinism_and_symmetry(sedov256):
+    cfg, V0 = sedov256
+    say("[gpu] 256^3 determinism + permutation symmetry")
+    cfg = dataclasses.replace(cfg, first_dt=1e-6)
+    a = run_gpu(cfg, V0, 2)
+    b = run_gpu(cfg, V0, 2)
+    assert_bitwise(a["V"], b["V"], "run-to-run")
+    assert a["flags"] == 0
+    # swap x and y (and vx, vy): the result must be the swapped result
+    Vs = V0[[0, 2, 1, 3, 4]].transpose(0, 1, 3, 2).copy()
+    s = run_gpu(cfg, Vs, 2)
+    back = s["V"][[0, 2, 1, 3, 4]].transpose(0, 1, 3, 2)
+    scale = np.abs(a["V"]).max(axis=(1, 2, 3), keepdims=True)
+    err = (np.abs(back - a["V"]) / scale).max()
+    assert err < 1e-12, err
+    assert s["dt"] == pytest.approx(a["dt"], rel=1e-13)
+
+
+def test_full_size_periodic_conservation(sedov256):
+    cfg, V0 = sedov256
+    say("[gpu] 256^3 periodic conservation")
+    cfg = dataclasses.replace(cfg, bc=(setups.BC_PERIODIC,) * 6, first_dt=1e-6)
+    g = cfg.gamma - 1.0
+    U0 = np.empty_like(V0)
+    U0[0] = V0[0]
+    U0[1:4] = V0[0] * V0[1:4]
+    U0[4] = 0.5 * V0[0] * (V0[1] ** 2 + V0[2] ** 2 + V0[3] ** 2) + V0[4] / g
+    r = run_gpu(cfg, V0, 3)
+    for nv in (0, 4):
+        s0 = np.sum(U0[nv], dtype=np.float64)
+        s1 = np.sum(r["U"][nv], dtype=np.float64)
+        assert abs(s1 - s0) <= 1e-12 * abs(s0), (nv, s0, s1)
+    for nv in (1, 2, 3):
+        tot = np.abs(U0[0]).sum()
+        assert abs(r["U"][nv].sum() - U0[nv].sum()) <= 1e-12 * tot
