@@ -71,6 +71,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay a captured HIP graph")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
+                    help="only run a plain device copy of MB megabytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; the hydro step shards by spatial "
@@ -79,6 +81,12 @@ def main() -> int:
         return 2
 
     import plutomx
+    if args.copy_probe:
+        nbytes = args.copy_probe * 1 << 20
+        ms = plutomx.copy_probe(nbytes, reps=5)
+        print(json.dumps({"copy_probe_bytes": nbytes, "ms": ms,
+                          "GBps_read_plus_write": 2 * nbytes / ms / 1e6}), flush=True)
+        return 0
     cfg = setups.c2_sedov(args.n)
     V0 = setups.sedov(cfg, seed=1234)
     h = plutomx.Hydro(cfg)
@@ -97,6 +105,7 @@ def main() -> int:
     t, dt, inv, flags = h.get_time()
     h.close()
 
+    copy_ms = plutomx.copy_probe(1 << 30, reps=5)
     elapsed = t1 - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = cfg.ncells * args.steps / elapsed / 1e6
@@ -109,7 +118,8 @@ def main() -> int:
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "kernel": "stage_kernel (fused PPM+HLLC stage, all directions)",
                 "mean_launch_ms": round(mean_launch_ms, 4),
-                "algorithmic_bytes_per_launch": bytes_per_launch}
+                "algorithmic_bytes_per_launch": bytes_per_launch,
+                "plain_copy_GBps": round(2 * (1 << 30) / copy_ms / 1e6, 1)}
 
     out = {"metric": METRIC, "value": round(value, 2), "unit": "Mcell-updates/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,

@@ -103,6 +103,12 @@ int pmx_hydro_stage_timing(pmx_hydro *h, int enable, int reset, double *total_ms
  * the conservative U (for zero-copy consumers such as torch). */
 int pmx_hydro_device_state(pmx_hydro *h, double **V, double **U);
 
+/* Measurement helper (no counterpart in the reference): times `reps`
+ * launches of a plain streaming copy of nbytes (multiple of 16) between two
+ * fresh device buffers; reports the mean milliseconds per copy.  Used to
+ * calibrate HBM byte counters and as the achievable-bandwidth reference. */
+int pmx_copy_probe(long long nbytes, int reps, double *ms_per_copy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -14,8 +14,12 @@ namespace pmx {
 static thread_local std::string g_err;
 void set_error(const std::string& msg) { g_err = msg; }
 
-constexpr int TILE_X = 32;
-constexpr int TILE_Y = 8;
+// Tile of the x-y plane per workgroup: every per-plane phase (x-states,
+// x-faces, y-states, y-faces) has at most 256 items, so each is a single
+// pass of the 256-thread workgroup.
+constexpr int TILE_X = 16;
+constexpr int TILE_Y = 14;
+constexpr int NTHREADS = 256;
 
 struct DevScalars {
   double t;                      // g_time
@@ -56,6 +60,15 @@ __global__ void finish_kernel(DevScalars* s, int ndim, double cfl, double cfl_ma
   if (cfl_max_var != 1.0) s->dt = dtnext;
 }
 
+// Plain streaming copy, used to calibrate the HBM byte counters and to
+// measure the bandwidth a simple kernel reaches on this device.
+__global__ void copy_probe_kernel(const double2* __restrict__ src, double2* __restrict__ dst,
+                                  long long n2) {
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long q = t; q < n2; q += stride) dst[q] = src[q];
+}
+
 }  // namespace pmx
 
 using namespace pmx;
@@ -86,8 +99,8 @@ namespace {
 
 template <int DIMS, int REC, int SOLVER, int MODE>
 void launch_stage(pmx_hydro* h, const StageArgs& a, int nblocks) {
-  hipLaunchKernelGGL((stage_kernel<DIMS, REC, SOLVER, MODE, TILE_X, TILE_Y>), dim3(nblocks),
-                     dim3(TILE_X * TILE_Y), 0, h->stream, a);
+  hipLaunchKernelGGL((stage_kernel<DIMS, REC, SOLVER, MODE, TILE_X, TILE_Y, NTHREADS>),
+                     dim3(nblocks), dim3(NTHREADS), 0, h->stream, a);
 }
 
 template <int DIMS, int REC, int SOLVER>
@@ -451,6 +464,33 @@ int pmx_hydro_stage_timing(pmx_hydro* h, int enable, int reset, double* total_ms
   if (launches) *launches = h->timed_launches;
   if (reset) { h->timed_ms = 0.0; h->timed_launches = 0; }
   h->timing = enable != 0;
+  return 0;
+}
+
+int pmx_copy_probe(long long nbytes, int reps, double* ms_per_copy) {
+  if (nbytes <= 0 || nbytes % 16 != 0 || reps <= 0) { set_error("bad argument"); return -1; }
+  double2 *a = nullptr, *b = nullptr;
+  hipEvent_t e0, e1;
+  PMX_HIP_CHECK(hipMalloc(&a, nbytes));
+  if (hipMalloc(&b, nbytes) != hipSuccess) { hipFree(a); set_error("hipMalloc"); return -1; }
+  hipMemset(a, 0, nbytes);
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  long long n2 = nbytes / 16;
+  hipLaunchKernelGGL(copy_probe_kernel, dim3(4096), dim3(256), 0, 0, a, b, n2);
+  hipEventRecord(e0, 0);
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(4096), dim3(256), 0, 0, a, b, n2);
+  hipEventRecord(e1, 0);
+  hipError_t err = hipEventSynchronize(e1);
+  float ms = 0.f;
+  hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  if (err != hipSuccess) { set_error(hipGetErrorString(err)); return -1; }
+  if (ms_per_copy) *ms_per_copy = ms / reps;
   return 0;
 }
 

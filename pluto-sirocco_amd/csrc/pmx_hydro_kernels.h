@@ -86,10 +86,28 @@ __device__ __forceinline__ void cell_states(const double (*s)[NV], double* vp, d
   }
 }
 
-template <int DIMS, int REC, int SOLVER, int MODE, int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
+// Map halo item `it` of a tile plane to LDS coordinates (r, c): first the
+// x-halo (H columns left and right of every tile row), then the y-halo
+// (H rows below and above every tile column).
+template <int H, int TX, int TY>
+__device__ __forceinline__ void halo_item(int it, int& r, int& c) {
+  constexpr int NHX = 2 * H * TY;
+  if (it < NHX) {
+    int cc = it % (2 * H);
+    r = H + it / (2 * H);
+    c = cc < H ? cc : TX + cc;
+  } else {
+    int jt = it - NHX;
+    int rr = jt / TX;
+    c = H + jt % TX;
+    r = rr < H ? rr : TY + rr;
+  }
+}
+
+template <int DIMS, int REC, int SOLVER, int MODE, int TX, int TY, int NT>
+__global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
   constexpr int H = RecTraits<REC>::H;
-  constexpr int NT = TX * TY;
+  constexpr int NC = TX * TY;          // cells of a tile plane
   constexpr int LW = TX + 2 * H;
   constexpr int LH = TY + 2 * H;
   constexpr int NSX = (TX + 2) * TY;   // x-states per tile plane
@@ -100,6 +118,9 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
   constexpr int NPL = NV * LH * LW;
   constexpr int NFYD = 7 * NFY;
   constexpr int NPLANE = NPL > NFYD ? NPL : NFYD;
+  constexpr int NHALO = 2 * H * (TX + TY);
+  constexpr int HPT = (NHALO + NT - 1) / NT;  // halo cells loaded per thread
+  static_assert(NC <= NT, "one thread per tile cell");
 
   __shared__ double sPlane[NPLANE];    // V plane [nv][LH][LW]; later y-fluxes [NFY][7]
   __shared__ double sSt[NST * 10];     // states [item][vp(5) vm(5)]
@@ -107,6 +128,7 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
   __shared__ double sRed[NT / 64];
 
   const int tid = threadIdx.x;
+  const bool owner = tid < NC;         // thread owns tile cell (tx, ty)
   const int tx = tid % TX, ty = tid / TX;
   const int nblocks = a.tiles_x * a.tiles_y * a.tiles_z;
   const int lb = xcd_remap(blockIdx.x, nblocks);
@@ -115,12 +137,12 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
   const int bz = lb / (a.tiles_x * a.tiles_y);
   const int i0 = bx * TX, j0 = by * TY;
   const int i = i0 + tx, j = j0 + ty;
-  const bool valid = (i < a.nx) && (j < a.ny);
+  const bool valid = owner && (i < a.nx) && (j < a.ny);
   const int k0 = (DIMS == 3) ? bz * a.cz : 0;
   const int k1 = (DIMS == 3) ? min(k0 + a.cz, a.nz) : 1;
   const double gamma = a.gamma, gmm1 = a.gmm1;
 
-  double cdt_local = 0.0, cdt_max = 0.0;
+  double cdt_max = 0.0;
   int fail = 0;
 
   // ---- z-direction register state -----------------------------------
@@ -131,20 +153,21 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
   double vpz[NV];         // right-face state of cell k
   double Fz[7];           // flux at k-1/2 (5), press, cmax
 
-  if (DIMS == 3) {
+  if (DIMS == 3 && owner) {
     // prologue: states of cells k0-1 and k0, flux at k0-1/2
     if (REC == REC_PPM) {
       double p[6][NV];  // v[k0-3 .. k0+2]
 #pragma unroll
       for (int s = 0; s < 6; s++) load_cell<DIMS>(a, a.Vin, i, j, k0 - 3 + s, p[s]);
-      double Ia[NV], Ib[NV], vm0[NV], vpm[NV], vmm[NV];
+      double vm0[NV], vpm[NV];
 #pragma unroll
       for (int nv = 0; nv < NV; nv++) {
-        Ia[nv] = ppm_iface(p[0][nv], p[1][nv], p[2][nv], p[3][nv]);  // I(k0-3/2)
-        Ib[nv] = ppm_iface(p[1][nv], p[2][nv], p[3][nv], p[4][nv]);  // I(k0-1/2)
-        Iz[nv] = ppm_iface(p[2][nv], p[3][nv], p[4][nv], p[5][nv]);  // I(k0+1/2)
-        ppm_limit(p[2][nv], Ib[nv], Ia[nv], vpm[nv], vmm[nv]);       // cell k0-1
-        ppm_limit(p[3][nv], Iz[nv], Ib[nv], vpz[nv], vm0[nv]);       // cell k0
+        double Ia = ppm_iface(p[0][nv], p[1][nv], p[2][nv], p[3][nv]);  // I(k0-3/2)
+        double Ib = ppm_iface(p[1][nv], p[2][nv], p[3][nv], p[4][nv]);  // I(k0-1/2)
+        Iz[nv] = ppm_iface(p[2][nv], p[3][nv], p[4][nv], p[5][nv]);     // I(k0+1/2)
+        double vmm;
+        ppm_limit(p[2][nv], Ib, Ia, vpm[nv], vmm);                       // cell k0-1
+        ppm_limit(p[3][nv], Iz[nv], Ib, vpz[nv], vm0[nv]);               // cell k0
       }
       riemann<SOLVER, 2, DIMS>(vpm, vm0, gamma, gmm1, Fz, Fz[5], Fz[6]);
 #pragma unroll
@@ -166,39 +189,52 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
     }
   }
 
+  // halo of the first plane; later planes are prefetched one plane ahead
+  double hq[HPT][NV];
+#pragma unroll
+  for (int q = 0; q < HPT; q++) {
+    int it = tid + q * NT, r, c;
+    if (it < NHALO) {
+      halo_item<H, TX, TY>(it, r, c);
+      load_cell<DIMS>(a, a.Vin, i0 + c - H, j0 + r - H, k0, hq[q]);
+    }
+  }
+
   for (int k = k0; k < k1; k++) {
     double vc[NV];  // own cell value at plane k
-    if (DIMS == 3) {
-      load_cell<DIMS>(a, a.Vin, i, j, k + WZ - 1, w[WZ - 1]);
+    if (owner) {
+      if (DIMS == 3) {
+        load_cell<DIMS>(a, a.Vin, i, j, k + WZ - 1, w[WZ - 1]);
 #pragma unroll
-      for (int nv = 0; nv < NV; nv++) vc[nv] = w[0][nv];
-    } else {
-      load_cell<DIMS>(a, a.Vin, i, j, 0, vc);
+        for (int nv = 0; nv < NV; nv++) vc[nv] = w[0][nv];
+      } else {
+        load_cell<DIMS>(a, a.Vin, i, j, 0, vc);
+      }
+    }
+    // U of this stage and V at the start of the step, needed by the update
+    double u[NV], v0[NV];
+    const long long o = ((long long)k * a.ny + (valid ? j : 0)) * a.nx + (valid ? i : 0);
+    if (MODE != MODE_FIRST && valid) {
+#pragma unroll
+      for (int nv = 0; nv < NV; nv++) {
+        u[nv] = a.U[o + nv * a.sv];
+        v0[nv] = a.V0[o + nv * a.sv];
+      }
     }
 
     // ---- P0: stage the plane with its x/y halo in LDS ----------------
+    if (owner) {
 #pragma unroll
-    for (int nv = 0; nv < NV; nv++) sPlane[(nv * LH + H + ty) * LW + H + tx] = vc[nv];
-    {
-      // x-halo: H columns left/right of each tile row; y-halo: H rows
-      // below/above each tile column.  (r, c) are LDS coordinates.
-      constexpr int NHX = 2 * H * TY, NHY = 2 * H * TX;
-      for (int it = tid; it < NHX + NHY; it += NT) {
-        int r, c;
-        if (it < NHX) {
-          int cc = it % (2 * H);
-          r = H + it / (2 * H);
-          c = cc < H ? cc : TX + cc;
-        } else {
-          int jt = it - NHX;
-          int rr = jt / TX;
-          c = H + jt % TX;
-          r = rr < H ? rr : TY + rr;
-        }
-        double q[NV];
-        load_cell<DIMS>(a, a.Vin, i0 + c - H, j0 + r - H, k, q);
+      for (int nv = 0; nv < NV; nv++) sPlane[(nv * LH + H + ty) * LW + H + tx] = vc[nv];
+    }
 #pragma unroll
-        for (int nv = 0; nv < NV; nv++) sPlane[(nv * LH + r) * LW + c] = q[nv];
+    for (int q = 0; q < HPT; q++) {
+      int it = tid + q * NT, r, c;
+      if (it < NHALO) {
+        halo_item<H, TX, TY>(it, r, c);
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) sPlane[(nv * LH + r) * LW + c] = hq[q][nv];
+        if (k + 1 < k1) load_cell<DIMS>(a, a.Vin, i0 + c - H, j0 + r - H, k + 1, hq[q]);
       }
     }
     __syncthreads();
@@ -278,102 +314,95 @@ __global__ __launch_bounds__(TX * TY) void stage_kernel(StageArgs a) {
     __syncthreads();
 
     // ---- P5: z-face, update, RK combination, cons->prim ---------------
-    double Fzn[7];
-    if (DIMS == 3) {
-      double vmn[NV], vpn[NV];
-      if (REC == REC_PPM) {
-        double In[NV];
-#pragma unroll
-        for (int nv = 0; nv < NV; nv++) {
-          In[nv] = ppm_iface(w[0][nv], w[1][nv], w[2][nv], w[3][nv]);  // I(k+3/2)
-          ppm_limit(w[1][nv], In[nv], Iz[nv], vpn[nv], vmn[nv]);      // cell k+1
-          Iz[nv] = In[nv];
-        }
-      } else {
-        plm_states<REC>(w[0], w[1], w[2], vpn, vmn);                  // cell k+1
-      }
-      riemann<SOLVER, 2, DIMS>(vpz, vmn, gamma, gmm1, Fzn, Fzn[5], Fzn[6]);
-#pragma unroll
-      for (int nv = 0; nv < NV; nv++) vpz[nv] = vpn[nv];
-    }
-
-    if (valid) {
-      const long long o = ((long long)k * a.ny + j) * a.nx + i;
-      double u[NV];
-      if (MODE == MODE_FIRST) {
-        prim_to_cons(vc, u, gmm1);
-      } else {
-#pragma unroll
-        for (int nv = 0; nv < NV; nv++) u[nv] = a.U[o + nv * a.sv];
-      }
-      // x
-      {
-        const double* Fm = &sFx[(ty * (TX + 1) + tx) * 7];
-        const double* Fp = Fm + 7;
-        double scrh = a.dtdx[0][i];
-        double rhs[NV];
-#pragma unroll
-        for (int nv = 0; nv < NV; nv++) rhs[nv] = -scrh * (Fp[nv] - Fm[nv]);
-        rhs[VX1] -= scrh * (Fp[5] - Fm[5]);
-#pragma unroll
-        for (int nv = 0; nv < NV; nv++) u[nv] += rhs[nv];
-        if (MODE == MODE_FIRST) cdt_local = 0.0 + 0.5 * (Fm[6] + Fp[6]) * a.inv_dl[0][i] * 1.0;
-      }
-      // y
-      {
-        const double* Fm = &sPlane[(ty * TX + tx) * 7];
-        const double* Fp = Fm + 7 * TX;
-        double scrh = a.dtdx[1][j];
-        double rhs[NV];
-#pragma unroll
-        for (int nv = 0; nv < NV; nv++) rhs[nv] = -scrh * (Fp[nv] - Fm[nv]);
-        rhs[VX2] -= scrh * (Fp[5] - Fm[5]);
-#pragma unroll
-        for (int nv = 0; nv < NV; nv++) u[nv] += rhs[nv];
-        if (MODE == MODE_FIRST) cdt_local += 0.5 * (Fm[6] + Fp[6]) * a.inv_dl[1][j] * 1.0;
-      }
-      // z
+    if (owner) {
+      double Fzn[7];
       if (DIMS == 3) {
-        double scrh = a.dtdx[2][k];
-        double rhs[NV];
+        double vmn[NV], vpn[NV];
+        if (REC == REC_PPM) {
 #pragma unroll
-        for (int nv = 0; nv < NV; nv++) rhs[nv] = -scrh * (Fzn[nv] - Fz[nv]);
-        rhs[VX3] -= scrh * (Fzn[5] - Fz[5]);
-#pragma unroll
-        for (int nv = 0; nv < NV; nv++) u[nv] += rhs[nv];
-        if (MODE == MODE_FIRST) cdt_local += 0.5 * (Fz[6] + Fzn[6]) * a.inv_dl[2][k] * 1.0;
-      }
-      if (MODE != MODE_FIRST) {
-        double v0[NV], u0[NV];
-#pragma unroll
-        for (int nv = 0; nv < NV; nv++) v0[nv] = a.V0[o + nv * a.sv];
-        prim_to_cons(v0, u0, gmm1);
-        if (MODE == MODE_WEIGHTED) {
-#pragma unroll
-          for (int nv = 0; nv < NV; nv++) u[nv] = a.w0 * u0[nv] + a.wc * u[nv];
+          for (int nv = 0; nv < NV; nv++) {
+            double In = ppm_iface(w[0][nv], w[1][nv], w[2][nv], w[3][nv]);  // I(k+3/2)
+            ppm_limit(w[1][nv], In, Iz[nv], vpn[nv], vmn[nv]);             // cell k+1
+            Iz[nv] = In;
+          }
         } else {
-          const double one_third = 1.0 / 3.0;
-#pragma unroll
-          for (int nv = 0; nv < NV; nv++) u[nv] = one_third * (u0[nv] + 2.0 * u[nv]);
+          plm_states<REC>(w[0], w[1], w[2], vpn, vmn);                     // cell k+1
         }
+        riemann<SOLVER, 2, DIMS>(vpz, vmn, gamma, gmm1, Fzn, Fzn[5], Fzn[6]);
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) vpz[nv] = vpn[nv];
       }
-      double vn[NV];
-      fail |= cons_to_prim(u, vn, gmm1);
+
+      if (valid) {
+        if (MODE == MODE_FIRST) prim_to_cons(vc, u, gmm1);
+        double cdt = 0.0;
+        // x
+        {
+          const double* Fm = &sFx[(ty * (TX + 1) + tx) * 7];
+          const double* Fp = Fm + 7;
+          double scrh = a.dtdx[0][i];
+          double rhs[NV];
 #pragma unroll
-      for (int nv = 0; nv < NV; nv++) a.Vout[o + nv * a.sv] = vn[nv];
-      if (a.write_u) {
+          for (int nv = 0; nv < NV; nv++) rhs[nv] = -scrh * (Fp[nv] - Fm[nv]);
+          rhs[VX1] -= scrh * (Fp[5] - Fm[5]);
 #pragma unroll
-        for (int nv = 0; nv < NV; nv++) a.U[o + nv * a.sv] = u[nv];
+          for (int nv = 0; nv < NV; nv++) u[nv] += rhs[nv];
+          if (MODE == MODE_FIRST) cdt = 0.0 + 0.5 * (Fm[6] + Fp[6]) * a.inv_dl[0][i] * 1.0;
+        }
+        // y
+        {
+          const double* Fm = &sPlane[(ty * TX + tx) * 7];
+          const double* Fp = Fm + 7 * TX;
+          double scrh = a.dtdx[1][j];
+          double rhs[NV];
+#pragma unroll
+          for (int nv = 0; nv < NV; nv++) rhs[nv] = -scrh * (Fp[nv] - Fm[nv]);
+          rhs[VX2] -= scrh * (Fp[5] - Fm[5]);
+#pragma unroll
+          for (int nv = 0; nv < NV; nv++) u[nv] += rhs[nv];
+          if (MODE == MODE_FIRST) cdt += 0.5 * (Fm[6] + Fp[6]) * a.inv_dl[1][j] * 1.0;
+        }
+        // z
+        if (DIMS == 3) {
+          double scrh = a.dtdx[2][k];
+          double rhs[NV];
+#pragma unroll
+          for (int nv = 0; nv < NV; nv++) rhs[nv] = -scrh * (Fzn[nv] - Fz[nv]);
+          rhs[VX3] -= scrh * (Fzn[5] - Fz[5]);
+#pragma unroll
+          for (int nv = 0; nv < NV; nv++) u[nv] += rhs[nv];
+          if (MODE == MODE_FIRST) cdt += 0.5 * (Fz[6] + Fzn[6]) * a.inv_dl[2][k] * 1.0;
+        }
+        if (MODE != MODE_FIRST) {
+          double u0[NV];
+          prim_to_cons(v0, u0, gmm1);
+          if (MODE == MODE_WEIGHTED) {
+#pragma unroll
+            for (int nv = 0; nv < NV; nv++) u[nv] = a.w0 * u0[nv] + a.wc * u[nv];
+          } else {
+            const double one_third = 1.0 / 3.0;
+#pragma unroll
+            for (int nv = 0; nv < NV; nv++) u[nv] = one_third * (u0[nv] + 2.0 * u[nv]);
+          }
+        }
+        double vn[NV];
+        fail |= cons_to_prim(u, vn, gmm1);
+#pragma unroll
+        for (int nv = 0; nv < NV; nv++) a.Vout[o + nv * a.sv] = vn[nv];
+        if (a.write_u) {
+#pragma unroll
+          for (int nv = 0; nv < NV; nv++) a.U[o + nv * a.sv] = u[nv];
+        }
+        if (MODE == MODE_FIRST) cdt_max = dmax(cdt_max, cdt);
       }
-      if (MODE == MODE_FIRST) cdt_max = dmax(cdt_max, cdt_local);
-    }
-    if (DIMS == 3) {
+      if (DIMS == 3) {
 #pragma unroll
-      for (int q = 0; q < 7; q++) Fz[q] = Fzn[q];
+        for (int q = 0; q < 7; q++) Fz[q] = Fzn[q];
 #pragma unroll
-      for (int s = 0; s < WZ - 1; s++)
+        for (int s = 0; s < WZ - 1; s++)
 #pragma unroll
-        for (int nv = 0; nv < NV; nv++) w[s][nv] = w[s + 1][nv];
+          for (int nv = 0; nv < NV; nv++) w[s][nv] = w[s + 1][nv];
+      }
     }
     __syncthreads();
   }

@@ -31,7 +31,7 @@ EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro
            "pmx_hydro_get_prim", "pmx_hydro_get_cons", "pmx_hydro_set_time",
            "pmx_hydro_get_time", "pmx_hydro_advance_step", "pmx_hydro_run",
            "pmx_hydro_sync", "pmx_hydro_run_graph", "pmx_hydro_stage_timing",
-           "pmx_hydro_device_state"]
+           "pmx_hydro_device_state", "pmx_copy_probe"]
 
 
 class PmxError(RuntimeError):
@@ -77,6 +77,7 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     lib.pmx_hydro_sync.argtypes = [vp]
     lib.pmx_hydro_stage_timing.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ip]
     lib.pmx_hydro_device_state.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
+    lib.pmx_copy_probe.argtypes = [ctypes.c_longlong, ctypes.c_int, dp]
     for name in EXPORTS:
         if name != "pmx_last_error":
             getattr(lib, name).restype = ctypes.c_int
@@ -100,6 +101,13 @@ def cell_widths(cfg: setups.HydroConfig, d: int) -> np.ndarray:
     if d >= cfg.dims:
         return np.ones(1)
     return np.full(n, (cfg.xend[d] - cfg.xbeg[d]) / float(n))
+
+
+def copy_probe(nbytes: int, reps: int = 10) -> float:
+    """Mean ms of a plain device copy of nbytes (bandwidth reference)."""
+    ms = ctypes.c_double()
+    _check(load_library().pmx_copy_probe(nbytes, reps, ctypes.byref(ms)), "copy_probe")
+    return ms.value
 
 
 class Hydro:
