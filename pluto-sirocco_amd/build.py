@@ -31,31 +31,39 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=
          "-I", str(HERE.parent / "include"), "-I", str(CSRC)]
 
 
-def _compile(src: pathlib.Path) -> pathlib.Path:
-    obj = BUILD / (src.stem + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", str(src), "-o", str(obj)]
+def _compile(src: pathlib.Path, extra=(), build_dir=BUILD) -> pathlib.Path:
+    obj = build_dir / (src.stem + ".o")
+    cmd = [HIPCC, *FLAGS, *extra, "-c", str(src), "-o", str(obj)]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{proc.stderr[-6000:]}")
     return obj
 
 
-def build(verbose: bool = False) -> pathlib.Path:
-    BUILD.mkdir(exist_ok=True)
+def build(verbose: bool = False, extra=(), out: pathlib.Path = LIB) -> pathlib.Path:
+    """Compile every csrc/*.hip and link `out`.  `extra` adds compiler
+    flags (e.g. -DPMX_TILE_X=32 for a tile-shape variant)."""
+    build_dir = BUILD if out == LIB else out.parent / (out.stem + "_obj")
+    build_dir.mkdir(exist_ok=True, parents=True)
     srcs = sorted(CSRC.glob("*.hip"))
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
-        objs = list(ex.map(_compile, srcs))
-    tmp = LIB.with_suffix(".so.tmp")
+        objs = list(ex.map(lambda p: _compile(p, extra, build_dir), srcs))
+    tmp = out.with_suffix(".so.tmp")
     cmd = [HIPCC, "-shared", f"--offload-arch={ARCH}", *map(str, objs), "-o", str(tmp)]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"link failed:\n{proc.stderr[-4000:]}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print(f"built {LIB}")
-    return LIB
+        print(f"built {out}")
+    return out
 
 
 if __name__ == "__main__":
-    build(verbose=True)
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("-D", action="append", default=[], help="extra preprocessor define")
+    ap.add_argument("-o", type=pathlib.Path, default=LIB)
+    a = ap.parse_args()
+    build(verbose=True, extra=[f"-D{d}" for d in a.D], out=a.o.resolve())
     sys.exit(0)

@@ -17,8 +17,14 @@ void set_error(const std::string& msg) { g_err = msg; }
 // Tile of the x-y plane per workgroup: every per-plane phase (x-states,
 // x-faces, y-states, y-faces) has at most 256 items, so each is a single
 // pass of the 256-thread workgroup.
-constexpr int TILE_X = 16;
-constexpr int TILE_Y = 14;
+#ifndef PMX_TILE_X
+#define PMX_TILE_X 16
+#endif
+#ifndef PMX_TILE_Y
+#define PMX_TILE_Y 14
+#endif
+constexpr int TILE_X = PMX_TILE_X;
+constexpr int TILE_Y = PMX_TILE_Y;
 constexpr int NTHREADS = 256;
 
 struct DevScalars {
@@ -268,8 +274,11 @@ int pmx_hydro_create(const pmx_hydro_desc* desc, pmx_hydro** out) {
   h->d = *desc;
   h->nx = desc->nx1; h->ny = desc->nx2; h->nz = nz;
   h->ncell = (long long)h->nx * h->ny * h->nz;
+  // z planes per workgroup: long enough to amortise the column prologue,
+  // short enough to give several workgroups per CU.  PMX_CZ overrides.
   h->cz = 16;
   if (h->nz < 64) h->cz = 8;
+  if (const char* e = getenv("PMX_CZ")) h->cz = atoi(e) > 0 ? atoi(e) : h->cz;
   size_t vbytes = sizeof(double) * NV * h->ncell;
   int ntot = h->nx + h->ny + h->nz;
 #define CK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { \

@@ -30,6 +30,13 @@ namespace pmx {
 
 enum { MODE_FIRST = 0, MODE_WEIGHTED = 1, MODE_THIRD = 2 };
 
+#ifndef PMX_WAVES_PER_EU
+#define PMX_WAVES_PER_EU 2   // occupancy the register allocation targets
+#endif
+#ifndef PMX_PREFETCH_U
+#define PMX_PREFETCH_U 0   // load U and V0 of the cell at the top of the plane loop
+#endif
+
 struct StageArgs {
   const double* Vin;   // stage input, halo source
   const double* V0;    // V at the start of the step (U0 = P2C(V0)); may alias Vout
@@ -105,7 +112,7 @@ __device__ __forceinline__ void halo_item(int it, int& r, int& c) {
 }
 
 template <int DIMS, int REC, int SOLVER, int MODE, int TX, int TY, int NT>
-__global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PER_EU, PMX_WAVES_PER_EU))) void stage_kernel(StageArgs a) {
   constexpr int H = RecTraits<REC>::H;
   constexpr int NC = TX * TY;          // cells of a tile plane
   constexpr int LW = TX + 2 * H;
@@ -122,9 +129,11 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
   constexpr int HPT = (NHALO + NT - 1) / NT;  // halo cells loaded per thread
   static_assert(NC <= NT, "one thread per tile cell");
 
-  __shared__ double sPlane[NPLANE];    // V plane [nv][LH][LW]; later y-fluxes [NFY][7]
-  __shared__ double sSt[NST * 10];     // states [item][vp(5) vm(5)]
-  __shared__ double sFx[NFX * 7];      // x-fluxes [face][flux(5) press cmax]
+  // LDS arrays are component-major so that consecutive lanes touch
+  // consecutive doubles (no bank conflicts).
+  __shared__ double sPlane[NPLANE];    // V plane [nv][LH][LW]; later y-fluxes [7][NFY]
+  __shared__ double sSt[NST * 10];     // states [vp(5) vm(5)][item]
+  __shared__ double sFx[NFX * 7];      // x-fluxes [flux(5) press cmax][face]
   __shared__ double sRed[NT / 64];
 
   const int tid = threadIdx.x;
@@ -214,7 +223,7 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
     // U of this stage and V at the start of the step, needed by the update
     double u[NV], v0[NV];
     const long long o = ((long long)k * a.ny + (valid ? j : 0)) * a.nx + (valid ? i : 0);
-    if (MODE != MODE_FIRST && valid) {
+    if (PMX_PREFETCH_U && MODE != MODE_FIRST && valid) {
 #pragma unroll
       for (int nv = 0; nv < NV; nv++) {
         u[nv] = a.U[o + nv * a.sv];
@@ -254,8 +263,8 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
       cell_states<REC>(s, vp, vm);
 #pragma unroll
       for (int nv = 0; nv < NV; nv++) {
-        sSt[it * 10 + nv] = vp[nv];
-        sSt[it * 10 + 5 + nv] = vm[nv];
+        sSt[nv * NST + it] = vp[nv];
+        sSt[(5 + nv) * NST + it] = vm[nv];
       }
     }
     __syncthreads();
@@ -267,12 +276,12 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
       double vL[NV], vR[NV], F[7];
 #pragma unroll
       for (int nv = 0; nv < NV; nv++) {
-        vL[nv] = sSt[sl * 10 + nv];
-        vR[nv] = sSt[sr * 10 + 5 + nv];
+        vL[nv] = sSt[nv * NST + sl];
+        vR[nv] = sSt[(5 + nv) * NST + sr];
       }
       riemann<SOLVER, 0, DIMS>(vL, vR, gamma, gmm1, F, F[5], F[6]);
 #pragma unroll
-      for (int q = 0; q < 7; q++) sFx[f * 7 + q] = F[q];
+      for (int q = 0; q < 7; q++) sFx[q * NFX + f] = F[q];
     }
     __syncthreads();
 
@@ -291,8 +300,8 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
       cell_states<REC>(s, vp, vm);
 #pragma unroll
       for (int nv = 0; nv < NV; nv++) {
-        sSt[it * 10 + nv] = vp[nv];
-        sSt[it * 10 + 5 + nv] = vm[nv];
+        sSt[nv * NST + it] = vp[nv];
+        sSt[(5 + nv) * NST + it] = vm[nv];
       }
     }
     __syncthreads();
@@ -304,12 +313,12 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
       double vL[NV], vR[NV], F[7];
 #pragma unroll
       for (int nv = 0; nv < NV; nv++) {
-        vL[nv] = sSt[sl * 10 + nv];
-        vR[nv] = sSt[sr * 10 + 5 + nv];
+        vL[nv] = sSt[nv * NST + sl];
+        vR[nv] = sSt[(5 + nv) * NST + sr];
       }
       riemann<SOLVER, 1, DIMS>(vL, vR, gamma, gmm1, F, F[5], F[6]);
 #pragma unroll
-      for (int q = 0; q < 7; q++) sPlane[f * 7 + q] = F[q];
+      for (int q = 0; q < 7; q++) sPlane[q * NFY + f] = F[q];
     }
     __syncthreads();
 
@@ -335,32 +344,39 @@ __global__ __launch_bounds__(NT) void stage_kernel(StageArgs a) {
 
       if (valid) {
         if (MODE == MODE_FIRST) prim_to_cons(vc, u, gmm1);
+        if (!PMX_PREFETCH_U && MODE != MODE_FIRST) {
+#pragma unroll
+          for (int nv = 0; nv < NV; nv++) {
+            u[nv] = a.U[o + nv * a.sv];
+            v0[nv] = a.V0[o + nv * a.sv];
+          }
+        }
         double cdt = 0.0;
         // x
         {
-          const double* Fm = &sFx[(ty * (TX + 1) + tx) * 7];
-          const double* Fp = Fm + 7;
+          const double* Fm = &sFx[ty * (TX + 1) + tx];   // face i-1/2; i+1/2 is Fm[1]
           double scrh = a.dtdx[0][i];
           double rhs[NV];
 #pragma unroll
-          for (int nv = 0; nv < NV; nv++) rhs[nv] = -scrh * (Fp[nv] - Fm[nv]);
-          rhs[VX1] -= scrh * (Fp[5] - Fm[5]);
+          for (int nv = 0; nv < NV; nv++) rhs[nv] = -scrh * (Fm[nv * NFX + 1] - Fm[nv * NFX]);
+          rhs[VX1] -= scrh * (Fm[5 * NFX + 1] - Fm[5 * NFX]);
 #pragma unroll
           for (int nv = 0; nv < NV; nv++) u[nv] += rhs[nv];
-          if (MODE == MODE_FIRST) cdt = 0.0 + 0.5 * (Fm[6] + Fp[6]) * a.inv_dl[0][i] * 1.0;
+          if (MODE == MODE_FIRST)
+            cdt = 0.0 + 0.5 * (Fm[6 * NFX] + Fm[6 * NFX + 1]) * a.inv_dl[0][i] * 1.0;
         }
         // y
         {
-          const double* Fm = &sPlane[(ty * TX + tx) * 7];
-          const double* Fp = Fm + 7 * TX;
+          const double* Fm = &sPlane[ty * TX + tx];       // face j-1/2; j+1/2 is Fm[TX]
           double scrh = a.dtdx[1][j];
           double rhs[NV];
 #pragma unroll
-          for (int nv = 0; nv < NV; nv++) rhs[nv] = -scrh * (Fp[nv] - Fm[nv]);
-          rhs[VX2] -= scrh * (Fp[5] - Fm[5]);
+          for (int nv = 0; nv < NV; nv++) rhs[nv] = -scrh * (Fm[nv * NFY + TX] - Fm[nv * NFY]);
+          rhs[VX2] -= scrh * (Fm[5 * NFY + TX] - Fm[5 * NFY]);
 #pragma unroll
           for (int nv = 0; nv < NV; nv++) u[nv] += rhs[nv];
-          if (MODE == MODE_FIRST) cdt += 0.5 * (Fm[6] + Fp[6]) * a.inv_dl[1][j] * 1.0;
+          if (MODE == MODE_FIRST)
+            cdt += 0.5 * (Fm[6 * NFY] + Fm[6 * NFY + TX]) * a.inv_dl[1][j] * 1.0;
         }
         // z
         if (DIMS == 3) {

@@ -10,6 +10,7 @@ raises.
 from __future__ import annotations
 
 import ctypes
+import os
 import pathlib
 
 import numpy as np
@@ -56,7 +57,7 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = pathlib.Path(path) if path else LIB_PATH
+    p = pathlib.Path(path) if path else pathlib.Path(os.environ.get("PMX_LIB", LIB_PATH))
     if not p.exists():
         raise PmxError(f"{p} is not built: run `python pluto-sirocco_amd/build.py`")
     lib = ctypes.CDLL(str(p))
