@@ -16,6 +16,7 @@ import os
 import pathlib
 import struct
 import subprocess
+import sys
 import tempfile
 
 import numpy as np
@@ -23,6 +24,9 @@ import numpy as np
 HERE = pathlib.Path(__file__).resolve().parent
 REF_DIR = HERE / "_ref"
 LIB_PATH = HERE / "_build" / "liboracle.so"
+PKG_DIR = HERE.parent / "pluto-sirocco_amd"
+if str(PKG_DIR) not in sys.path:
+    sys.path.insert(0, str(PKG_DIR))
 
 
 def build(with_reference: bool | None = None) -> None:
@@ -54,7 +58,7 @@ def read_dump(path: pathlib.Path, shape) -> dict:
 
 
 def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
-                  timeout: float = 3600.0) -> dict:
+                  timeout: float = 3600.0, fluxes=None) -> dict:
     """Run the reference for ``nsteps`` (>= 2) steps from ``V0``.
 
     PLUTO's ``-maxsteps N`` integrates N+1 steps (Src/main.c:233-240), so
@@ -69,6 +73,9 @@ def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
         tmp = pathlib.Path(tmp)
         (tmp / "pluto.ini").write_text(cfg.pluto_ini())
         np.ascontiguousarray(V0, dtype="<f8").tofile(tmp / "ic.bin")
+        if hasattr(cfg, "nangles"):  # line-driven wind: SIROCCO flux files
+            import setups
+            setups.write_flux_files(cfg, tmp, fluxes)
         env = dict(os.environ)
         env["PMX_IC"] = str(tmp / "ic.bin")
         env["PMX_DUMP"] = str(tmp / "dump.bin")
@@ -157,3 +164,85 @@ def run_oracle(cfg, V0: np.ndarray, nsteps: int, dt0: float | None = None,
         raise RuntimeError(f"orc_run failed ({rc})")
     return {"V": V, "U": U, "t": float(t[0]), "dt": float(dt[0]),
             "dt_seq": hist[:nsteps].copy()}
+
+
+# ----------------------------------------------------------------------
+# line-driven wind restatement
+# ----------------------------------------------------------------------
+class _OrcWindCfg(ctypes.Structure):
+    _dp = ctypes.POINTER(ctypes.c_double)
+    _fields_ = [("nx1", ctypes.c_int), ("nx2", ctypes.c_int), ("cooling", ctypes.c_int),
+                ("bc", ctypes.c_int * 4),
+                ("gamma", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("cfl_max_var", ctypes.c_double),
+                ("mu", ctypes.c_double), ("t_iso", ctypes.c_double),
+                ("l_star", ctypes.c_double), ("f_x", ctypes.c_double),
+                ("f_uv", ctypes.c_double), ("t_x", ctypes.c_double),
+                ("krad", ctypes.c_double), ("alpharad", ctypes.c_double),
+                ("cent_mass", ctypes.c_double),
+                ("unit_density", ctypes.c_double), ("unit_length", ctypes.c_double),
+                ("unit_velocity", ctypes.c_double),
+                ("x1", _dp), ("x1l", _dp), ("x1r", _dp), ("dx1", _dp),
+                ("x2", _dp), ("x2l", _dp), ("x2r", _dp), ("dx2", _dp),
+                ("nangles", ctypes.c_int),
+                ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp)]
+
+
+def run_wind_oracle(cfg, V0: np.ndarray, nsteps: int, dt0: float | None = None,
+                    t0: float = 0.0, fluxes=None, grid=None) -> dict:
+    """Run the wind restatement (orc_wind_run) for nsteps steps from step
+    number 0.  ``fluxes`` = (flux_r, flux_t, flux_p) [nangles][1][nt][nr]
+    (default: setups.wind_fluxes); ``grid`` = (radial, polar) dicts with
+    x, xl, xr, dx including ghosts (default: setups.wind_grid)."""
+    import setups
+    L = lib()
+    if not hasattr(L, "_wind_bound"):
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.orc_wind_run.argtypes = [ctypes.POINTER(_OrcWindCfg), dp, ctypes.c_int, dp, dp,
+                                   dp, dp]
+        L.orc_wind_run.restype = ctypes.c_int
+        L._wind_bound = True
+    gr, gt = grid if grid is not None else setups.wind_grid(cfg)
+    fr, ft, fp = fluxes if fluxes is not None else setups.wind_fluxes(cfg)
+    keep = []
+
+    def arr(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return _ptr(a)
+
+    c = _OrcWindCfg()
+    c.nx1, c.nx2, c.cooling = cfg.nx[0], cfg.nx[1], cfg.cooling
+    for s in range(4):
+        c.bc[s] = cfg.bc[s]
+    for name in ("gamma", "cfl", "cfl_max_var", "mu", "t_iso", "l_star", "f_x", "f_uv",
+                 "t_x", "krad", "alpharad", "cent_mass", "unit_density", "unit_length",
+                 "unit_velocity"):
+        setattr(c, name, getattr(cfg, name))
+    c.x1, c.x1l, c.x1r, c.dx1 = arr(gr["x"]), arr(gr["xl"]), arr(gr["xr"]), arr(gr["dx"])
+    c.x2, c.x2l, c.x2r, c.dx2 = arr(gt["x"]), arr(gt["xl"]), arr(gt["xr"]), arr(gt["dx"])
+    c.nangles = cfg.nangles
+    c.flux_r, c.flux_t, c.flux_p = arr(fr), arr(ft), arr(fp)
+    V = np.ascontiguousarray(V0, dtype=np.float64).copy()
+    U = np.empty_like(V)
+    dt = np.array([cfg.first_dt if dt0 is None else dt0])
+    t = np.array([t0])
+    hist = np.empty(max(nsteps, 1))
+    rc = L.orc_wind_run(ctypes.byref(c), _ptr(V), nsteps, _ptr(dt), _ptr(t), _ptr(U),
+                        _ptr(hist))
+    if rc != 0:
+        raise RuntimeError(f"orc_wind_run failed ({rc})")
+    return {"V": V, "U": U, "t": float(t[0]), "dt": float(dt[0]),
+            "dt_seq": hist[:nsteps].copy()}
+
+
+WIND_COUNTERS = ("rkf12_ok", "ck45_calls", "ck45_sub", "ck45_reject", "bl_solve",
+                 "bl_widen", "bl_second", "below_minT", "dt_cfl", "dt_cool", "dt_maxvar",
+                 "c2p_fail", "floor_T")
+
+
+def wind_counters() -> dict:
+    """Branch counters of the last run_wind_oracle call."""
+    out = (ctypes.c_long * len(WIND_COUNTERS))()
+    lib().orc_wind_counters(out, len(WIND_COUNTERS))
+    return dict(zip(WIND_COUNTERS, list(out)))

@@ -21,6 +21,7 @@
 #include <stdlib.h>
 #include <string.h>
 #include "oracle.h"
+#include "hd_internal.h"
 
 #define NV 5
 #define RHO 0
@@ -100,7 +101,7 @@ int orc_cons_to_prim(double gamma, double *u, double *v)
 /* ------------------------------------------------------------------ */
 /* Riemann solvers on one interface                                    */
 /* ------------------------------------------------------------------ */
-static void phys_flux(int n, const double *v, const double *u, double *f)
+void orc_phys_flux(int n, const double *v, const double *u, double *f)
 {
   /* Src/HD/fluxes.c:36-44: normal momentum flux carries no pressure */
   f[RHO] = u[VX1 + n];
@@ -110,7 +111,7 @@ static void phys_flux(int n, const double *v, const double *u, double *f)
   f[ENG] = (u[ENG] + v[PRS])*v[VX1 + n];
 }
 
-static void hllc(double gamma, int n, const double *vL, const double *uL,
+void orc_hllc(double gamma, int n, const double *vL, const double *uL,
                  const double *vR, const double *uR, double *flux,
                  double *press, double *cmax, double *mach)
 {
@@ -121,8 +122,8 @@ static void hllc(double gamma, int n, const double *vL, const double *uL,
   double aL, aR, SL, SR, scrh, vxl, vxr, qL, qR, wL, wR, vs;
   int nv;
 
-  phys_flux(n, vL, uL, fL);
-  phys_flux(n, vR, uR, fR);
+  orc_phys_flux(n, vL, uL, fL);
+  orc_phys_flux(n, vR, uR, fR);
 
   aL = sqrt(a2L);
   aR = sqrt(a2R);
@@ -177,7 +178,7 @@ static void hllc(double gamma, int n, const double *vL, const double *uL,
   }
 }
 
-static void roe(double gamma, int ndim, int n, const double *ql,
+void orc_roe(double gamma, int ndim, int n, const double *ql,
                 const double *uL, const double *qr, const double *uR,
                 double *flux, double *press, double *cmax, double *mach)
 {
@@ -196,8 +197,8 @@ static void roe(double gamma, int ndim, int n, const double *ql,
   b = VX1 + (n + 2)%3;                 /* (n,t,b) = x-y-z, y-z-x, z-x-y   */
 
   memset(Rc, 0, sizeof(Rc));
-  phys_flux(n, ql, uL, fL);
-  phys_flux(n, qr, uR, fR);
+  orc_phys_flux(n, ql, uL, fL);
+  orc_phys_flux(n, qr, uR, fR);
 
   for (nv = NV; nv--;) dv[nv] = qr[nv] - ql[nv];
   s = sqrt(qr[RHO]/ql[RHO]);
@@ -276,9 +277,9 @@ void orc_riemann(int solver, double gamma, int dir, const double *vL,
   orc_prim_to_cons(gamma, vL, uL);
   orc_prim_to_cons(gamma, vR, uR);
   if (solver == ORC_ROE)
-    roe(gamma, 3, dir, vL, uL, vR, uR, flux, press, cmax, &mach);
+    orc_roe(gamma, 3, dir, vL, uL, vR, uR, flux, press, cmax, &mach);
   else
-    hllc(gamma, dir, vL, uL, vR, uR, flux, press, cmax, &mach);
+    orc_hllc(gamma, dir, vL, uL, vR, uR, flux, press, cmax, &mach);
 }
 
 /* ------------------------------------------------------------------ */
@@ -438,10 +439,10 @@ static void update_stage(St *s, double dt)
         orc_prim_to_cons(c->gamma, vp[ip], uL);
         orc_prim_to_cons(c->gamma, vm[ip + 1], uR);
         if (c->solver == ORC_ROE)
-          roe(c->gamma, s->ndim, d, vp[ip], uL, vm[ip + 1], uR, flux[ip],
+          orc_roe(c->gamma, s->ndim, d, vp[ip], uL, vm[ip + 1], uR, flux[ip],
               &press[ip], &cmax[ip], &s->maxMach);
         else
-          hllc(c->gamma, d, vp[ip], uL, vm[ip + 1], uR, flux[ip], &press[ip],
+          orc_hllc(c->gamma, d, vp[ip], uL, vm[ip + 1], uR, flux[ip], &press[ip],
                &cmax[ip], &s->maxMach);
       }
       for (ip = nbeg; ip <= nend; ip++) {           /* rhs.c:188-209 */

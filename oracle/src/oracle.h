@@ -55,6 +55,41 @@ int  orc_cons_to_prim(double gamma, double *u, double *v);
 void orc_riemann(int solver, double gamma, int dir, const double *vL,
                  const double *vR, double *flux, double *press, double *cmax);
 
+/* ------------------------------------------------------------------ */
+/* Line-driven disk wind: 2-D spherical (r, theta), PLM (van Leer,       */
+/* general-grid weights) + HLLC + RK2, one passive tracer, point-mass    */
+/* gravity, SIROCCO-mode line force, split cooling.                      */
+/* V has 6 variables: rho, v_r, v_theta, v_phi, p, tracer.                */
+/* ------------------------------------------------------------------ */
+enum { ORC_COOL_NONE = 0, ORC_COOL_BLONDIN = 1, ORC_COOL_PLAIN = 2 };
+
+typedef struct {
+  int    nx1, nx2;            /* interior cells (r, theta)                    */
+  int    cooling;             /* ORC_COOL_*                                    */
+  int    bc[4];               /* r-beg, r-end, theta-beg, theta-end            */
+  double gamma, cfl, cfl_max_var;
+  double mu, t_iso, l_star, f_x, f_uv, t_x, krad, alpharad, cent_mass;
+  double unit_density, unit_length, unit_velocity;
+  /* grid with 2 ghost cells per side: nx + 4 entries each */
+  const double *x1, *x1l, *x1r, *dx1;
+  const double *x2, *x2l, *x2r, *dx2;
+  int    nangles;             /* 36 */
+  const double *flux_r, *flux_t, *flux_p;   /* [nangles][nx2][nx1]            */
+} orc_wind_cfg;
+
+/* Advance V ([6][nx2][nx1], in/out) by nsteps steps of the reference's
+ * main loop with cooling (Src/main.c:215-337, :383-485, :521-697) starting
+ * at step number 0.  dt/t in/out as in orc_run.  U_out optional. */
+int orc_wind_run(const orc_wind_cfg *cfg, double *V, int nsteps, double *dt,
+                 double *t, double *U_out, double *dt_hist);
+
+/* Branch counters of the last orc_wind_run: rkf12 accepted, ck45 calls,
+ * ck45 substeps, ck45 rejections, BLONDIN solves, bracket widenings,
+ * second (heat-cool root) solves, cells below the 1e4 K cut, dt limited by
+ * CFL / cooling / CFL_max_var, cons->prim failures, temperature floors.
+ * Copies min(n, count) entries; returns the count. */
+int orc_wind_counters(long *out, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -192,3 +192,229 @@ def c2_sedov(n: int = 256) -> HydroConfig:
     return HydroConfig(dims=3, nx=(n, n, n), recon=PPM, limiter=LIM_DEFAULT,
                        solver=HLLC, rk=3, bc=(BC_OUTFLOW,) * 6, cfl=0.3,
                        first_dt=1.e-6)
+
+
+# ----------------------------------------------------------------------
+# Line-driven disk wind (2-D spherical r-theta)
+# ----------------------------------------------------------------------
+NVAR_WIND = 6                 # rho, v_r, v_theta, v_phi, p, tracer
+TRC = 5
+CONST = {  # Src/pluto.h:296-321
+    "amu": 1.66053886e-24, "c": 2.99792458e10, "G": 6.6726e-8, "kB": 1.3806505e-16,
+    "mp": 1.67262171e-24, "Msun": 2.e33, "PI": 3.14159265358979, "sigmaT": 6.6524e-25,
+}
+COOL_NONE, COOL_BLONDIN, COOL_PLAIN = 0, 1, 2
+_COOL_NAMES = {COOL_BLONDIN: "BLONDIN", COOL_PLAIN: "TABULATED"}
+
+
+@dataclasses.dataclass
+class WindConfig:
+    """2-D spherical line-driven wind: PLM (van Leer, general-grid
+    coefficients) + HLLC + RK2, point-mass gravity, SIROCCO-mode line force
+    with 36 angle bins, one passive tracer, and a split cooling step
+    (BLONDIN root solve or the sub-cycled Radiat() ODE integrator)."""
+    nx: tuple = (512, 256)                   # (N_r, N_theta)
+    r_range: tuple = (1.0, 100.0)            # code units, logarithmic grid
+    theta_range: tuple = (0.0, 0.5 * 3.14159265358979)
+    cooling: int = COOL_PLAIN
+    solver: int = HLLC
+    gamma: float = 5.0 / 3.0
+    cfl: float = 0.4
+    cfl_max_var: float = 1.1
+    first_dt: float = 1.e-4
+    bc: tuple = (BC_OUTFLOW, BC_OUTFLOW, BC_REFLECTIVE, BC_REFLECTIVE)
+    mu: float = 0.6
+    t_iso: float = 1.e5
+    l_star: float = 1.e38
+    f_x: float = 0.1
+    f_uv: float = 0.1
+    t_x: float = 1.6e5
+    krad: float = 0.2
+    alpharad: float = -0.6
+    cent_mass: float = 0.6 * 2.e33
+    nangles: int = 36
+    # units of the problem (definitions.h)
+    unit_density: float = 1e-14
+    unit_length: float = 1e9
+    unit_velocity: float = 1e9
+
+    dims = 2
+    nghost = 2
+
+    @property
+    def shape(self) -> tuple:
+        return (NVAR_WIND, 1, self.nx[1], self.nx[0])
+
+    @property
+    def ncells(self) -> int:
+        return self.nx[0] * self.nx[1]
+
+    def reference_problem(self) -> str:
+        return {COOL_BLONDIN: "wind_blondin", COOL_PLAIN: "wind_plain"}[self.cooling]
+
+    def pluto_ini(self) -> str:
+        lines = ["[Grid]", "",
+                 f"X1-grid    1  {self.r_range[0]!r}  {self.nx[0]}  l+  {self.r_range[1]!r}",
+                 f"X2-grid    1  {self.theta_range[0]!r}  {self.nx[1]}  u  "
+                 f"{self.theta_range[1]!r}",
+                 "X3-grid    1  0.0  1  u  1.0",
+                 "", "[Time]", "",
+                 f"CFL              {self.cfl!r}",
+                 f"CFL_max_var      {self.cfl_max_var!r}",
+                 "tstop            1.e30",
+                 f"first_dt         {self.first_dt!r}",
+                 "", "[Solver]", "", f"Solver         {_SOLVER_NAMES[self.solver]}",
+                 "", "[Boundary]", ""]
+        labels = ["X1-beg", "X1-end", "X2-beg", "X2-end"]
+        for s, lab in enumerate(labels):
+            lines.append(f"{lab}        {_BC_NAMES[self.bc[s]]}")
+        lines += ["X3-beg        outflow", "X3-end        outflow",
+                  "", "[Static Grid Output]", "",
+                  "uservar    0",
+                  "dbl       -1.0  -1   single_file",
+                  "flt       -1.0  -1   single_file",
+                  "vtk       -1.0  -1   single_file",
+                  "tab       -1.0  -1", "ppm       -1.0  -1", "png       -1.0  -1",
+                  "log        1000000", "analysis  -1.0  1",
+                  "", "[Parameters]", "",
+                  f"MU             {self.mu!r}", f"T_ISO          {self.t_iso!r}",
+                  f"L_star         {self.l_star!r}", f"f_x            {self.f_x!r}",
+                  f"f_uv           {self.f_uv!r}", f"T_x            {self.t_x!r}",
+                  f"KRAD           {self.krad!r}", f"ALPHARAD       {self.alpharad!r}",
+                  f"CENT_MASS      {self.cent_mass!r}", f"GAMMA_EOS      {self.gamma!r}", ""]
+        return "\n".join(lines)
+
+
+def make_grid(kind: str, xL: float, xR: float, n: int, ng: int):
+    """Cell faces, centres and widths of one direction including ng ghost
+    cells, built like the reference (Src/set_grid.c: MakeGrid 'u' and 'l+'
+    segments, ghost cells copying the adjacent width, x = (xl + xr)/2).
+    Scalar libm arithmetic (math module) keeps the bits identical."""
+    nt = n + 2 * ng
+    xl = [0.0] * (nt + 1)
+    xr = [0.0] * nt
+    dx = [0.0] * nt
+    ib, ie = ng, ng + n - 1
+    if kind == "u":
+        for i in range(ib, ie + 1):
+            dx[i] = (xR - xL) / float(n)
+            xl[i] = xL + float(i - ib) * dx[i]
+            xr[i] = xl[i] + dx[i]
+    elif kind == "l+":
+        dy = math.log10((xR + abs(xL) - xL) / abs(xL))
+        dy /= float(n)
+        xl[ib] = xL
+        for i in range(ib, ie + 1):
+            dx[i] = (xl[i] + abs(xL) - xL) * (math.pow(10.0, dy) - 1.0)
+            xr[i] = xl[i] + dx[i]
+            xl[i + 1] = xr[i]
+    else:
+        raise ValueError(kind)
+    for i in range(ng):
+        dx[i] = dx[ib]
+        xl[i] = xl[ib] - (ng - i) * dx[ib]
+        xr[i] = xl[i] + dx[ib]
+        dx[ie + i + 1] = dx[ie]
+        xl[ie + i + 1] = xl[ie] + (i + 1) * dx[ie]
+        xr[ie + i + 1] = xl[ie] + (i + 2) * dx[ie]
+    xl = xl[:nt]
+    x = [0.5 * (xl[i] + xr[i]) for i in range(nt)]
+    return {"x": np.array(x), "xl": np.array(xl), "xr": np.array(xr), "dx": np.array(dx)}
+
+
+def wind_grid(cfg: WindConfig):
+    return (make_grid("l+", cfg.r_range[0], cfg.r_range[1], cfg.nx[0], cfg.nghost),
+            make_grid("u", cfg.theta_range[0], cfg.theta_range[1], cfg.nx[1], cfg.nghost))
+
+
+def wind_ic(cfg: WindConfig, seed: int = 1234, rho0_cgs: float = 1.e-12,
+            temp: float = 1.e5, noise: float = 0.01) -> np.ndarray:
+    """rho = rho0 r^-2 (1 + noise (2u-1)), T = temp (p = rho T / (KELVIN mu)),
+    Keplerian rotation v_phi = sqrt(G M / r) sin(theta), tracer = 1."""
+    gr, gt = wind_grid(cfg)
+    g = cfg.nghost
+    r = gr["x"][g:-g][None, None, :]
+    th = gt["x"][g:-g][None, :, None]
+    kelvin = cfg.unit_velocity * cfg.unit_velocity * CONST["amu"] / CONST["kB"]
+    nz, ny, nx = 1, cfg.nx[1], cfg.nx[0]
+    idx = np.arange(ny * nx, dtype=np.uint64).reshape(1, ny, nx)
+    V = np.zeros(cfg.shape)
+    V[RHO] = rho0_cgs / cfg.unit_density * r ** -2 * (1.0 + noise * (2.0 * uniform01(seed, idx) - 1.0))
+    gm_code = CONST["G"] * cfg.cent_mass / (cfg.unit_length * cfg.unit_velocity ** 2)
+    V[VX3] = np.sqrt(gm_code / r) * np.sin(th) * np.ones_like(V[RHO])
+    V[PRS] = V[RHO] * temp / (kelvin * cfg.mu)
+    V[TRC] = 1.0
+    return V
+
+
+def wind_ic_mixed(cfg: WindConfig, seed: int = 77, rho0_cgs: float = 1.e-10,
+                  rho_decades: float = 2.0, t_range: tuple = (3.e3, 3.e7),
+                  v_cgs: float = 3.e7) -> np.ndarray:
+    """Stress state for the wind step: density rho0 r^-2 times a log-uniform
+    factor spanning +-rho_decades/2 decades, temperature log-uniform over
+    t_range (cells below 1e4 K and stiffly cooling cells both occur),
+    random poloidal velocities up to v_cgs, Keplerian rotation, tracer in
+    {0, 1} so the Radiat() tracer gate takes both branches."""
+    gr, gt = wind_grid(cfg)
+    g = cfg.nghost
+    r = gr["x"][g:-g][None, None, :]
+    th = gt["x"][g:-g][None, :, None]
+    kelvin = cfg.unit_velocity * cfg.unit_velocity * CONST["amu"] / CONST["kB"]
+    ny, nx = cfg.nx[1], cfg.nx[0]
+    n = ny * nx
+    idx = np.arange(n, dtype=np.uint64).reshape(1, ny, nx)
+    u = [uniform01(seed + k, idx) for k in range(5)]
+    V = np.zeros(cfg.shape)
+    V[RHO] = rho0_cgs / cfg.unit_density * r ** -2 * 10.0 ** (rho_decades * (u[0] - 0.5))
+    lt0, lt1 = np.log10(t_range[0]), np.log10(t_range[1])
+    T = 10.0 ** (lt0 + (lt1 - lt0) * u[1])
+    V[PRS] = V[RHO] * T / (kelvin * cfg.mu)
+    V[VX1] = v_cgs / cfg.unit_velocity * (2.0 * u[2] - 1.0)
+    V[VX2] = v_cgs / cfg.unit_velocity * (2.0 * u[3] - 1.0)
+    gm_code = CONST["G"] * cfg.cent_mass / (cfg.unit_length * cfg.unit_velocity ** 2)
+    V[VX3] = np.sqrt(gm_code / r) * np.sin(th) * np.ones_like(V[RHO])
+    V[TRC] = (u[4] > 0.1).astype(np.float64)
+    return V
+
+
+def wind_fluxes(cfg: WindConfig):
+    """Synthetic 36-bin UV flux field (cgs) standing in for SIROCCO's
+    directional_flux_{r,theta,phi}.dat: a central source L_uv/(4 pi r^2)
+    spread over all bins with weight ~ max(cos(theta_a - theta), 0)^2 + 0.05,
+    theta_a = (a + 1/2) 2 pi / 36.  Returns (flux_r, flux_t, flux_p), each
+    [nangles][1][N_theta][N_r]."""
+    gr, gt = wind_grid(cfg)
+    g = cfg.nghost
+    r = gr["x"][g:-g][None, :] * cfg.unit_length
+    th = gt["x"][g:-g][:, None]
+    luv = cfg.l_star * cfg.f_uv
+    f0 = luv / (4.0 * CONST["PI"] * r * r)
+    a = (np.arange(cfg.nangles) + 0.5) * (2.0 * CONST["PI"]) / cfg.nangles
+    w = np.maximum(np.cos(a[:, None, None] - th[None]), 0.0) ** 2 + 0.05
+    w = w / w.sum(axis=0, keepdims=True)
+    fr = f0[None] * w * np.cos(a[:, None, None] - th[None])
+    ft = f0[None] * w * np.sin(a[:, None, None] - th[None])
+    fp = np.zeros_like(fr)
+    return fr[:, None], ft[:, None], fp[:, None]
+
+
+def write_flux_files(cfg: WindConfig, directory, fluxes=None) -> None:
+    """Write directional_flux_{r,theta,phi}.dat in the format the reference
+    parses (Src/LineDriven/line_connect.c:84-165)."""
+    import pathlib
+    directory = pathlib.Path(directory)
+    fr, ft, fp = fluxes if fluxes is not None else wind_fluxes(cfg)
+    gr, gt = wind_grid(cfg)
+    g = cfg.nghost
+    xs, ts = gr["x"][g:-g], gt["x"][g:-g]
+    for name, arr in (("r", fr), ("theta", ft), ("phi", fp)):
+        with open(directory / f"directional_flux_{name}.dat", "w") as f:
+            f.write(f"# synthetic directional flux ({name})\n# nangles {cfg.nangles}\n")
+            for j in range(cfg.nx[1]):
+                for i in range(cfg.nx[0]):
+                    vals = " ".join(f"{v:.17e}" for v in arr[:, 0, j, i])
+                    f.write(f"{i} {j} 0 {xs[i] * cfg.unit_length:.17e} {ts[j]:.17e} {vals}\n")
+
+
+def c3_wind(nr: int = 512, nt: int = 256, cooling: int = COOL_PLAIN) -> WindConfig:
+    return WindConfig(nx=(nr, nt), cooling=cooling)

@@ -2,7 +2,7 @@
 
 Run in a container that holds the reference sources (/root/reference):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [hydro] [wind]
 
 For every case below the unmodified PLUTO reference (oracle/_ref/<problem>/
 pluto, built by oracle/ref/Makefile) is run from a seeded initial condition;
@@ -58,9 +58,43 @@ def cases():
     yield "kh_ppm_hllc_rk2_mixed_24", p2, setups.kelvin_helmholtz(p2, seed=15), 30
 
 
-def main():
+def wind_cases():
+    """Spherical line-driven wind: benign default IC (both cooling modes) and
+    a mixed IC whose cells hit the stiff CK45 path with rejected sub-steps,
+    cooling- and CFL-limited dt, BLONDIN bracket widening and the second
+    (equilibrium) root solve, sub-1e4 K cells and cons->prim floors."""
+    for cool, tag in ((setups.COOL_BLONDIN, "blondin"), (setups.COOL_PLAIN, "plain")):
+        cfg = setups.WindConfig(nx=(24, 12), cooling=cool)
+        yield f"wind_{tag}_default_24x12", cfg, setups.wind_ic(cfg, seed=21), 10
+    cfg = setups.WindConfig(nx=(24, 12), cooling=setups.COOL_BLONDIN, first_dt=1.e-2)
+    yield "wind_blondin_mixed_24x12", cfg, setups.wind_ic_mixed(cfg, rho0_cgs=1.e-9), 8
+    cfg = setups.WindConfig(nx=(24, 12), cooling=setups.COOL_BLONDIN, first_dt=5.e-2)
+    yield "wind_blondin_mixed_cfl_24x12", cfg, setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11), 8
+    cfg = setups.WindConfig(nx=(24, 12), cooling=setups.COOL_PLAIN, first_dt=1.e-2)
+    yield "wind_plain_mixed_24x12", cfg, setups.wind_ic_mixed(cfg, rho0_cgs=1.e-9), 8
+
+
+def wind_inputs(cfg):
+    """Grid and flux arrays exactly as handed to the reference."""
+    gr, gt = setups.wind_grid(cfg)
+    fr, ft, fp = setups.wind_fluxes(cfg)
+    return {"x1": gr["x"], "x1l": gr["xl"], "x1r": gr["xr"], "dx1": gr["dx"],
+            "x2": gt["x"], "x2l": gt["xl"], "x2r": gt["xr"], "dx2": gt["dx"],
+            "flux_r": fr, "flux_t": ft, "flux_p": fp}
+
+
+def main(which=("hydro", "wind")):
     refrun.build(with_reference=True)
-    for name, cfg, V0, n in cases():
+    for name, cfg, V0, n in (wind_cases() if "wind" in which else ()):
+        inp = wind_inputs(cfg)
+        fl = (inp["flux_r"], inp["flux_t"], inp["flux_p"])
+        r = refrun.run_reference(cfg, V0, n, fluxes=fl)
+        meta = dataclasses.asdict(cfg)
+        np.savez_compressed(OUT / f"{name}.npz", V0=V0, V=r["V"], U=r["U"],
+                            dt_seq=r["dt_seq"], t=np.array(r["t"]), dt=np.array(r["dt"]),
+                            nsteps=np.array(n), config=np.array(json.dumps(meta)), **inp)
+        print(f"{name}: {n} steps, t={r['t']:.6e}, next dt={r['dt']:.6e}")
+    for name, cfg, V0, n in (cases() if "hydro" in which else ()):
         r = refrun.run_reference(cfg, V0, n)
         meta = dataclasses.asdict(cfg)
         np.savez_compressed(OUT / f"{name}.npz", V0=V0, V=r["V"], U=r["U"],
@@ -70,4 +104,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(tuple(sys.argv[1:]) or ("hydro", "wind"))

@@ -1,0 +1,97 @@
+"""CPU tests pinning the line-driven wind restatement (oracle/src/wind.c).
+
+The golden fixtures were written by the compiled, unmodified reference
+(problems wind_blondin / wind_plain, tests/golden/make_golden.py) from the
+grid and flux arrays stored beside them; the oracle must reproduce V, Uc,
+the dt sequence, t and the next dt bit for bit.  Each fixture also states
+which branches of the cooling and time-step logic it reaches, so a fixture
+that silently stops covering (say) the stiff Cash-Karp path fails here.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import golden_io
+import setups
+from oracle import refrun
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _build():
+    refrun.build(with_reference=False)
+
+
+def run_fixture(name):
+    cfg, g = golden_io.load(name)
+    grid = ({"x": g["x1"], "xl": g["x1l"], "xr": g["x1r"], "dx": g["dx1"]},
+            {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
+    r = refrun.run_wind_oracle(cfg, g["V0"], int(g["nsteps"]),
+                               fluxes=(g["flux_r"], g["flux_t"], g["flux_p"]), grid=grid)
+    return cfg, g, r, refrun.wind_counters()
+
+
+@pytest.mark.parametrize("name", golden_io.names("wind"))
+def test_wind_oracle_matches_reference_bitwise(name):
+    cfg, g, r, _ = run_fixture(name)
+    assert np.array_equal(r["V"], g["V"]), np.abs(r["V"] - g["V"]).max()
+    assert np.array_equal(r["U"], g["U"]), np.abs(r["U"] - g["U"]).max()
+    assert np.array_equal(r["dt_seq"], g["dt_seq"])
+    assert r["t"] == float(g["t"]) and r["dt"] == float(g["dt"])
+
+
+# branches each fixture must reach (counter > 0)
+COVERAGE = {
+    "wind_blondin_default_24x12": ["bl_solve", "dt_maxvar"],
+    "wind_plain_default_24x12": ["rkf12_ok", "dt_maxvar"],
+    "wind_blondin_mixed_24x12": ["bl_solve", "bl_widen", "below_minT", "c2p_fail"],
+    "wind_blondin_mixed_cfl_24x12": ["bl_widen", "dt_cfl", "c2p_fail"],
+    "wind_plain_mixed_24x12": ["rkf12_ok", "ck45_calls", "ck45_reject", "dt_cool"],
+}
+
+
+@pytest.mark.parametrize("name", sorted(COVERAGE))
+def test_wind_fixture_branch_coverage(name):
+    _, _, _, cnt = run_fixture(name)
+    missing = [k for k in COVERAGE[name] if cnt[k] == 0]
+    assert not missing, (missing, cnt)
+
+
+def test_wind_fixture_inputs_match_generators():
+    """The stored grid equals the generator's (so GPU tests that build the
+    grid on their host feed the same geometry)."""
+    cfg, g = golden_io.load("wind_plain_default_24x12")
+    gr, gt = setups.wind_grid(cfg)
+    np.testing.assert_allclose(g["x1"], gr["x"], rtol=1e-15)
+    np.testing.assert_allclose(g["x2"], gt["x"], rtol=1e-15)
+
+
+def test_log_grid_closed_form():
+    """'l+' grid: faces r_k = r0 * (r1/r0)^(k/N) for r0 > 0, so consecutive
+    width ratios are constant and the last face lands on r1."""
+    g = setups.make_grid("l+", 1.0, 100.0, 64, 2)
+    xl, xr = g["xl"][2:-2], g["xr"][2:-2]
+    expect = 100.0 ** (np.arange(65) / 64.0)
+    np.testing.assert_allclose(np.r_[xl, xr[-1]], expect, rtol=1e-12)
+    ratio = g["dx"][3:-2] / g["dx"][2:-3]
+    np.testing.assert_allclose(ratio, ratio[0], rtol=1e-12)
+
+
+def test_wind_mass_budget_without_sources():
+    """Mass is conserved up to the boundary fluxes: in a short run from a
+    state at rest (zero radial/polar velocity, reflective polar walls) the
+    total mass sum(rho dV) changes only through the two radial boundaries,
+    and the change is far below one cell's mass."""
+    cfg = setups.WindConfig(nx=(24, 12), cooling=setups.COOL_BLONDIN, first_dt=1e-5)
+    V0 = setups.wind_ic(cfg, seed=3)
+    r = refrun.run_wind_oracle(cfg, V0, 2)
+    gr, gt = setups.wind_grid(cfg)
+    rl, rr = gr["xl"][2:-2], gr["xr"][2:-2]
+    tl, tr = gt["xl"][2:-2], gt["xr"][2:-2]
+    dV = ((rr ** 3 - rl ** 3) / 3.0)[None, :] * np.abs(np.cos(tl) - np.cos(tr))[:, None]
+    m0 = (V0[0, 0] * dV).sum()
+    m1 = (r["V"][0, 0] * dV).sum()
+    assert abs(m1 - m0) / m0 < 1e-3
+    # the grid volume is the closed form (r1^3 - r0^3)/3 * (1 - cos(pi/2))
+    assert math.isclose(dV.sum(), (100.0 ** 3 - 1.0) / 3.0 * (1.0 - math.cos(tr[-1])),
+                        rel_tol=1e-12)
