@@ -103,6 +103,85 @@ int pmx_hydro_stage_timing(pmx_hydro *h, int enable, int reset, double *total_ms
  * the conservative U (for zero-copy consumers such as torch). */
 int pmx_hydro_device_state(pmx_hydro *h, double **V, double **U);
 
+/* ------------------------------------------------------------------ */
+/* Line-driven disk wind on a 2-D spherical (r, theta) grid.             */
+/*                                                                      */
+/* HD + IDEAL with one passive tracer: V = (rho, v_r, v_theta, v_phi, p, */
+/* tracer), U = (rho, m_r, m_theta, m_phi, E, rho*tracer), host arrays   */
+/* A[nv][j][i] (i = radius fastest), interior only.  PLM (van Leer,      */
+/* general-grid weights) + HLLC + RK2, point-mass gravity, the SIROCCO-  */
+/* mode line force and one split cooling step per time step.             */
+/* ------------------------------------------------------------------ */
+#define PMX_COOL_NONE    0
+#define PMX_COOL_BLONDIN 1   /* COOLING BLONDIN: Brent root solve per cell */
+#define PMX_COOL_PLAIN   2   /* cv_idl CoolingSource: RKF12 / Cash-Karp 45 */
+
+typedef struct pmx_wind pmx_wind;
+
+/* Mirrors the reference's set-up of the wind problem: definitions.h
+ * (UNIT_*), pluto.ini ([Time] CFL / CFL_max_var, [Boundary], [Parameters]
+ * MU, L_star, f_x, T_x, KRAD, ALPHARAD, CENT_MASS, GAMMA_EOS) and the
+ * Grid arrays x, xl, xr, dx of both directions WITH 2 ghost cells per side
+ * (nx + 4 entries, Src/structs.h:133-187).  flux_r / flux_t / flux_p are
+ * the SIROCCO directional fluxes (cgs) [nangles][nx2][nx1] that
+ * read_sirocco_fluxes() loads (Src/LineDriven/line_connect.c:43-165). */
+typedef struct {
+  int nx1, nx2;                 /* interior cells (r, theta)            */
+  int cooling;                  /* PMX_COOL_*                           */
+  int bc[4];                    /* r-beg, r-end, theta-beg, theta-end   */
+  double gamma, cfl, cfl_max_var;
+  double mu, l_star, f_x, t_x, krad, alpharad, cent_mass;
+  double unit_density, unit_length, unit_velocity;
+  const double *x1, *x1l, *x1r, *dx1;
+  const double *x2, *x2l, *x2r, *dx2;
+  int nangles;                  /* 1 .. 64 (the reference's files hold 36) */
+  const double *flux_r, *flux_t, *flux_p;
+} pmx_wind_desc;
+
+/* Create a context: uploads geometry, PLM weights and the line-force
+ * tables (the one-time part of VGradCalc, line_connect.c:534-625). */
+int pmx_wind_create(const pmx_wind_desc *desc, pmx_wind **out);
+int pmx_wind_destroy(pmx_wind *w);
+
+/* Primitive state V[6][nx2][nx1] host <-> device. */
+int pmx_wind_set_prim(pmx_wind *w, const double *V);
+int pmx_wind_get_prim(pmx_wind *w, double *V);
+/* Uc after the last RK stage, U[6][nx2][nx1]. */
+int pmx_wind_get_cons(pmx_wind *w, double *U);
+
+/* SIROCCO ionisation parameter and radiation temperature per cell
+ * ([nx2][nx1]), used by the cooling once g_time > 3.  With xi == NULL they
+ * are set from the current state exactly as read_sirocco_heatcool(flag 0)
+ * does at start-up (line_connect.c:291-306): xi = L_x / n_H / r^2,
+ * t_r = T_x. */
+int pmx_wind_set_heatcool(pmx_wind *w, const double *xi, const double *t_r);
+
+/* g_time, g_dt and the step counter g_stepNumber of the next step. */
+int pmx_wind_set_time(pmx_wind *w, double t, double dt, long step);
+/* Also returns Dts->invDt_hyp, Dts->dt_cool and the OR of the cons->prim
+ * flags since the last call (bit 8: a cooling sub-step limit was hit). */
+int pmx_wind_get_time(pmx_wind *w, double *t, double *dt, long *step, double *inv_dt_hyp,
+                      double *dt_cool, int *flags);
+
+/* AdvanceStep (Src/Time_Stepping/rk_step.c:29, Src/prototypes.h:5) for
+ * the wind: both RK2 stages with boundaries, line force, sources and the
+ * stage-1 CFL reduction, using the context's dt.  Asynchronous. */
+int pmx_wind_advance_step(pmx_wind *w);
+/* SplitSource (Src/split_source.c:28, Src/prototypes.h:204): the cooling
+ * step over the context's dt; the plain integrator also lowers dt_cool.
+ * Asynchronous. */
+int pmx_wind_split_source(pmx_wind *w);
+/* nsteps iterations of the reference's main loop with cooling
+ * (Src/main.c:215-337, :383-485, :521-697): Strang order by step parity,
+ * invDt / dt_cool reset and NextTimeStep every second step, all on the
+ * device.  Asynchronous; pmx_wind_sync() waits. */
+int pmx_wind_run(pmx_wind *w, int nsteps);
+int pmx_wind_sync(pmx_wind *w);
+/* Per-kernel timing (HIP events on the context's stream) of the stage and
+ * cooling kernels while enabled: summed milliseconds and launch counts. */
+int pmx_wind_timing(pmx_wind *w, int enable, int reset, double *stage_ms, int *stage_launches,
+                    double *cool_ms, int *cool_launches);
+
 /* Measurement helper (no counterpart in the reference): times `reps`
  * launches of a plain streaming copy of nbytes (multiple of 16) between two
  * fresh device buffers; reports the mean milliseconds per copy.  Used to

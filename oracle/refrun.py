@@ -188,12 +188,8 @@ class _OrcWindCfg(ctypes.Structure):
                 ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp)]
 
 
-def run_wind_oracle(cfg, V0: np.ndarray, nsteps: int, dt0: float | None = None,
-                    t0: float = 0.0, fluxes=None, grid=None) -> dict:
-    """Run the wind restatement (orc_wind_run) for nsteps steps from step
-    number 0.  ``fluxes`` = (flux_r, flux_t, flux_p) [nangles][1][nt][nr]
-    (default: setups.wind_fluxes); ``grid`` = (radial, polar) dicts with
-    x, xl, xr, dx including ghosts (default: setups.wind_grid)."""
+def _wind_struct(cfg, fluxes=None, grid=None):
+    """The orc_wind_cfg of a WindConfig; returns (struct, arrays to keep)."""
     import setups
     L = lib()
     if not hasattr(L, "_wind_bound"):
@@ -201,6 +197,9 @@ def run_wind_oracle(cfg, V0: np.ndarray, nsteps: int, dt0: float | None = None,
         L.orc_wind_run.argtypes = [ctypes.POINTER(_OrcWindCfg), dp, ctypes.c_int, dp, dp,
                                    dp, dp]
         L.orc_wind_run.restype = ctypes.c_int
+        L.orc_wind_cool.argtypes = [ctypes.POINTER(_OrcWindCfg), dp, ctypes.c_double,
+                                    ctypes.c_double, dp, dp, dp]
+        L.orc_wind_cool.restype = ctypes.c_int
         L._wind_bound = True
     gr, gt = grid if grid is not None else setups.wind_grid(cfg)
     fr, ft, fp = fluxes if fluxes is not None else setups.wind_fluxes(cfg)
@@ -223,17 +222,44 @@ def run_wind_oracle(cfg, V0: np.ndarray, nsteps: int, dt0: float | None = None,
     c.x2, c.x2l, c.x2r, c.dx2 = arr(gt["x"]), arr(gt["xl"]), arr(gt["xr"]), arr(gt["dx"])
     c.nangles = cfg.nangles
     c.flux_r, c.flux_t, c.flux_p = arr(fr), arr(ft), arr(fp)
+    return c, keep
+
+
+def run_wind_oracle(cfg, V0: np.ndarray, nsteps: int, dt0: float | None = None,
+                    t0: float = 0.0, fluxes=None, grid=None) -> dict:
+    """Run the wind restatement (orc_wind_run) for nsteps steps from step
+    number 0.  ``fluxes`` = (flux_r, flux_t, flux_p) [nangles][1][nt][nr]
+    (default: setups.wind_fluxes); ``grid`` = (radial, polar) dicts with
+    x, xl, xr, dx including ghosts (default: setups.wind_grid)."""
+    c, keep = _wind_struct(cfg, fluxes, grid)
     V = np.ascontiguousarray(V0, dtype=np.float64).copy()
     U = np.empty_like(V)
     dt = np.array([cfg.first_dt if dt0 is None else dt0])
     t = np.array([t0])
     hist = np.empty(max(nsteps, 1))
-    rc = L.orc_wind_run(ctypes.byref(c), _ptr(V), nsteps, _ptr(dt), _ptr(t), _ptr(U),
-                        _ptr(hist))
+    rc = lib().orc_wind_run(ctypes.byref(c), _ptr(V), nsteps, _ptr(dt), _ptr(t), _ptr(U),
+                            _ptr(hist))
     if rc != 0:
         raise RuntimeError(f"orc_wind_run failed ({rc})")
     return {"V": V, "U": U, "t": float(t[0]), "dt": float(dt[0]),
             "dt_seq": hist[:nsteps].copy()}
+
+
+def run_wind_cool_oracle(cfg, V0: np.ndarray, dt: float, t: float = 0.0, xi=None, t_r=None,
+                         fluxes=None, grid=None) -> dict:
+    """One SplitSource (cooling) call of the restatement on V0."""
+    c, keep = _wind_struct(cfg, fluxes, grid)
+    V = np.ascontiguousarray(V0, dtype=np.float64).copy()
+    dtc = np.array([0.0])
+    xi_p = t_p = None
+    if xi is not None:
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        t_r = np.ascontiguousarray(t_r, dtype=np.float64)
+        xi_p, t_p = _ptr(xi), _ptr(t_r)
+    rc = lib().orc_wind_cool(ctypes.byref(c), _ptr(V), dt, t, xi_p, t_p, _ptr(dtc))
+    if rc != 0:
+        raise RuntimeError(f"orc_wind_cool failed ({rc})")
+    return {"V": V, "dt_cool": float(dtc[0])}
 
 
 WIND_COUNTERS = ("rkf12_ok", "ck45_calls", "ck45_sub", "ck45_reject", "bl_solve",

@@ -83,7 +83,14 @@ typedef struct {
 int orc_wind_run(const orc_wind_cfg *cfg, double *V, int nsteps, double *dt,
                  double *t, double *U_out, double *dt_hist);
 
-/* Branch counters of the last orc_wind_run: rkf12 accepted, ck45 calls,
+/* One SplitSource (cooling) call on V ([6][nx2][nx1], in/out) at time t
+ * with step dt.  xi / t_r ([nx2][nx1]) are the SIROCCO arrays, or NULL for
+ * the start-up values derived from V.  *dt_cool receives the plain
+ * integrator's limit (1e38 if none). */
+int orc_wind_cool(const orc_wind_cfg *cfg, double *V, double dt, double t, const double *xi,
+                  const double *t_r, double *dt_cool);
+
+/* Branch counters of the last orc_wind_run / orc_wind_cool: rkf12 accepted, ck45 calls,
  * ck45 substeps, ck45 rejections, BLONDIN solves, bracket widenings,
  * second (heat-cool root) solves, cells below the 1e4 K cut, dt limited by
  * CFL / cooling / CFL_max_var, cons->prim failures, temperature floors.

@@ -797,14 +797,14 @@ static void split_source(W *w, double dt)
 }
 
 /* ------------------------------------------------------------------ */
-int orc_wind_run(const orc_wind_cfg *c, double *Vin, int nsteps, double *dt, double *t,
-                 double *U_out, double *dt_hist)
+/* Set up the restatement's state from the configuration and the interior
+ * primitive state Vin ([6][nx2][nx1]). */
+static void w_init(W *pw, const orc_wind_cfg *c, const double *Vin)
 {
   W w;
-  int i, j, nv, a, d, step;
+  int i, j, nv, a, d;
   size_t nc;
   memset(&w, 0, sizeof(w));
-  memset(cnt, 0, sizeof(cnt));
   w.c = c;
   w.n1 = c->nx1; w.n2 = c->nx2;
   w.t1 = c->nx1 + 2*NG; w.t2 = c->nx2 + 2*NG;
@@ -859,12 +859,49 @@ int orc_wind_run(const orc_wind_cfg *c, double *Vin, int nsteps, double *dt, dou
       w.tr_s[CI(&w, j, i)] = c->t_x;
     }
 
+  *pw = w;
+}
+
+static void w_free(W *pw)
+{
+  W w = *pw;
+  int d, a;
+  free(w.V); free(w.U); free(w.U0); free(w.Cdt); free(w.xgc1); free(w.rt); free(w.inv_dx1);
+  free(w.xgc2); free(w.s); free(w.sp); free(w.inv_dx2); free(w.dV); free(w.A1); free(w.A2);
+  for (d = 0; d < 2; d++) for (a = 0; a < 6; a++) free(w.pc[d][a]);
+  free(w.fr); free(w.ft); free(w.fp); free(w.roff); free(w.toff); free(w.moff); free(w.dvds);
+  free(w.x11_0); free(w.x11_1); free(w.x22_0); free(w.x22_1); free(w.xi_s); free(w.tr_s);
+}
+
+/* Copy the interior of the state back to [nv][nx2][nx1] arrays. */
+static void w_out(const W *w, double *Vout, double *U_out)
+{
+  int i, j, nv;
+  for (nv = 0; nv < NVW; nv++)
+    for (j = 0; j < w->n2; j++)
+      for (i = 0; i < w->n1; i++) {
+        size_t o = ((size_t)nv*w->n2 + j)*w->n1 + i;
+        if (Vout) Vout[o] = w->V[VI(w, nv, j + NG, i + NG)];
+        if (U_out) U_out[o] = w->U[NVW*CI(w, j + NG, i + NG) + nv];
+      }
+}
+
+int orc_wind_run(const orc_wind_cfg *c, double *Vin, int nsteps, double *dt, double *t,
+                 double *U_out, double *dt_hist)
+{
+  W w;
+  int step;
+  memset(cnt, 0, sizeof(cnt));
+  w_init(&w, c, Vin);
   w.t = *t;
   w.invDt_hyp = 0.0;
   w.dt_cool = 1.e38;
   for (step = 0; step < nsteps; step++) {       /* main.c:215-337 */
+    /* with cooling, invDt / dt_cool are reset and dt recomputed only every
+       second step (main.c:406-415, :329-333); without, every step */
+    int cooling = c->cooling != ORC_COOL_NONE;
     if (dt_hist) dt_hist[step] = *dt;
-    if ((step - 1)%2 == 1) { w.invDt_hyp = 0.0; w.dt_cool = 1.e38; }
+    if (!cooling || (step - 1)%2 == 1) { w.invDt_hyp = 0.0; w.dt_cool = 1.e38; }
     if ((step%2) == 0) {
       advance_step(&w, *dt);
       split_source(&w, *dt);
@@ -874,7 +911,7 @@ int orc_wind_run(const orc_wind_cfg *c, double *Vin, int nsteps, double *dt, dou
     }
     w.t += *dt;
     *t = w.t;
-    if (step%2 == 1) {                          /* NextTimeStep, main.c:521-697 */
+    if (!cooling || step%2 == 1) {              /* NextTimeStep, main.c:521-697 */
       double dt_hyp = 1.0/w.invDt_hyp, dtn;
       dt_hyp *= c->cfl;
       dtn = dt_hyp;
@@ -887,17 +924,29 @@ int orc_wind_run(const orc_wind_cfg *c, double *Vin, int nsteps, double *dt, dou
     }
   }
 
-  for (nv = 0; nv < NVW; nv++)
+  w_out(&w, Vin, U_out);
+  w_free(&w);
+  return 0;
+}
+
+int orc_wind_cool(const orc_wind_cfg *c, double *V, double dt, double t, const double *xi,
+                  const double *t_r, double *dt_cool)
+{
+  W w;
+  int i, j;
+  memset(cnt, 0, sizeof(cnt));
+  w_init(&w, c, V);
+  if (xi)
     for (j = 0; j < w.n2; j++)
       for (i = 0; i < w.n1; i++) {
-        size_t o = ((size_t)nv*w.n2 + j)*w.n1 + i;
-        Vin[o] = w.V[VI(&w, nv, j + NG, i + NG)];
-        if (U_out) U_out[o] = w.U[NVW*CI(&w, j + NG, i + NG) + nv];
+        w.xi_s[CI(&w, j + NG, i + NG)] = xi[(size_t)j*w.n1 + i];
+        w.tr_s[CI(&w, j + NG, i + NG)] = t_r[(size_t)j*w.n1 + i];
       }
-  free(w.V); free(w.U); free(w.U0); free(w.Cdt); free(w.xgc1); free(w.rt); free(w.inv_dx1);
-  free(w.xgc2); free(w.s); free(w.sp); free(w.inv_dx2); free(w.dV); free(w.A1); free(w.A2);
-  for (d = 0; d < 2; d++) for (a = 0; a < 6; a++) free(w.pc[d][a]);
-  free(w.fr); free(w.ft); free(w.fp); free(w.roff); free(w.toff); free(w.moff); free(w.dvds);
-  free(w.x11_0); free(w.x11_1); free(w.x22_0); free(w.x22_1); free(w.xi_s); free(w.tr_s);
+  w.t = t;
+  w.dt_cool = 1.e38;
+  split_source(&w, dt);
+  if (dt_cool) *dt_cool = w.dt_cool;
+  w_out(&w, V, NULL);
+  w_free(&w);
   return 0;
 }

@@ -1,0 +1,890 @@
+// libplutomx: line-driven disk wind on a 2-D spherical grid (see the wind
+// section of include/plutomx.h).
+//
+// Kernels
+//   wind_stage_kernel<S>  one RK2 stage: boundaries (by index mapping, no
+//                         ghost storage), VGradCalc, PLM, HLLC + tracer,
+//                         spherical RHS, geometric / gravity / line-force
+//                         sources, RK combination, cons->prim and the
+//                         stage-1 CFL reduction.  One thread per cell.
+//   wind_cool_kernel<M>   SplitSource cooling (BLONDIN or the plain
+//                         integrator) and the dt_cool reduction.
+//   wind_end_kernel       time update and NextTimeStep (every second step
+//                         with cooling, as the reference).
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/plutomx.h"
+#include "pmx_common.h"
+#include "pmx_wind_math.h"
+
+namespace pmx {
+namespace wind {
+
+constexpr int NG = 2;          // ghost cells of the reference grid (PLM)
+constexpr int BX = 64, BY = 2; // stage-kernel workgroup
+constexpr int NLF = 7;         // line-force tables per (angle, cell)
+enum { LF_F1, LF_F2, LF_SIN, LF_COS, LF_DS, LF_FR, LF_FT };
+
+struct Scalars {
+  double t, dt, inv_dt_hyp, dt_cool;
+  unsigned long long cdt_bits;     // max C_dt of the current step's stage 1
+  unsigned long long dtcool_bits;  // running min of the cooling dt candidates
+  int err;
+  int pad;
+};
+
+// Geometry and line-force tables on the device.  Radial and polar 1-D
+// arrays include the 2 ghost cells (index = interior index + 2).
+struct Tables {
+  const double *x1, *dx1, *x1r, *rt, *inv_dx1, *dVr, *pr;   // pr: [6][nr+4]
+  const double *x2, *dx2, *s, *sp, *dmu, *inv_dx2, *ct, *sin2, *cos2, *pt;
+  const double *lf;            // [NLF][na][nt][nr]
+  const double *f1c, *f2c;     // [nr], [nt]: centre bilinear weights
+  const double *sta, *cta;     // [na]: sin / cos of the bin angle
+};
+
+struct StageArgs {
+  const double* Vin;   // stage input (stencil)
+  const double* Vn;    // V^n (stage 2: U^n = P2C(V^n))
+  double* U;
+  double* Vout;
+  int nr, nt, na;
+  int bc[4];
+  long long sv;
+  Const k;
+  Tables T;
+  Scalars* s;
+};
+
+__device__ __forceinline__ void ld6(const StageArgs& a, int jj, int ii, double* v) {
+  bool fr = false, ft = false;
+  int im = map_bc(ii, a.nr, a.bc[0], a.bc[1], fr);
+  int jm = map_bc(jj, a.nt, a.bc[2], a.bc[3], ft);
+  const double* p = a.Vin + (long long)jm * a.nr + im;
+#pragma unroll
+  for (int nv = 0; nv < NVW; nv++) v[nv] = p[nv * a.sv];
+  if (fr) v[VX1] = -v[VX1];
+  if (ft) v[VX2] = -v[VX2];
+}
+
+__device__ __forceinline__ void ld_vel(const StageArgs& a, int jj, int ii, double& v1,
+                                       double& v2) {
+  bool fr = false, ft = false;
+  int im = map_bc(ii, a.nr, a.bc[0], a.bc[1], fr);
+  int jm = map_bc(jj, a.nt, a.bc[2], a.bc[3], ft);
+  const double* p = a.Vin + (long long)jm * a.nr + im;
+  v1 = p[VX1 * a.sv];
+  v2 = p[VX2 * a.sv];
+  if (fr) v1 = -v1;
+  if (ft) v2 = -v2;
+}
+
+__device__ __forceinline__ void ldc(const double* tab, int n, int idx, double* c) {
+#pragma unroll
+  for (int q = 0; q < 6; q++) c[q] = tab[q * (n + 2 * NG) + idx];
+}
+
+// HLLC at one interface plus the tracer's upwind flux (Src/adv_flux.c:42-52)
+template <int DIR>
+__device__ __forceinline__ void face(const double* vL, const double* vR, const Const& k,
+                                     double* F, double& p, double& cmax) {
+  hllc<DIR>(vL, vR, k.gamma, k.gmm1, F, p, cmax);
+  F[TRC] = F[RHO] * (F[RHO] > 0.0 ? vL[TRC] : vR[TRC]);
+}
+
+// One RK2 stage (Src/Time_Stepping/rk_step.c:117-330 with UpdateStage,
+// Src/Time_Stepping/update_stage.c:35-410, for GEOMETRY SPHERICAL in 2-D).
+template <int STAGE>
+__global__ __launch_bounds__(BX* BY) void wind_stage_kernel(StageArgs a) {
+  const int i = blockIdx.x * BX + threadIdx.x;
+  const int j = blockIdx.y * BY + threadIdx.y;
+  const bool active = i < a.nr && j < a.nt;
+  double cdt = 0.0;
+  int flags = 0;
+  if (active) {
+    const Const& k = a.k;
+    const Tables& T = a.T;
+    const double dt = a.s->dt;
+    const int I = i + NG, J = j + NG;
+    const long long cell = (long long)j * a.nr + i;
+
+    // ---- x1 stencil and reconstruction -------------------------------
+    double vc[NVW], w1[NVW], e1[NVW];
+    ld6(a, j, i, vc);
+    ld6(a, j, i - 1, w1);
+    ld6(a, j, i + 1, e1);
+    double c[6];
+    double vpi[NVW], vmi[NVW];
+    ldc(T.pr, a.nr, I, c);
+    plm_vl_general(w1, vc, e1, c, vpi, vmi);
+
+    // ---- VGradCalc + LineForce for both sweeps (line_connect.c:628-738,
+    //      :815-903): x1 uses vc = (vp + vm)/2, x2 the cell centre ------
+    double g1r = 0.0, g2t = 0.0;
+    {
+      double P1[3][3], P2[3][3];
+#pragma unroll
+      for (int dj = 0; dj < 3; dj++)
+#pragma unroll
+        for (int di = 0; di < 3; di++) ld_vel(a, j + dj - 1, i + di - 1, P1[dj][di], P2[dj][di]);
+      // corner averages, line_connect.c:638-660 (P[dj][di] = V(j+dj-1, i+di-1))
+      double v11_0 = (P1[0][0] + P1[0][1] + P1[1][0] + P1[1][1]) / 4.0;
+      double v11_1 = (P2[0][0] + P2[0][1] + P2[1][0] + P2[1][1]) / 4.0;
+      double v12_0 = (P1[2][0] + P1[1][0] + P1[2][1] + P1[1][1]) / 4.0;
+      double v12_1 = (P2[2][0] + P2[1][0] + P2[2][1] + P2[1][1]) / 4.0;
+      double v22_0 = (P1[2][1] + P1[2][2] + P1[1][2] + P1[1][1]) / 4.0;
+      double v22_1 = (P2[2][1] + P2[2][2] + P2[1][2] + P2[1][1]) / 4.0;
+      double v21_0 = (P1[1][2] + P1[0][2] + P1[0][1] + P1[1][1]) / 4.0;
+      double v21_1 = (P2[1][2] + P2[0][2] + P2[0][1] + P2[1][1]) / 4.0;
+      // bilinear at the cell centre (line_connect.c:746-777)
+      double f1 = T.f1c[i], f2 = T.f2c[j], t1, t2;
+      t1 = (1.0 - f1) * v11_0 + f1 * v21_0;
+      t2 = (1.0 - f1) * v12_0 + f1 * v22_0;
+      double ans0 = (1.0 - f2) * t1 + f2 * t2;
+      t1 = (1.0 - f1) * v11_1 + f1 * v21_1;
+      t2 = (1.0 - f1) * v12_1 + f1 * v22_1;
+      double ans1 = (1.0 - f2) * t1 + f2 * t2;
+      const double sj = T.sin2[J], cj = T.cos2[J];
+      double vx1 = (ans0 * k.UV * sj + ans1 * k.UV * cj);
+      double vz1 = (ans0 * k.UV * cj - ans1 * k.UV * sj);
+      // thermal speeds of the two LineForce calls (line_connect.c:848-866)
+      double vc1[NVW];
+#pragma unroll
+      for (int nv = 0; nv < NVW; nv++) vc1[nv] = 0.5 * (vpi[nv] + vmi[nv]);
+      double rho1 = vc1[RHO] * k.UD, rho2 = vc[RHO] * k.UD;
+      double T1 = vc1[PRS] / vc1[RHO] * k.KELVIN * k.mu;
+      double T2 = vc[PRS] / vc[RHO] * k.KELVIN * k.mu;
+      double vth1 = sqrt((2.0 * C_kB * T1) / C_mp);
+      double vth2 = sqrt((2.0 * C_kB * T2) / C_mp);
+      const long long sa = a.sv * a.na;
+      const double* lf = T.lf + cell;
+      for (int ang = 0; ang < a.na; ang++) {
+        const double* q = lf + (long long)ang * a.sv;
+        double ds = q[LF_DS * sa];
+        double dvds;
+        if (ds == -999) {
+          dvds = -999;
+        } else {
+          double g1 = q[LF_F1 * sa], g2 = q[LF_F2 * sa];
+          t1 = (1.0 - g1) * v11_0 + g1 * v21_0;
+          t2 = (1.0 - g1) * v12_0 + g1 * v22_0;
+          double b0 = (1.0 - g2) * t1 + g2 * t2;
+          t1 = (1.0 - g1) * v11_1 + g1 * v21_1;
+          t2 = (1.0 - g1) * v12_1 + g1 * v22_1;
+          double b1 = (1.0 - g2) * t1 + g2 * t2;
+          double so = q[LF_SIN * sa], co = q[LF_COS * sa];
+          double vx2 = (b0 * k.UV * so + b1 * k.UV * co);
+          double vz2 = (b0 * k.UV * co - b1 * k.UV * so);
+          double st = T.sta[ang], ctt = T.cta[ang];
+          double u1 = st * vx1 + ctt * vz1;
+          double u2 = st * vx2 + ctt * vz2;
+          dvds = fabs((u2 - u1) / ds);
+        }
+        double fr = q[LF_FR * sa], ft = q[LF_FT * sa];
+        g1r += ((force_mult(k, rho1, vth1, dvds) * k.sigma_e * fr / C_c)) / k.UA;
+        g2t += ((force_mult(k, rho2, vth2, dvds) * k.sigma_e * ft / C_c)) / k.UA;
+      }
+    }
+
+    // ---- x1 faces i-1/2 and i+1/2 --------------------------------------
+    double rhs1[NVW], Fm[NVW], Fp[NVW], pm, pp, cm1, cp1;
+    {
+      double w2[NVW], e2[NVW], vpl[NVW], vml[NVW], vpr[NVW], vmr[NVW];
+      ld6(a, j, i - 2, w2);
+      ld6(a, j, i + 2, e2);
+      ldc(T.pr, a.nr, I - 1, c);
+      plm_vl_general(w2, w1, vc, c, vpl, vml);
+      ldc(T.pr, a.nr, I + 1, c);
+      plm_vl_general(vc, e1, e2, c, vpr, vmr);
+      face<0>(vpl, vmi, k, Fm, pm, cm1);
+      face<0>(vpi, vmr, k, Fp, pp, cp1);
+    }
+    {
+      const double dmu = T.dmu[J];
+      double xrm = T.x1r[I - 1], xrp = T.x1r[I];
+      double Am = xrm * xrm * dmu, Ap = xrp * xrp * dmu;
+      double fAm[NVW], fAp[NVW];
+#pragma unroll
+      for (int nv = 0; nv < NVW; nv++) { fAm[nv] = Fm[nv] * Am; fAp[nv] = Fp[nv] * Ap; }
+      fAm[VX3] *= fabs(xrm);
+      fAp[VX3] *= fabs(xrp);
+      double dtdV = dt / (T.dVr[I] * dmu);
+      double dtdl = dt / T.dx1[I];
+#pragma unroll
+      for (int nv = 0; nv < NVW; nv++) rhs1[nv] = -dtdV * (fAp[nv] - fAm[nv]);
+      rhs1[VX1] -= dtdl * (pp - pm);
+      rhs1[VX3] /= fabs(T.x1[I]);
+      // rhs_source.c:227-297
+      double vs[NVW];
+#pragma unroll
+      for (int nv = 0; nv < NVW; nv++) vs[nv] = 0.5 * (vpi[nv] + vmi[nv]);
+      double r_1 = 1.0 / T.x1[I];
+      double vphi = vs[VX3];
+      double Sm = vs[RHO] * (vs[VX2] * vs[VX2] + vphi * vphi);
+      Sm += -0.0 - 0.0;
+      rhs1[VX1] += dt * Sm * r_1;
+      double g0 = -1.0 * k.gm_code / (T.x1[I] * T.x1[I]);
+      rhs1[VX1] += dt * vs[RHO] * g0;
+      rhs1[ENG] += dt * 0.5 * (Fp[RHO] + Fm[RHO]) * g0;
+      rhs1[VX1] += dt * vs[RHO] * g1r;
+      rhs1[ENG] += dt * 0.5 * (Fp[RHO] + Fm[RHO]) * g1r;
+    }
+
+    // ---- x2 faces j-1/2 and j+1/2 --------------------------------------
+    double rhs2[NVW], Gm[NVW], Gp[NVW], qm, qp, cm2, cp2;
+    {
+      double s1[NVW], n1[NVW], s2[NVW], n2[NVW];
+      ld6(a, j - 1, i, s1);
+      ld6(a, j + 1, i, n1);
+      ld6(a, j - 2, i, s2);
+      ld6(a, j + 2, i, n2);
+      double vpc[NVW], vmc[NVW], vpl[NVW], vml[NVW], vpr[NVW], vmr[NVW];
+      ldc(T.pt, a.nt, J, c);
+      plm_vl_general(s1, vc, n1, c, vpc, vmc);
+      ldc(T.pt, a.nt, J - 1, c);
+      plm_vl_general(s2, s1, vc, c, vpl, vml);
+      ldc(T.pt, a.nt, J + 1, c);
+      plm_vl_general(vc, n1, n2, c, vpr, vmr);
+      face<1>(vpl, vmc, k, Gm, qm, cm2);
+      face<1>(vpc, vmr, k, Gp, qp, cp2);
+    }
+    {
+      double base = fabs(T.x1[I]) * T.dx1[I];
+      double Am = base * T.sp[J - 1], Ap = base * T.sp[J];
+      double fAm[NVW], fAp[NVW];
+#pragma unroll
+      for (int nv = 0; nv < NVW; nv++) { fAm[nv] = Gm[nv] * Am; fAp[nv] = Gp[nv] * Ap; }
+      fAm[VX3] *= fabs(T.sp[J - 1]);
+      fAp[VX3] *= fabs(T.sp[J]);
+      double dtdV = dt / (T.dVr[I] * T.dmu[J]);
+      double dtdl = dt / T.dx2[J] * (1.0 / T.rt[I]);
+#pragma unroll
+      for (int nv = 0; nv < NVW; nv++) rhs2[nv] = -dtdV * (fAp[nv] - fAm[nv]);
+      rhs2[VX2] -= dtdl * (qp - qm);
+      rhs2[VX3] /= fabs(T.s[J]);
+      // rhs_source.c:313-396
+      double r_1 = 1.0 / T.rt[I], ct = T.ct[J];
+      double vphi = vc[VX3];
+      double Sm = vc[RHO] * (-vc[VX2] * vc[VX1] + ct * vphi * vphi);
+      Sm += 0.0 - ct * 0.0;
+      rhs2[VX2] += dt * Sm * r_1;
+      const double g1 = 0.0, g2 = 0.0;
+      rhs2[VX2] += dt * vc[RHO] * g1;
+      rhs2[ENG] += dt * 0.5 * (Gp[RHO] + Gm[RHO]) * g1;
+      rhs2[VX3] += dt * vc[RHO] * g2;
+      rhs2[ENG] += dt * vc[RHO] * vc[VX3] * g2;
+      const double l2 = 0.0;
+      rhs2[VX2] += dt * vc[RHO] * g2t;
+      rhs2[ENG] += dt * 0.5 * (Gp[RHO] + Gm[RHO]) * g2t;
+      rhs2[VX3] += dt * vc[RHO] * l2;
+      rhs2[ENG] += dt * vc[RHO] * vc[VX3] * l2;
+    }
+
+    // ---- update, RK2 combination, cons->prim ---------------------------
+    double u[NVW];
+    if (STAGE == 1) {
+      p2c6(vc, u, k.gmm1);
+    } else {
+#pragma unroll
+      for (int nv = 0; nv < NVW; nv++) u[nv] = a.U[nv * a.sv + cell];
+    }
+#pragma unroll
+    for (int nv = 0; nv < NVW; nv++) u[nv] += rhs1[nv];
+#pragma unroll
+    for (int nv = 0; nv < NVW; nv++) u[nv] += rhs2[nv];
+    if (STAGE == 2) {
+      double vn[NVW], u0[NVW];
+#pragma unroll
+      for (int nv = 0; nv < NVW; nv++) vn[nv] = a.Vn[nv * a.sv + cell];
+      p2c6(vn, u0, k.gmm1);
+#pragma unroll
+      for (int nv = 0; nv < NVW; nv++) u[nv] = 0.5 * u0[nv] + 0.5 * u[nv];
+    }
+    double vo[NVW];
+    flags = c2p6(u, vo, k.gmm1);
+#pragma unroll
+    for (int nv = 0; nv < NVW; nv++) {
+      a.U[nv * a.sv + cell] = u[nv];
+      a.Vout[nv * a.sv + cell] = vo[nv];
+    }
+    if (STAGE == 1) {   // update_stage.c:303-316
+      cdt = 0.0;
+      cdt += 0.5 * (cm1 + cp1) * T.inv_dx1[I] * 1.0;
+      cdt += 0.5 * (cm2 + cp2) * (T.inv_dx2[J] * (1.0 / T.x1[I])) * 1.0;
+    }
+  }
+  // block reductions: max C_dt (stage 1), OR of the cons->prim flags
+  __shared__ double smax[BX * BY / 64];
+  __shared__ int sflag[BX * BY / 64];
+  const int tid = threadIdx.y * BX + threadIdx.x;
+  for (int off = 32; off > 0; off >>= 1) {
+    cdt = dmax(cdt, __shfl_xor(cdt, off));
+    flags |= __shfl_xor(flags, off);
+  }
+  if ((tid & 63) == 0) { smax[tid >> 6] = cdt; sflag[tid >> 6] = flags; }
+  __syncthreads();
+  if (tid == 0) {
+    double m = smax[0];
+    int f = sflag[0];
+    for (int q = 1; q < BX * BY / 64; q++) { m = dmax(m, smax[q]); f |= sflag[q]; }
+    if (STAGE == 1 && m > 0.0)
+      atomicMax(&a.s->cdt_bits, (unsigned long long)__double_as_longlong(m));
+    if (f) atomicOr(&a.s->err, f);
+    if (STAGE == 2 && blockIdx.x == 0 && blockIdx.y == 0) {
+      // update_stage.c:389-394: invDt = MAX(invDt, C_dt) over the domain,
+      // then / DIMENSIONS (stage-1 values, complete when stage 2 runs)
+      double inv = a.s->inv_dt_hyp;
+      inv = dmax(inv, __longlong_as_double((long long)a.s->cdt_bits));
+      inv /= (double)2;
+      a.s->inv_dt_hyp = inv;
+      a.s->cdt_bits = 0ull;
+    }
+  }
+}
+
+struct CoolArgs {
+  double* V;
+  int nr, nt;
+  long long sv;
+  Const k;
+  const double* x1;      // ghost-inclusive radial centres
+  const double* xi_s;
+  const double* tr_s;
+  Scalars* s;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(128) void wind_cool_kernel(CoolArgs a) {
+  const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  double cand = 1.0e300;
+  int bad = 0;
+  if (cell < a.sv) {
+    const int i = (int)(cell % a.nr);
+    const double t = a.s->t, dt = a.s->dt;
+    double rho = a.V[RHO * a.sv + cell], p = a.V[PRS * a.sv + cell];
+    double pn;
+    if (MODE == PMX_COOL_BLONDIN) {
+      pn = blondin_cell(a.k, rho, p, a.x1[i + NG], a.xi_s[cell], a.tr_s[cell], t, dt);
+    } else {
+      RadCell R;
+      R.rho = rho;
+      R.trc = a.V[TRC * a.sv + cell];
+      R.x1 = a.x1[i + NG];
+      R.xi_s = a.xi_s[cell];
+      R.tr_s = a.tr_s[cell];
+      R.t = t;
+      pn = plain_cell(a.k, R, p, dt, cand, bad);
+    }
+    a.V[PRS * a.sv + cell] = pn;
+  }
+  if (MODE == PMX_COOL_PLAIN) {
+    __shared__ double smin[2];
+    __shared__ int sbad[2];
+    for (int off = 32; off > 0; off >>= 1) {
+      cand = dmin(cand, __shfl_xor(cand, off));
+      bad |= __shfl_xor(bad, off);
+    }
+    if ((threadIdx.x & 63) == 0) { smin[threadIdx.x >> 6] = cand; sbad[threadIdx.x >> 6] = bad; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double m = dmin(smin[0], smin[1]);
+      if (m >= 0.0)
+        atomicMin(&a.s->dtcool_bits, (unsigned long long)__double_as_longlong(m));
+      if (sbad[0] | sbad[1]) atomicOr(&a.s->err, 256);
+    }
+  }
+}
+
+// Src/main.c:296 (t += dt) and NextTimeStep (main.c:521-697) followed by
+// the invDt / dt_cool reset that the next step performs first
+// (main.c:406-415).  With cooling both happen every second step
+// (main.c:329-333), without every step: `next` says whether this is one.
+__global__ void wind_end_kernel(Scalars* s, int next, double cfl, double cfl_max_var) {
+  double dt = s->dt;
+  s->t += dt;
+  double dt_cool = __longlong_as_double((long long)s->dtcool_bits);
+  s->dt_cool = dt_cool;
+  if (next) {
+    double dt_hyp = 1.0 / s->inv_dt_hyp;
+    dt_hyp *= cfl;
+    double dtn = dt_hyp;
+    dtn = dmin(dtn, dt_cool);
+    dtn = dmin(dtn, cfl_max_var * dt);
+    if (cfl_max_var != 1.0) s->dt = dtn;
+    s->inv_dt_hyp = 0.0;
+    s->dtcool_bits = (unsigned long long)__double_as_longlong(1.e38);
+  }
+}
+
+__global__ void heatcool_from_state_kernel(const double* V, int nr, long long sv,
+                                           const double* x1, Const k, double* xi_s,
+                                           double* tr_s) {
+  const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= sv) return;
+  const int i = (int)(cell % nr);
+  double rho = V[RHO * sv + cell] * k.UD, r = x1[i + NG] * k.UL;
+  double nH = rho / (1.43 * C_mp);
+  xi_s[cell] = k.lx / nH / (r * r);
+  tr_s[cell] = k.t_x;
+}
+
+}  // namespace wind
+}  // namespace pmx
+
+using namespace pmx;
+using namespace pmx::wind;
+
+struct pmx_wind {
+  int nr, nt, na, cooling;
+  int bc[4];
+  double cfl, cfl_max_var;
+  long long sv;
+  Const k;
+  double* V[2];       // V[0] = V^n (state), V[1] = stage-1 result
+  double* U;
+  double* xi_s;
+  double* tr_s;
+  double* tab;        // one allocation for all geometry tables
+  double* lf;
+  Tables T;
+  Scalars* s;
+  long step;
+  hipStream_t stream;
+  bool timing;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pend_stage, pend_cool;
+  double stage_ms, cool_ms;
+  int stage_n, cool_n;
+};
+
+namespace {
+
+constexpr double C_AH = 1.008, C_AHe = 4.004, C_AZ = 30.0;   // Src/pluto.h:296-301
+constexpr double C_G = 6.6726e-8, C_PI = 3.14159265358979, C_sigmaT = 6.6524e-25;
+
+int timed_begin(pmx_wind* w, hipEvent_t* e) {
+  if (!w->timing) return 0;
+  if (w->ev_pool.size() < 2) {
+    for (int q = 0; q < 64; q++) {
+      hipEvent_t ev;
+      PMX_HIP_CHECK(hipEventCreate(&ev));
+      w->ev_pool.push_back(ev);
+    }
+  }
+  e[0] = w->ev_pool.back(); w->ev_pool.pop_back();
+  e[1] = w->ev_pool.back(); w->ev_pool.pop_back();
+  PMX_HIP_CHECK(hipEventRecord(e[0], w->stream));
+  return 0;
+}
+
+int timed_end(pmx_wind* w, hipEvent_t* e, bool cool) {
+  if (!w->timing) return 0;
+  PMX_HIP_CHECK(hipEventRecord(e[1], w->stream));
+  (cool ? w->pend_cool : w->pend_stage).push_back({e[0], e[1]});
+  return 0;
+}
+
+int collect(pmx_wind* w) {
+  for (int c = 0; c < 2; c++) {
+    auto& v = c ? w->pend_cool : w->pend_stage;
+    for (auto& p : v) {
+      float ms = 0.f;
+      PMX_HIP_CHECK(hipEventSynchronize(p.second));
+      PMX_HIP_CHECK(hipEventElapsedTime(&ms, p.first, p.second));
+      if (c) { w->cool_ms += ms; w->cool_n++; } else { w->stage_ms += ms; w->stage_n++; }
+      w->ev_pool.push_back(p.first);
+      w->ev_pool.push_back(p.second);
+    }
+    v.clear();
+  }
+  return 0;
+}
+
+StageArgs stage_args(pmx_wind* w, const double* Vin, double* Vout) {
+  StageArgs a;
+  a.Vin = Vin;
+  a.Vn = w->V[0];
+  a.U = w->U;
+  a.Vout = Vout;
+  a.nr = w->nr; a.nt = w->nt; a.na = w->na;
+  for (int q = 0; q < 4; q++) a.bc[q] = w->bc[q];
+  a.sv = w->sv;
+  a.k = w->k;
+  a.T = w->T;
+  a.s = w->s;
+  return a;
+}
+
+int advance(pmx_wind* w) {
+  dim3 grid((w->nr + BX - 1) / BX, (w->nt + BY - 1) / BY), block(BX, BY);
+  hipEvent_t e[2];
+  if (timed_begin(w, e)) return -1;
+  hipLaunchKernelGGL(wind_stage_kernel<1>, grid, block, 0, w->stream,
+                     stage_args(w, w->V[0], w->V[1]));
+  PMX_HIP_CHECK(hipGetLastError());
+  if (timed_end(w, e, false)) return -1;
+  if (timed_begin(w, e)) return -1;
+  hipLaunchKernelGGL(wind_stage_kernel<2>, grid, block, 0, w->stream,
+                     stage_args(w, w->V[1], w->V[0]));
+  PMX_HIP_CHECK(hipGetLastError());
+  return timed_end(w, e, false);
+}
+
+int cool(pmx_wind* w) {
+  if (w->cooling == PMX_COOL_NONE) return 0;
+  CoolArgs a;
+  a.V = w->V[0];
+  a.nr = w->nr; a.nt = w->nt;
+  a.sv = w->sv;
+  a.k = w->k;
+  a.x1 = w->T.x1;
+  a.xi_s = w->xi_s;
+  a.tr_s = w->tr_s;
+  a.s = w->s;
+  dim3 grid((unsigned)((w->sv + 127) / 128)), block(128);
+  hipEvent_t e[2];
+  if (timed_begin(w, e)) return -1;
+  if (w->cooling == PMX_COOL_BLONDIN)
+    hipLaunchKernelGGL(wind_cool_kernel<PMX_COOL_BLONDIN>, grid, block, 0, w->stream, a);
+  else
+    hipLaunchKernelGGL(wind_cool_kernel<PMX_COOL_PLAIN>, grid, block, 0, w->stream, a);
+  PMX_HIP_CHECK(hipGetLastError());
+  return timed_end(w, e, true);
+}
+
+// Src/set_geometry.c (SPHERICAL) and Src/States/plm_coeffs.c:30-91 on the
+// ghost-inclusive 1-D arrays, evaluated on the host with the reference's
+// expressions (the trigonometric ones with the host libm, as in the
+// reference).
+void plm_coeffs(const double* x, const double* xr, const double* dx, const double* xgc, int n,
+                double* pc) {
+  for (int q = 0; q < 6 * n; q++) pc[q] = 0.0;
+  for (int i = 1; i <= n - 2; i++) {
+    pc[2 * n + i] = dx[i] / (xgc[i + 1] - xgc[i]);
+    pc[3 * n + i] = dx[i] / (xgc[i] - xgc[i - 1]);
+    pc[0 * n + i] = (xgc[i + 1] - xgc[i]) / (xr[i] - xgc[i]);
+    pc[1 * n + i] = (xgc[i] - xgc[i - 1]) / (xgc[i] - xr[i - 1]);
+    pc[4 * n + i] = (xr[i] - xgc[i]) / dx[i];
+    pc[5 * n + i] = (xgc[i] - xr[i - 1]) / dx[i];
+  }
+  (void)x;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmx_wind_destroy(pmx_wind* w) {
+  if (!w) return 0;
+  if (w->stream) hipStreamSynchronize(w->stream);
+  for (auto e : w->ev_pool) hipEventDestroy(e);
+  for (auto& p : w->pend_stage) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
+  for (auto& p : w->pend_cool) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
+  hipFree(w->V[0]); hipFree(w->V[1]); hipFree(w->U); hipFree(w->xi_s); hipFree(w->tr_s);
+  hipFree(w->tab); hipFree(w->lf); hipFree(w->s);
+  if (w->stream) hipStreamDestroy(w->stream);
+  delete w;
+  return 0;
+}
+
+int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
+  if (!d || !out) { set_error("null argument"); return -1; }
+  if (d->nx1 < 2 || d->nx2 < 2) { set_error("grid smaller than the ghost width"); return -1; }
+  if (d->nangles < 1 || d->nangles > NANG_MAX) { set_error("nangles out of range"); return -1; }
+  if (d->cooling < PMX_COOL_NONE || d->cooling > PMX_COOL_PLAIN) {
+    set_error("unknown cooling mode"); return -1;
+  }
+  for (int q = 0; q < 4; q++)
+    if (d->bc[q] < 0 || d->bc[q] > 2) { set_error("unknown boundary type"); return -1; }
+  if (!d->x1 || !d->x1l || !d->x1r || !d->dx1 || !d->x2 || !d->x2l || !d->x2r || !d->dx2 ||
+      !d->flux_r || !d->flux_t || !d->flux_p) {
+    set_error("missing grid or flux arrays"); return -1;
+  }
+  pmx_wind* w = new pmx_wind();
+  w->nr = d->nx1; w->nt = d->nx2; w->na = d->nangles; w->cooling = d->cooling;
+  for (int q = 0; q < 4; q++) w->bc[q] = d->bc[q];
+  w->cfl = d->cfl; w->cfl_max_var = d->cfl_max_var;
+  w->sv = (long long)w->nr * w->nt;
+  w->step = 0;
+
+  // constants (Src/pluto.h:700-716, main.c, the problem's definitions.h)
+  Const& k = w->k;
+  k.gamma = d->gamma;
+  k.gmm1 = d->gamma - 1.0;
+  k.UD = d->unit_density; k.UL = d->unit_length; k.UV = d->unit_velocity;
+  k.UT = (k.UL / k.UV);
+  k.UP = (k.UD * k.UV * k.UV);
+  k.UA = (k.UV * k.UV / k.UL);
+  k.KELVIN = (k.UV * k.UV * C_amu / C_kB);
+  k.mu = d->mu;
+  {
+    const double FHe = 0.2741 / C_AHe * C_AH / 0.7110;           // Src/pluto.h:718-720
+    const double FZ = (1.0 - 0.7110 - 0.2741) / C_AZ * C_AH / 0.7110;
+    k.mu_tab = (C_AH + FHe * C_AHe + FZ * C_AZ) / (2.0 + 3.0 * FHe + FZ * (1.0 + C_AZ * 0.5));
+  }
+  k.gm_code = C_G * d->cent_mass / (k.UL * k.UV * k.UV);
+  k.sigma_e = C_sigmaT / C_amu / 1.18;
+  k.krad = d->krad; k.alpharad = d->alpharad;
+  k.lx = d->l_star * d->f_x;
+  k.t_x = d->t_x;
+  k.E_cost = k.UL / k.UD / pow(k.UV, 3.0);
+
+  const int tr = w->nr + 2 * NG, tt = w->nt + 2 * NG, nr = w->nr, nt = w->nt, na = w->na;
+  // host tables: radial 6 + 6 PLM rows, polar 9 + 6 PLM rows, centre weights, bin angles
+  std::vector<double> x1(d->x1, d->x1 + tr), x1r(d->x1r, d->x1r + tr), x1l(d->x1l, d->x1l + tr),
+      dx1(d->dx1, d->dx1 + tr);
+  std::vector<double> x2(d->x2, d->x2 + tt), x2r(d->x2r, d->x2r + tt), x2l(d->x2l, d->x2l + tt),
+      dx2(d->dx2, d->dx2 + tt);
+  std::vector<double> rt(tr), inv_dx1(tr), dVr(tr), xgc1(tr), pr(6 * tr);
+  for (int i = 0; i < tr; i++) {
+    double xL = x1l[i], xR = x1r[i], x = x1[i], dx = dx1[i];
+    xgc1[i] = x + 2.0 * x * dx * dx / (12.0 * x * x + dx * dx);
+    rt[i] = (xR * xR * xR - xL * xL * xL) / (xR * xR - xL * xL) / 1.5;
+    inv_dx1[i] = 1.0 / dx;
+    dVr[i] = fabs(xR * xR * xR - xL * xL * xL) / 3.0;
+  }
+  std::vector<double> s(tt), sp(tt), dmu(tt), inv_dx2(tt), ct(tt), sin2(tt), cos2(tt), xgc2(tt),
+      pt(6 * tt);
+  for (int j = 0; j < tt; j++) {
+    double xL = x2l[j], xR = x2r[j];
+    xgc2[j] = sin(xR) - sin(xL) + xL * cos(xL) - xR * cos(xR);
+    xgc2[j] /= cos(xL) - cos(xR);
+    sp[j] = fabs(sin(xR));
+    s[j] = fabs(sin(x2[j]));
+    dmu[j] = fabs(cos(xL) - cos(xR));
+    inv_dx2[j] = 1.0 / dx2[j];
+    ct[j] = 1.0 / tan(x2[j]);
+    sin2[j] = sin(x2[j]);
+    cos2[j] = cos(x2[j]);
+  }
+  plm_coeffs(x1.data(), x1r.data(), dx1.data(), xgc1.data(), tr, pr.data());
+  plm_coeffs(x2.data(), x2r.data(), dx2.data(), xgc2.data(), tt, pt.data());
+
+  // VGradCalc set-up (line_connect.c:534-625): offset point per (bin,
+  // cell), turned into bilinear weights and its trigonometric factors.
+  const double UL = k.UL;
+  std::vector<double> f1c(nr), f2c(nt), sta(na), cta(na);
+  std::vector<double> lf((size_t)NLF * na * w->sv, 0.0);
+  for (int a = 0; a < na; a++) {
+    double ta = (a + 0.5) * (2.0 * C_PI) / 36.0;
+    sta[a] = sin(ta);
+    cta[a] = cos(ta);
+  }
+  for (int i = 0; i < nr; i++) {
+    int I = i + NG;
+    double x11_0 = (x1[I - 1] + x1[I]) / 2.0 * UL, x22_0 = (x1[I + 1] + x1[I]) / 2.0 * UL;
+    f1c[i] = (x1[I] * UL - x11_0) / (x22_0 - x11_0);
+  }
+  for (int j = 0; j < nt; j++) {
+    int J = j + NG;
+    double x11_1 = (x2[J - 1] + x2[J]) / 2.0, x22_1 = (x2[J + 1] + x2[J]) / 2.0;
+    f2c[j] = (x2[J] - x11_1) / (x22_1 - x11_1);
+  }
+  const size_t sa = (size_t)na * w->sv;
+  for (int a = 0; a < na; a++)
+    for (int j = 0; j < nt; j++)
+      for (int i = 0; i < nr; i++) {
+        int I = i + NG, J = j + NG;
+        size_t c = (size_t)j * nr + i, q = (size_t)a * w->sv + c;
+        double x11_0 = (x1[I - 1] + x1[I]) / 2.0 * UL, x11_1 = (x2[J - 1] + x2[J]) / 2.0;
+        double x22_0 = (x1[I + 1] + x1[I]) / 2.0 * UL, x22_1 = (x2[J + 1] + x2[J]) / 2.0;
+        double maxds = fabs(x22_0 - x11_0);
+        if (maxds > (x1[I] * UL * fabs(x22_1 - x11_1))) maxds = x1[I] * UL * fabs(x22_1 - x11_1);
+        maxds /= 2.0;
+        double fr = d->flux_r[q], ft = d->flux_t[q], fp = d->flux_p[q];
+        double mod_flux = sqrt(pow(fr, 2) + pow(ft, 2) + pow(fp, 2));
+        lf[LF_FR * sa + q] = fr;
+        lf[LF_FT * sa + q] = ft;
+        if (mod_flux == 0.0) {
+          lf[LF_DS * sa + q] = -999;
+          continue;
+        }
+        double x = x1[I] * sin(x2[J]) * UL, z = x1[I] * cos(x2[J]) * UL;
+        double ta = (a + 0.5) * (2.0 * C_PI) / 36.0;
+        double ddx1 = maxds * sin(ta), ddx2 = maxds * cos(ta);
+        double ds = sqrt(ddx1 * ddx1 + ddx2 * ddx2);
+        double roff = sqrt((x + ddx1) * (x + ddx1) + (z + ddx2) * (z + ddx2));
+        double toff = atan((x + ddx1) / (z + ddx2));
+        lf[LF_F1 * sa + q] = (roff - x11_0) / (x22_0 - x11_0);
+        lf[LF_F2 * sa + q] = (toff - x11_1) / (x22_1 - x11_1);
+        lf[LF_SIN * sa + q] = sin(toff);
+        lf[LF_COS * sa + q] = cos(toff);
+        lf[LF_DS * sa + q] = ds;
+      }
+
+  // pack the geometry tables into one buffer
+  std::vector<std::pair<const double**, const std::vector<double>*>> parts = {
+      {&w->T.x1, &x1}, {&w->T.dx1, &dx1}, {&w->T.x1r, &x1r}, {&w->T.rt, &rt},
+      {&w->T.inv_dx1, &inv_dx1}, {&w->T.dVr, &dVr}, {&w->T.pr, &pr},
+      {&w->T.x2, &x2}, {&w->T.dx2, &dx2}, {&w->T.s, &s}, {&w->T.sp, &sp}, {&w->T.dmu, &dmu},
+      {&w->T.inv_dx2, &inv_dx2}, {&w->T.ct, &ct}, {&w->T.sin2, &sin2}, {&w->T.cos2, &cos2},
+      {&w->T.pt, &pt}, {&w->T.f1c, &f1c}, {&w->T.f2c, &f2c}, {&w->T.sta, &sta},
+      {&w->T.cta, &cta}};
+  size_t ntab = 0;
+  for (auto& p : parts) ntab += p.second->size();
+  std::vector<double> packed;
+  packed.reserve(ntab);
+  std::vector<size_t> offs;
+  for (auto& p : parts) {
+    offs.push_back(packed.size());
+    packed.insert(packed.end(), p.second->begin(), p.second->end());
+  }
+
+#define CK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { \
+    set_error(std::string(#x) + ": " + hipGetErrorString(e_)); pmx_wind_destroy(w); return -1; } } while (0)
+  CK(hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking));
+  const size_t vbytes = sizeof(double) * NVW * w->sv;
+  CK(hipMalloc(&w->V[0], vbytes));
+  CK(hipMalloc(&w->V[1], vbytes));
+  CK(hipMalloc(&w->U, vbytes));
+  CK(hipMalloc(&w->xi_s, sizeof(double) * w->sv));
+  CK(hipMalloc(&w->tr_s, sizeof(double) * w->sv));
+  CK(hipMalloc(&w->tab, sizeof(double) * packed.size()));
+  CK(hipMalloc(&w->lf, sizeof(double) * lf.size()));
+  CK(hipMalloc(&w->s, sizeof(Scalars)));
+  CK(hipMemcpy(w->tab, packed.data(), sizeof(double) * packed.size(), hipMemcpyHostToDevice));
+  CK(hipMemcpy(w->lf, lf.data(), sizeof(double) * lf.size(), hipMemcpyHostToDevice));
+  CK(hipMemset(w->V[0], 0, vbytes));
+  CK(hipMemset(w->V[1], 0, vbytes));
+  CK(hipMemset(w->U, 0, vbytes));
+  CK(hipMemset(w->xi_s, 0, sizeof(double) * w->sv));
+  CK(hipMemset(w->tr_s, 0, sizeof(double) * w->sv));
+  for (size_t q = 0; q < parts.size(); q++) *parts[q].first = w->tab + offs[q];
+  w->T.lf = w->lf;
+  Scalars s0;
+  memset(&s0, 0, sizeof(s0));
+  s0.dt_cool = 1.e38;
+  s0.dtcool_bits = (unsigned long long)0;
+  memcpy(&s0.dtcool_bits, &s0.dt_cool, sizeof(double));
+  CK(hipMemcpy(w->s, &s0, sizeof(Scalars), hipMemcpyHostToDevice));
+#undef CK
+  *out = w;
+  return 0;
+}
+
+int pmx_wind_set_prim(pmx_wind* w, const double* V) {
+  if (!w || !V) { set_error("null argument"); return -1; }
+  PMX_HIP_CHECK(hipMemcpyAsync(w->V[0], V, sizeof(double) * NVW * w->sv, hipMemcpyHostToDevice,
+                               w->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  return 0;
+}
+
+int pmx_wind_get_prim(pmx_wind* w, double* V) {
+  if (!w || !V) { set_error("null argument"); return -1; }
+  PMX_HIP_CHECK(hipMemcpyAsync(V, w->V[0], sizeof(double) * NVW * w->sv, hipMemcpyDeviceToHost,
+                               w->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  return 0;
+}
+
+int pmx_wind_get_cons(pmx_wind* w, double* U) {
+  if (!w || !U) { set_error("null argument"); return -1; }
+  PMX_HIP_CHECK(hipMemcpyAsync(U, w->U, sizeof(double) * NVW * w->sv, hipMemcpyDeviceToHost,
+                               w->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  return 0;
+}
+
+int pmx_wind_set_heatcool(pmx_wind* w, const double* xi, const double* t_r) {
+  if (!w) { set_error("null argument"); return -1; }
+  if (xi) {
+    if (!t_r) { set_error("t_r missing"); return -1; }
+    PMX_HIP_CHECK(hipMemcpyAsync(w->xi_s, xi, sizeof(double) * w->sv, hipMemcpyHostToDevice,
+                                 w->stream));
+    PMX_HIP_CHECK(hipMemcpyAsync(w->tr_s, t_r, sizeof(double) * w->sv, hipMemcpyHostToDevice,
+                                 w->stream));
+  } else {
+    hipLaunchKernelGGL(heatcool_from_state_kernel, dim3((unsigned)((w->sv + 255) / 256)),
+                       dim3(256), 0, w->stream, w->V[0], w->nr, w->sv, w->T.x1, w->k, w->xi_s,
+                       w->tr_s);
+    PMX_HIP_CHECK(hipGetLastError());
+  }
+  PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  return 0;
+}
+
+int pmx_wind_set_time(pmx_wind* w, double t, double dt, long step) {
+  if (!w) { set_error("null argument"); return -1; }
+  Scalars s0;
+  PMX_HIP_CHECK(hipMemcpyAsync(&s0, w->s, sizeof(Scalars), hipMemcpyDeviceToHost, w->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  s0.t = t;
+  s0.dt = dt;
+  PMX_HIP_CHECK(hipMemcpyAsync(w->s, &s0, sizeof(Scalars), hipMemcpyHostToDevice, w->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  w->step = step;
+  return 0;
+}
+
+int pmx_wind_get_time(pmx_wind* w, double* t, double* dt, long* step, double* inv_dt_hyp,
+                      double* dt_cool, int* flags) {
+  if (!w) { set_error("null argument"); return -1; }
+  Scalars s0;
+  PMX_HIP_CHECK(hipMemcpyAsync(&s0, w->s, sizeof(Scalars), hipMemcpyDeviceToHost, w->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  if (t) *t = s0.t;
+  if (dt) *dt = s0.dt;
+  if (step) *step = w->step;
+  if (inv_dt_hyp) *inv_dt_hyp = s0.inv_dt_hyp;
+  if (dt_cool) {
+    double c;
+    memcpy(&c, &s0.dtcool_bits, sizeof(double));
+    *dt_cool = c;
+  }
+  if (flags) *flags = s0.err;
+  return 0;
+}
+
+int pmx_wind_advance_step(pmx_wind* w) {
+  if (!w) { set_error("null argument"); return -1; }
+  return advance(w);
+}
+
+int pmx_wind_split_source(pmx_wind* w) {
+  if (!w) { set_error("null argument"); return -1; }
+  return cool(w);
+}
+
+int pmx_wind_run(pmx_wind* w, int nsteps) {
+  if (!w || nsteps < 0) { set_error("bad argument"); return -1; }
+  for (int n = 0; n < nsteps; n++) {
+    const long st = w->step;
+    if (st % 2 == 0) {
+      if (advance(w) || cool(w)) return -1;
+    } else {
+      if (cool(w) || advance(w)) return -1;
+    }
+    const int next = (w->cooling == PMX_COOL_NONE) || (st % 2 == 1);
+    hipLaunchKernelGGL(wind_end_kernel, dim3(1), dim3(1), 0, w->stream, w->s, next, w->cfl,
+                       w->cfl_max_var);
+    PMX_HIP_CHECK(hipGetLastError());
+    w->step++;
+  }
+  return 0;
+}
+
+int pmx_wind_sync(pmx_wind* w) {
+  if (!w) { set_error("null argument"); return -1; }
+  PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  return 0;
+}
+
+int pmx_wind_timing(pmx_wind* w, int enable, int reset, double* stage_ms, int* stage_launches,
+                    double* cool_ms, int* cool_launches) {
+  if (!w) { set_error("null argument"); return -1; }
+  if (collect(w)) return -1;
+  if (stage_ms) *stage_ms = w->stage_ms;
+  if (stage_launches) *stage_launches = w->stage_n;
+  if (cool_ms) *cool_ms = w->cool_ms;
+  if (cool_launches) *cool_launches = w->cool_n;
+  if (reset) { w->stage_ms = w->cool_ms = 0.0; w->stage_n = w->cool_n = 0; }
+  w->timing = enable != 0;
+  return 0;
+}
+
+}  // extern "C"

@@ -32,7 +32,11 @@ EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro
            "pmx_hydro_get_prim", "pmx_hydro_get_cons", "pmx_hydro_set_time",
            "pmx_hydro_get_time", "pmx_hydro_advance_step", "pmx_hydro_run",
            "pmx_hydro_sync", "pmx_hydro_run_graph", "pmx_hydro_stage_timing",
-           "pmx_hydro_device_state", "pmx_copy_probe"]
+           "pmx_hydro_device_state", "pmx_copy_probe",
+           "pmx_wind_create", "pmx_wind_destroy", "pmx_wind_set_prim", "pmx_wind_get_prim",
+           "pmx_wind_get_cons", "pmx_wind_set_heatcool", "pmx_wind_set_time",
+           "pmx_wind_get_time", "pmx_wind_advance_step", "pmx_wind_split_source",
+           "pmx_wind_run", "pmx_wind_sync", "pmx_wind_timing"]
 
 
 class PmxError(RuntimeError):
@@ -47,6 +51,23 @@ class HydroDesc(ctypes.Structure):
                 ("dx1", ctypes.POINTER(ctypes.c_double)),
                 ("dx2", ctypes.POINTER(ctypes.c_double)),
                 ("dx3", ctypes.POINTER(ctypes.c_double))]
+
+
+class WindDesc(ctypes.Structure):
+    _dp = ctypes.POINTER(ctypes.c_double)
+    _fields_ = [("nx1", ctypes.c_int), ("nx2", ctypes.c_int), ("cooling", ctypes.c_int),
+                ("bc", ctypes.c_int * 4),
+                ("gamma", ctypes.c_double), ("cfl", ctypes.c_double),
+                ("cfl_max_var", ctypes.c_double),
+                ("mu", ctypes.c_double), ("l_star", ctypes.c_double), ("f_x", ctypes.c_double),
+                ("t_x", ctypes.c_double), ("krad", ctypes.c_double),
+                ("alpharad", ctypes.c_double), ("cent_mass", ctypes.c_double),
+                ("unit_density", ctypes.c_double), ("unit_length", ctypes.c_double),
+                ("unit_velocity", ctypes.c_double),
+                ("x1", _dp), ("x1l", _dp), ("x1r", _dp), ("dx1", _dp),
+                ("x2", _dp), ("x2l", _dp), ("x2r", _dp), ("dx2", _dp),
+                ("nangles", ctypes.c_int),
+                ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp)]
 
 
 _lib = None
@@ -79,6 +100,20 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     lib.pmx_hydro_stage_timing.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ip]
     lib.pmx_hydro_device_state.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
     lib.pmx_copy_probe.argtypes = [ctypes.c_longlong, ctypes.c_int, dp]
+    lp = ctypes.POINTER(ctypes.c_long)
+    lib.pmx_wind_create.argtypes = [ctypes.POINTER(WindDesc), ctypes.POINTER(vp)]
+    lib.pmx_wind_destroy.argtypes = [vp]
+    lib.pmx_wind_set_prim.argtypes = [vp, dp]
+    lib.pmx_wind_get_prim.argtypes = [vp, dp]
+    lib.pmx_wind_get_cons.argtypes = [vp, dp]
+    lib.pmx_wind_set_heatcool.argtypes = [vp, dp, dp]
+    lib.pmx_wind_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_long]
+    lib.pmx_wind_get_time.argtypes = [vp, dp, dp, lp, dp, dp, ip]
+    lib.pmx_wind_advance_step.argtypes = [vp]
+    lib.pmx_wind_split_source.argtypes = [vp]
+    lib.pmx_wind_run.argtypes = [vp, ctypes.c_int]
+    lib.pmx_wind_sync.argtypes = [vp]
+    lib.pmx_wind_timing.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ip, dp, ip]
     for name in EXPORTS:
         if name != "pmx_last_error":
             getattr(lib, name).restype = ctypes.c_int
@@ -198,3 +233,113 @@ class Hydro:
         _check(self._lib.pmx_hydro_stage_timing(self._h, int(enable), int(reset),
                                                 ctypes.byref(ms), ctypes.byref(n)), "timing")
         return ms.value, n.value
+
+
+class Wind:
+    """The line-driven disk wind (2-D spherical) advanced on the GPU.
+
+    ``advance_step()`` is the reference's AdvanceStep for this problem,
+    ``split_source()`` its SplitSource (cooling), ``run(n)`` n iterations of
+    the main loop with cooling, all on the device."""
+
+    def __init__(self, cfg: setups.WindConfig, fluxes=None, grid=None):
+        self.cfg = cfg
+        lib = load_library()
+        gr, gt = grid if grid is not None else setups.wind_grid(cfg)
+        fr, ft, fp = fluxes if fluxes is not None else setups.wind_fluxes(cfg)
+        self._keep = []
+
+        def arr(a):
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            self._keep.append(a)
+            return _dptr(a)
+
+        d = WindDesc()
+        d.nx1, d.nx2, d.cooling = cfg.nx[0], cfg.nx[1], cfg.cooling
+        for q in range(4):
+            d.bc[q] = cfg.bc[q]
+        for name in ("gamma", "cfl", "cfl_max_var", "mu", "l_star", "f_x", "t_x", "krad",
+                     "alpharad", "cent_mass", "unit_density", "unit_length", "unit_velocity"):
+            setattr(d, name, getattr(cfg, name))
+        d.x1, d.x1l, d.x1r, d.dx1 = arr(gr["x"]), arr(gr["xl"]), arr(gr["xr"]), arr(gr["dx"])
+        d.x2, d.x2l, d.x2r, d.dx2 = arr(gt["x"]), arr(gt["xl"]), arr(gt["xr"]), arr(gt["dx"])
+        d.nangles = cfg.nangles
+        d.flux_r, d.flux_t, d.flux_p = arr(fr), arr(ft), arr(fp)
+        h = ctypes.c_void_p()
+        _check(lib.pmx_wind_create(ctypes.byref(d), ctypes.byref(h)), "pmx_wind_create")
+        self._keep = []
+        self._h = h
+        self._lib = lib
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.pmx_wind_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_prim(self, V: np.ndarray, init_heatcool: bool = True) -> None:
+        """Upload V[6][1][nt][nr]; with init_heatcool the SIROCCO xi / T_r
+        arrays are set from this state as the reference does at start-up."""
+        V = np.ascontiguousarray(V, dtype=np.float64)
+        _check(self._lib.pmx_wind_set_prim(self._h, _dptr(V)), "set_prim")
+        if init_heatcool:
+            _check(self._lib.pmx_wind_set_heatcool(self._h, None, None), "set_heatcool")
+
+    def set_heatcool(self, xi: np.ndarray, t_r: np.ndarray) -> None:
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        t_r = np.ascontiguousarray(t_r, dtype=np.float64)
+        _check(self._lib.pmx_wind_set_heatcool(self._h, _dptr(xi), _dptr(t_r)), "set_heatcool")
+
+    def get_prim(self) -> np.ndarray:
+        out = np.empty(self.cfg.shape)
+        _check(self._lib.pmx_wind_get_prim(self._h, _dptr(out)), "get_prim")
+        return out
+
+    def get_cons(self) -> np.ndarray:
+        out = np.empty(self.cfg.shape)
+        _check(self._lib.pmx_wind_get_cons(self._h, _dptr(out)), "get_cons")
+        return out
+
+    def set_time(self, t: float, dt: float, step: int = 0) -> None:
+        _check(self._lib.pmx_wind_set_time(self._h, t, dt, step), "set_time")
+
+    def get_time(self) -> dict:
+        t, dt, inv, dtc = (ctypes.c_double() for _ in range(4))
+        st, fl = ctypes.c_long(), ctypes.c_int()
+        _check(self._lib.pmx_wind_get_time(self._h, ctypes.byref(t), ctypes.byref(dt),
+                                           ctypes.byref(st), ctypes.byref(inv),
+                                           ctypes.byref(dtc), ctypes.byref(fl)), "get_time")
+        return {"t": t.value, "dt": dt.value, "step": st.value, "inv_dt_hyp": inv.value,
+                "dt_cool": dtc.value, "flags": fl.value}
+
+    def advance_step(self) -> None:
+        _check(self._lib.pmx_wind_advance_step(self._h), "advance_step")
+
+    def split_source(self) -> None:
+        _check(self._lib.pmx_wind_split_source(self._h), "split_source")
+
+    def run(self, nsteps: int) -> None:
+        _check(self._lib.pmx_wind_run(self._h, nsteps), "run")
+
+    def sync(self) -> None:
+        _check(self._lib.pmx_wind_sync(self._h), "sync")
+
+    def timing(self, enable: bool, reset: bool = True) -> dict:
+        sm, cm = ctypes.c_double(), ctypes.c_double()
+        sn, cn = ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.pmx_wind_timing(self._h, int(enable), int(reset), ctypes.byref(sm),
+                                         ctypes.byref(sn), ctypes.byref(cm), ctypes.byref(cn)),
+               "timing")
+        return {"stage_ms": sm.value, "stage_launches": sn.value, "cool_ms": cm.value,
+                "cool_launches": cn.value}

@@ -1,0 +1,156 @@
+"""GPU parity tests of the line-driven wind path (libplutomx pmx_wind_*).
+
+Tolerances.  Everything in the wind step that uses only +, -, *, / and
+sqrt is bit-identical to the reference; the only differences come from
+the device math library's pow / exp / log10 / cosh, which can differ from
+the host libm in the last bit.  They enter through the force multiplier
+M = k t^alpha of the line force and through the cooling rates.  So:
+
+* with k = 0 (no force multiplier) and no cooling the GPU step must be
+  bit-identical to the oracle;
+* otherwise every variable must satisfy |gpu - ref| <= 1e-12 |ref| +
+  1e-12 max|ref| (relative error, with the variable's own scale as the
+  floor for components that pass through zero, such as v_r in a wind that
+  starts at rest), and the dt sequence must agree to 1e-12.
+"""
+import numpy as np
+import pytest
+
+import golden_io
+import setups
+from oracle import refrun
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def pmx():
+    import plutomx
+    plutomx.load_library()
+    return plutomx
+
+
+def assert_close(a, b, what, rtol=RTOL):
+    for nv in range(b.shape[0]):
+        x, y = a[nv], b[nv]
+        scale = np.max(np.abs(y))
+        bad = np.abs(x - y) > rtol * np.abs(y) + rtol * scale
+        assert not bad.any(), (what, nv, np.max(np.abs(x - y) / np.maximum(np.abs(y), 1e-300)),
+                               int(bad.sum()))
+
+
+def gpu_run(pmx, cfg, V0, nsteps, fluxes=None, grid=None, stepwise=False):
+    with pmx.Wind(cfg, fluxes=fluxes, grid=grid) as w:
+        w.set_prim(V0)
+        w.set_time(0.0, cfg.first_dt, 0)
+        dts = []
+        for _ in range(nsteps):
+            dts.append(w.get_time()["dt"])
+            w.run(1)
+        w.sync()
+        return w.get_prim(), w.get_cons(), np.array(dts), w.get_time()
+
+
+def fixture_inputs(g):
+    grid = ({"x": g["x1"], "xl": g["x1l"], "xr": g["x1r"], "dx": g["dx1"]},
+            {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
+    return (g["flux_r"], g["flux_t"], g["flux_p"]), grid
+
+
+@pytest.mark.parametrize("name", golden_io.names("wind"))
+def test_wind_gpu_matches_reference_fixture(pmx, name):
+    cfg, g = golden_io.load(name)
+    fluxes, grid = fixture_inputs(g)
+    V, U, dts, tm = gpu_run(pmx, cfg, g["V0"], int(g["nsteps"]), fluxes, grid)
+    assert_close(V, g["V"], name + " V")
+    assert_close(U, g["U"], name + " U")
+    np.testing.assert_allclose(dts, g["dt_seq"], rtol=RTOL, atol=0)
+    assert abs(tm["t"] - float(g["t"])) <= RTOL * float(g["t"])
+    assert abs(tm["dt"] - float(g["dt"])) <= RTOL * float(g["dt"])
+
+
+def test_wind_bitwise_without_force_multiplier(pmx):
+    """k = 0 removes the only libm call from the hydro + line-force step:
+    the GPU must then reproduce the oracle bit for bit (mixed state with
+    floors, 64 x 32 grid, 10 steps)."""
+    import dataclasses
+    cfg = setups.WindConfig(nx=(64, 32), cooling=setups.COOL_NONE, first_dt=1e-3)
+    cfg = dataclasses.replace(cfg, krad=0.0)
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1e-11)
+    o = refrun.run_wind_oracle(cfg, V0, 10)
+    V, U, dts, tm = gpu_run(pmx, cfg, V0, 10)
+    assert np.array_equal(V, o["V"]), np.abs(V - o["V"]).max()
+    assert np.array_equal(U, o["U"]), np.abs(U - o["U"]).max()
+    assert np.array_equal(dts, o["dt_seq"]) and tm["dt"] == o["dt"]
+
+
+@pytest.mark.parametrize("cooling", [setups.COOL_BLONDIN, setups.COOL_PLAIN])
+def test_wind_c3_grid_matches_oracle(pmx, cooling):
+    """The 512 x 256 wind grid of the benchmark workload, 4 steps."""
+    cfg = setups.c3_wind(512, 256, cooling)
+    V0 = setups.wind_ic(cfg)
+    o = refrun.run_wind_oracle(cfg, V0, 4)
+    V, U, dts, tm = gpu_run(pmx, cfg, V0, 4)
+    assert_close(V, o["V"], "V")
+    assert_close(U, o["U"], "U")
+    np.testing.assert_allclose(dts, o["dt_seq"], rtol=RTOL, atol=0)
+
+
+@pytest.mark.parametrize("cooling", [setups.COOL_BLONDIN, setups.COOL_PLAIN])
+@pytest.mark.parametrize("t0", [0.0, 5.0])
+def test_wind_single_cooling_call(pmx, cooling, t0):
+    """SplitSource alone on a mixed state (stiff cells, sub-1e4 K cells,
+    both tracer branches), before and after the g_time = 3 switch to the
+    SIROCCO xi / T_r arrays; dt_cool of the plain integrator too."""
+    cfg = setups.WindConfig(nx=(64, 32), cooling=cooling)
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1e-10)
+    o = refrun.run_wind_cool_oracle(cfg, V0, 1e-2, t0)
+    with pmx.Wind(cfg) as w:
+        w.set_prim(V0)
+        w.set_time(t0, 1e-2, 0)
+        w.split_source()
+        w.sync()
+        V, tm = w.get_prim(), w.get_time()
+    assert np.array_equal(V[[0, 1, 2, 3, 5]], o["V"][[0, 1, 2, 3, 5]])
+    np.testing.assert_allclose(V[4], o["V"][4], rtol=RTOL, atol=0)
+    if cooling == setups.COOL_PLAIN:
+        assert abs(tm["dt_cool"] - o["dt_cool"]) <= RTOL * o["dt_cool"]
+    assert tm["flags"] & 256 == 0
+
+
+def test_wind_granular_calls_match_run(pmx):
+    """advance_step / split_source in the reference's Strang order give the
+    same state as run() (which also keeps dt on the device)."""
+    cfg = setups.WindConfig(nx=(48, 24), cooling=setups.COOL_PLAIN)
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1e-11)
+    with pmx.Wind(cfg) as w:
+        w.set_prim(V0)
+        w.set_time(0.0, 1e-3, 0)
+        w.run(2)
+        w.sync()
+        Va = w.get_prim()
+    with pmx.Wind(cfg) as w:
+        w.set_prim(V0)
+        w.set_time(0.0, 1e-3, 0)
+        w.advance_step()
+        w.split_source()
+        w.sync()
+        w.set_time(1e-3, 1e-3, 1)
+        w.split_source()
+        w.advance_step()
+        w.sync()
+        Vb = w.get_prim()
+    assert np.array_equal(Va, Vb)
+
+
+def test_wind_run_is_deterministic(pmx):
+    cfg = setups.c3_wind(512, 256, setups.COOL_PLAIN)
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1e-11)
+    out = []
+    for _ in range(2):
+        V, U, dts, tm = gpu_run(pmx, cfg, V0, 6)
+        out.append((V, U, dts))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
