@@ -158,6 +158,12 @@ int pmx_wind_set_heatcool(pmx_wind *w, const double *xi, const double *t_r);
 
 /* g_time, g_dt and the step counter g_stepNumber of the next step. */
 int pmx_wind_set_time(pmx_wind *w, double t, double dt, long step);
+/* The timeStep fields the step reduces into: Dts->invDt_hyp (max with the
+ * stage-1 C_dt, then / DIMENSIONS) and Dts->dt_cool (min with the plain
+ * integrator's limits).  The reference's Integrate() resets them
+ * (Src/main.c:406-415); a driver that calls pmx_wind_advance_step /
+ * pmx_wind_split_source itself passes its values here first. */
+int pmx_wind_set_dts(pmx_wind *w, double inv_dt_hyp, double dt_cool);
 /* Also returns Dts->invDt_hyp, Dts->dt_cool and the OR of the cons->prim
  * flags since the last call (bit 8: a cooling sub-step limit was hit). */
 int pmx_wind_get_time(pmx_wind *w, double *t, double *dt, long *step, double *inv_dt_hyp,

@@ -31,7 +31,7 @@ namespace pmx {
 enum { MODE_FIRST = 0, MODE_WEIGHTED = 1, MODE_THIRD = 2 };
 
 #ifndef PMX_WAVES_PER_EU
-#define PMX_WAVES_PER_EU 2   // occupancy the register allocation targets
+#define PMX_WAVES_PER_EU 3   // occupancy the register allocation targets (3 measured fastest)
 #endif
 #ifndef PMX_PREFETCH_U
 #define PMX_PREFETCH_U 0   // load U and V0 of the cell at the top of the plane loop

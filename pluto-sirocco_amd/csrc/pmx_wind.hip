@@ -821,6 +821,19 @@ int pmx_wind_set_time(pmx_wind* w, double t, double dt, long step) {
   return 0;
 }
 
+int pmx_wind_set_dts(pmx_wind* w, double inv_dt_hyp, double dt_cool) {
+  if (!w) { set_error("null argument"); return -1; }
+  Scalars s0;
+  PMX_HIP_CHECK(hipMemcpyAsync(&s0, w->s, sizeof(Scalars), hipMemcpyDeviceToHost, w->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  s0.inv_dt_hyp = inv_dt_hyp;
+  s0.dt_cool = dt_cool;
+  memcpy(&s0.dtcool_bits, &dt_cool, sizeof(double));
+  PMX_HIP_CHECK(hipMemcpyAsync(w->s, &s0, sizeof(Scalars), hipMemcpyHostToDevice, w->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  return 0;
+}
+
 int pmx_wind_get_time(pmx_wind* w, double* t, double* dt, long* step, double* inv_dt_hyp,
                       double* dt_cool, int* flags) {
   if (!w) { set_error("null argument"); return -1; }

@@ -35,7 +35,7 @@ EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro
            "pmx_hydro_device_state", "pmx_copy_probe",
            "pmx_wind_create", "pmx_wind_destroy", "pmx_wind_set_prim", "pmx_wind_get_prim",
            "pmx_wind_get_cons", "pmx_wind_set_heatcool", "pmx_wind_set_time",
-           "pmx_wind_get_time", "pmx_wind_advance_step", "pmx_wind_split_source",
+           "pmx_wind_get_time", "pmx_wind_set_dts", "pmx_wind_advance_step", "pmx_wind_split_source",
            "pmx_wind_run", "pmx_wind_sync", "pmx_wind_timing"]
 
 
@@ -109,6 +109,7 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     lib.pmx_wind_set_heatcool.argtypes = [vp, dp, dp]
     lib.pmx_wind_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_long]
     lib.pmx_wind_get_time.argtypes = [vp, dp, dp, lp, dp, dp, ip]
+    lib.pmx_wind_set_dts.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     lib.pmx_wind_advance_step.argtypes = [vp]
     lib.pmx_wind_split_source.argtypes = [vp]
     lib.pmx_wind_run.argtypes = [vp, ctypes.c_int]
@@ -322,6 +323,9 @@ class Wind:
                                            ctypes.byref(dtc), ctypes.byref(fl)), "get_time")
         return {"t": t.value, "dt": dt.value, "step": st.value, "inv_dt_hyp": inv.value,
                 "dt_cool": dtc.value, "flags": fl.value}
+
+    def set_dts(self, inv_dt_hyp: float, dt_cool: float = 1.e38) -> None:
+        _check(self._lib.pmx_wind_set_dts(self._h, inv_dt_hyp, dt_cool), "set_dts")
 
     def advance_step(self) -> None:
         _check(self._lib.pmx_wind_advance_step(self._h), "advance_step")
