@@ -188,6 +188,70 @@ int pmx_wind_sync(pmx_wind *w);
 int pmx_wind_timing(pmx_wind *w, int enable, int reset, double *stage_ms, int *stage_launches,
                     double *cool_ms, int *cool_launches);
 
+/* ------------------------------------------------------------------ */
+/* MLP cooling surrogate (no counterpart in the reference).              */
+/*                                                                      */
+/* 4 -> 64 -> 64 -> 64 -> 1, SiLU, BF16 weights and activations on the   */
+/* matrix cores, FP32 accumulation, FP32 biases.  Inputs: log10 rho      */
+/* [g/cm^3], log10 T [K], log10 xi, log10(dt / t_cool) with t_cool =     */
+/* E / |dE/dt| of the cell's state; output: log10(T_final / T_0).        */
+/* Parameters are a float32 vector of 8705 entries: W1[64][4] b1[64]     */
+/* W2[64][64] b2[64] W3[64][64] b3[64] w4[64] b4.                        */
+/* ------------------------------------------------------------------ */
+typedef struct pmx_mlp pmx_mlp;
+
+int pmx_mlp_create(unsigned long long seed, pmx_mlp **out);
+int pmx_mlp_destroy(pmx_mlp *m);
+int pmx_mlp_set_params(pmx_mlp *m, const float *P);   /* also resets Adam's moments */
+int pmx_mlp_get_params(pmx_mlp *m, float *P);
+/* n training samples drawn log-uniformly over the hull rho 1e-16..1e-10
+ * g/cm^3, T 1e4..1e8 K, xi 1..1e6, dt 1e-2..1e2 t_cool, labelled by the
+ * plain cooling integrator (the device code of pmx_wind_split_source with
+ * PMX_COOL_PLAIN) with the physics scalars of `desc` (its arrays are not
+ * read).  The dataset stays on the device. */
+int pmx_mlp_sample(pmx_mlp *m, const pmx_wind_desc *desc, long long n, unsigned long long seed);
+int pmx_mlp_set_dataset(pmx_mlp *m, const float *X, const float *Y, long long n);
+/* X [n][4] raw features, Y [n] labels, nev [n] Radiat() calls the plain
+ * integrator made per sample, aux [n][4] the sample's exact inputs (rho and
+ * p in code units, xi, dt in code units); any pointer may be NULL. */
+int pmx_mlp_get_dataset(pmx_mlp *m, float *X, float *Y, int *nev, double *aux, long long n);
+/* Mean loss and gradient (8705 floats) of L = sum (y - Y/2)^2 / count over
+ * the first `count` samples (y: network output, Y: label). */
+int pmx_mlp_loss_grad(pmx_mlp *m, long long count, float *loss, float *grad);
+/* `steps` Adam steps (batch a multiple of 64, cosine learning rate lr0 ->
+ * lr1); loss_hist (optional) receives each step's mean loss. */
+int pmx_mlp_train(pmx_mlp *m, int steps, int batch, float lr0, float lr1, float *loss_hist);
+/* One cooling step for n independent cells (no hydro), each with its own
+ * density rho and pressure p (code units), ionisation parameter xi and dt
+ * (code units), tracer 1, T_r = T_x: mode 0 the plain integrator, 1 the
+ * surrogate (network + refinement, plain fallback), 2 the surrogate's
+ * refinement without the network.  p_out receives the new pressures;
+ * stats (optional, 4 entries): cells given to the network, accepted,
+ * sent to the plain integrator, Radiat() calls; *ms (optional) the
+ * device time of the cooling kernels. */
+int pmx_mlp_cool_cells(pmx_mlp *m, const pmx_wind_desc *desc, long long n, const double *rho,
+                       const double *p, const double *xi, const double *dt, double *p_out,
+                       int mode, long long *stats, double *ms);
+/* Y[n] = predicted log10(T_final / T_0) for raw features X[n][4]. */
+int pmx_mlp_predict(pmx_mlp *m, const float *X, long long n, float *Y);
+
+/* Surrogate-cooled SplitSource (no counterpart in the reference) for a
+ * wind context with PMX_COOL_PLAIN: cells the reference integrator treats
+ * as stiff (|dE/dt| > 0.5 E / dt) and that lie inside the training hull get
+ * the network's guess, refined by Newton's method on the cooling ODE's time
+ * integral; cells outside the hull or whose refinement does not converge
+ * take the plain integrator, as do all non-stiff cells.  m == NULL switches
+ * back to the plain integrator.  The context keeps a pointer to m. */
+int pmx_wind_set_surrogate(pmx_wind *w, pmx_mlp *m, int guess);
+/* guess: 0 = the network's prediction starts the refinement; 1 = ablation
+ * without the network (start from the explicit step limited to one
+ * e-fold), to measure what the network contributes. */
+/* Running totals since the last reset: stiff cells handed to the network,
+ * accepted refinements, cells sent to the plain integrator, and Radiat()
+ * calls of all cooling steps (with or without the surrogate). */
+int pmx_wind_surrogate_stats(pmx_wind *w, long long *listed, long long *accepted,
+                             long long *fallback, long long *radiat_calls, int reset);
+
 /* Measurement helper (no counterpart in the reference): times `reps`
  * launches of a plain streaming copy of nbytes (multiple of 16) between two
  * fresh device buffers; reports the mean milliseconds per copy.  Used to

@@ -200,6 +200,8 @@ def _wind_struct(cfg, fluxes=None, grid=None):
         L.orc_wind_cool.argtypes = [ctypes.POINTER(_OrcWindCfg), dp, ctypes.c_double,
                                     ctypes.c_double, dp, dp, dp]
         L.orc_wind_cool.restype = ctypes.c_int
+        L.orc_wind_cool_time.argtypes = [ctypes.POINTER(_OrcWindCfg)] + [ctypes.c_double] * 6
+        L.orc_wind_cool_time.restype = ctypes.c_double
         L._wind_bound = True
     gr, gt = grid if grid is not None else setups.wind_grid(cfg)
     fr, ft, fp = fluxes if fluxes is not None else setups.wind_fluxes(cfg)
@@ -272,3 +274,10 @@ def wind_counters() -> dict:
     out = (ctypes.c_long * len(WIND_COUNTERS))()
     lib().orc_wind_counters(out, len(WIND_COUNTERS))
     return dict(zip(WIND_COUNTERS, list(out)))
+
+
+def wind_cool_time(cfg, rho, p0, xi, p1, tol):
+    """Exact time for p0 -> p1 under the cooling ODE (oracle
+    orc_wind_cool_time); NaN if unreachable."""
+    c, keep = _wind_struct(cfg)
+    return lib().orc_wind_cool_time(ctypes.byref(c), rho, p0, xi, cfg.t_x, p1, tol)

@@ -90,6 +90,14 @@ int orc_wind_run(const orc_wind_cfg *cfg, double *V, int nsteps, double *dt,
 int orc_wind_cool(const orc_wind_cfg *cfg, double *V, double dt, double t, const double *xi,
                   const double *t_r, double *dt_cool);
 
+/* Exact time for a cell's energy to go from p0 to p1 (code units) under
+ * the cooling ODE dE/dt = Radiat(E) with density rho, ionisation parameter
+ * xi and T_r fixed (tracer 1, g_time > 3); adaptive Simpson in ln E to the
+ * absolute tolerance tol.  NaN if the rate vanishes or changes sign on the
+ * way (p1 cannot be reached). */
+double orc_wind_cool_time(const orc_wind_cfg *cfg, double rho, double p0, double xi, double t_r,
+                          double p1, double tol);
+
 /* Branch counters of the last orc_wind_run / orc_wind_cool: rkf12 accepted, ck45 calls,
  * ck45 substeps, ck45 rejections, BLONDIN solves, bracket widenings,
  * second (heat-cool root) solves, cells below the 1e4 K cut, dt limited by

@@ -950,3 +950,66 @@ int orc_wind_cool(const orc_wind_cfg *c, double *V, double dt, double t, const d
   w_free(&w);
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Exact time for one cell's energy to go from E0 = p0/(gamma-1) to      */
+/* E1 = p1/(gamma-1) under dE/dt = Radiat(E) (rho, xi, T_r fixed,        */
+/* tracer 1, g_time > 3): the integral of dE / f(E), by adaptive Simpson  */
+/* quadrature in ln E.  Returns NaN if f vanishes or changes sign on the  */
+/* path (the trajectory cannot get there).  Checks the surrogate's        */
+/* refinement independently of the plain integrator.                      */
+/* ------------------------------------------------------------------ */
+static double ct_g(const Rad *R, double u, int *bad)
+{
+  double v[NVC], rhs[NVC], E = exp(u);
+  memset(v, 0, sizeof(v)); memset(rhs, 0, sizeof(rhs));
+  v[RHO] = R->w->V[0]; v[TRC] = 1.0; v[RHOE] = E;
+  radiat(R, v, rhs);
+  if (!(rhs[RHOE] != 0.0) || rhs[RHOE] != rhs[RHOE]) { *bad = 1; return 0.0; }
+  return E/rhs[RHOE];
+}
+
+static double ct_simpson(const Rad *R, double a, double b, double fa, double fm, double fb,
+                         double whole, double tol, int depth, int *bad, double s0)
+{
+  double m = 0.5*(a + b), lm = 0.5*(a + m), rm = 0.5*(m + b);
+  double flm = ct_g(R, lm, bad), frm = ct_g(R, rm, bad);
+  double left, right;
+  if (*bad || flm*s0 <= 0.0 || frm*s0 <= 0.0) { *bad = 1; return 0.0; }
+  left = (m - a)/6.0*(fa + 4.0*flm + fm);
+  right = (b - m)/6.0*(fm + 4.0*frm + fb);
+  if (depth > 48 || fabs(left + right - whole) <= 15.0*tol)
+    return left + right + (left + right - whole)/15.0;
+  return ct_simpson(R, a, m, fa, flm, fm, left, 0.5*tol, depth + 1, bad, s0) +
+         ct_simpson(R, m, b, fm, frm, fb, right, 0.5*tol, depth + 1, bad, s0);
+}
+
+double orc_wind_cool_time(const orc_wind_cfg *c, double rho, double p0, double xi, double t_r,
+                          double p1, double tol)
+{
+  W w;
+  Rad R;
+  double rv[1];
+  double a, b, fa, fb, fm, whole, s0;
+  int bad = 0;
+  memset(&w, 0, sizeof(w));
+  w.c = c;
+  w.UD = c->unit_density; w.UL = c->unit_length; w.UV = c->unit_velocity;
+  w.KELVIN = (w.UV*w.UV*C_amu/C_kB);
+  w.t = 5.0;
+  rv[0] = rho;
+  w.V = rv;
+  R.w = &w; R.x1 = 1.0; R.xi_s = xi; R.tr_s = t_r;
+  a = log(p0/(c->gamma - 1.0));
+  b = log(p1/(c->gamma - 1.0));
+  if (a == b) return 0.0;
+  fa = ct_g(&R, a, &bad); fb = ct_g(&R, b, &bad); fm = ct_g(&R, 0.5*(a + b), &bad);
+  s0 = fa > 0.0 ? 1.0 : -1.0;
+  /* f keeps its sign on the path and points from p0 towards p1 */
+  if (bad || fb*fa <= 0.0 || fm*fa <= 0.0 || (b - a)*fa <= 0.0) return NAN;
+  whole = (b - a)/6.0*(fa + 4.0*fm + fb);
+  {
+    double t = ct_simpson(&R, a, b, fa, fm, fb, whole, tol, 0, &bad, s0);
+    return bad ? NAN : t;
+  }
+}

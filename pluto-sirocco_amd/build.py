@@ -28,6 +28,7 @@ HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
          "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+         "-Werror=infinite-recursion", "-Werror=return-type",
          "-I", str(HERE.parent / "include"), "-I", str(CSRC)]
 
 

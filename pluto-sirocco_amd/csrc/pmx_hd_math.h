@@ -15,8 +15,8 @@ enum { REC_PLM_VL = 0, REC_PLM_MC = 1, REC_PLM_MM = 2, REC_PLM_DEFAULT = 3,
        REC_PLM_FLAT = 4, REC_PPM = 5 };
 enum { SOL_HLLC = 0, SOL_ROE = 1 };
 
-__device__ __forceinline__ double dmax(double a, double b) { return a >= b ? a : b; }  // macros.h:147
-__device__ __forceinline__ double dmin(double a, double b) { return a <= b ? a : b; }  // macros.h:150
+__host__ __device__ __forceinline__ double dmax(double a, double b) { return a >= b ? a : b; }  // macros.h:147
+__host__ __device__ __forceinline__ double dmin(double a, double b) { return a <= b ? a : b; }  // macros.h:150
 __device__ __forceinline__ double absmin(double a, double b) {                        // macros.h:153
   return fabs(a) < fabs(b) ? a : b;
 }

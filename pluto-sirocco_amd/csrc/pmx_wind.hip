@@ -20,6 +20,7 @@
 #include "../../include/plutomx.h"
 #include "pmx_common.h"
 #include "pmx_wind_math.h"
+#include "pmx_wind_host.h"
 
 namespace pmx {
 namespace wind {
@@ -28,14 +29,6 @@ constexpr int NG = 2;          // ghost cells of the reference grid (PLM)
 constexpr int BX = 64, BY = 2; // stage-kernel workgroup
 constexpr int NLF = 7;         // line-force tables per (angle, cell)
 enum { LF_F1, LF_F2, LF_SIN, LF_COS, LF_DS, LF_FR, LF_FT };
-
-struct Scalars {
-  double t, dt, inv_dt_hyp, dt_cool;
-  unsigned long long cdt_bits;     // max C_dt of the current step's stage 1
-  unsigned long long dtcool_bits;  // running min of the cooling dt candidates
-  int err;
-  int pad;
-};
 
 // Geometry and line-force tables on the device.  Radial and polar 1-D
 // arrays include the 2 ghost cells (index = interior index + 2).
@@ -346,22 +339,12 @@ __global__ __launch_bounds__(BX* BY) void wind_stage_kernel(StageArgs a) {
   }
 }
 
-struct CoolArgs {
-  double* V;
-  int nr, nt;
-  long long sv;
-  Const k;
-  const double* x1;      // ghost-inclusive radial centres
-  const double* xi_s;
-  const double* tr_s;
-  Scalars* s;
-};
-
 template <int MODE>
-__global__ __launch_bounds__(128) void wind_cool_kernel(CoolArgs a) {
+__global__ __launch_bounds__(128) void wind_cool_kernel(CoolArgs a, unsigned long long* calls) {
   const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double cand = 1.0e300;
   int bad = 0;
+  long long ncalls = 0;
   if (cell < a.sv) {
     const int i = (int)(cell % a.nr);
     const double t = a.s->t, dt = a.s->dt;
@@ -377,7 +360,9 @@ __global__ __launch_bounds__(128) void wind_cool_kernel(CoolArgs a) {
       R.xi_s = a.xi_s[cell];
       R.tr_s = a.tr_s[cell];
       R.t = t;
+      R.nev = 0;
       pn = plain_cell(a.k, R, p, dt, cand, bad);
+      ncalls = R.nev;
     }
     a.V[PRS * a.sv + cell] = pn;
   }
@@ -387,7 +372,9 @@ __global__ __launch_bounds__(128) void wind_cool_kernel(CoolArgs a) {
     for (int off = 32; off > 0; off >>= 1) {
       cand = dmin(cand, __shfl_xor(cand, off));
       bad |= __shfl_xor(bad, off);
+      ncalls += __shfl_xor(ncalls, off);
     }
+    if ((threadIdx.x & 63) == 0 && ncalls) atomicAdd(calls, (unsigned long long)ncalls);
     if ((threadIdx.x & 63) == 0) { smin[threadIdx.x >> 6] = cand; sbad[threadIdx.x >> 6] = bad; }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -459,6 +446,8 @@ struct pmx_wind {
   std::vector<std::pair<hipEvent_t, hipEvent_t>> pend_stage, pend_cool;
   double stage_ms, cool_ms;
   int stage_n, cool_n;
+  pmx_mlp* mlp;        // surrogate for the stiff cells (PMX_COOL_PLAIN), or null
+  SurrWork sw;
 };
 
 namespace {
@@ -545,13 +534,18 @@ int cool(pmx_wind* w) {
   a.xi_s = w->xi_s;
   a.tr_s = w->tr_s;
   a.s = w->s;
+  a.dtc = nullptr;
   dim3 grid((unsigned)((w->sv + 127) / 128)), block(128);
   hipEvent_t e[2];
   if (timed_begin(w, e)) return -1;
-  if (w->cooling == PMX_COOL_BLONDIN)
-    hipLaunchKernelGGL(wind_cool_kernel<PMX_COOL_BLONDIN>, grid, block, 0, w->stream, a);
+  if (w->mlp) {
+    if (mlp::surrogate_cool(w->mlp, a, w->sw, w->stream)) return -1;
+  } else if (w->cooling == PMX_COOL_BLONDIN)
+    hipLaunchKernelGGL(wind_cool_kernel<PMX_COOL_BLONDIN>, grid, block, 0, w->stream, a,
+                       w->sw.cnt + 7);
   else
-    hipLaunchKernelGGL(wind_cool_kernel<PMX_COOL_PLAIN>, grid, block, 0, w->stream, a);
+    hipLaunchKernelGGL(wind_cool_kernel<PMX_COOL_PLAIN>, grid, block, 0, w->stream, a,
+                       w->sw.cnt + 7);
   PMX_HIP_CHECK(hipGetLastError());
   return timed_end(w, e, true);
 }
@@ -576,6 +570,36 @@ void plm_coeffs(const double* x, const double* xr, const double* dx, const doubl
 
 }  // namespace
 
+namespace pmx {
+namespace wind {
+// Problem constants from the descriptor's scalars (Src/pluto.h:700-720,
+// the problem's definitions.h), with the reference's expressions.
+Const make_const(const pmx_wind_desc* d) {
+  Const k;
+  k.gamma = d->gamma;
+  k.gmm1 = d->gamma - 1.0;
+  k.UD = d->unit_density; k.UL = d->unit_length; k.UV = d->unit_velocity;
+  k.UT = (k.UL / k.UV);
+  k.UP = (k.UD * k.UV * k.UV);
+  k.UA = (k.UV * k.UV / k.UL);
+  k.KELVIN = (k.UV * k.UV * C_amu / C_kB);
+  k.mu = d->mu;
+  {
+    const double FHe = 0.2741 / C_AHe * C_AH / 0.7110;           // Src/pluto.h:718-720
+    const double FZ = (1.0 - 0.7110 - 0.2741) / C_AZ * C_AH / 0.7110;
+    k.mu_tab = (C_AH + FHe * C_AHe + FZ * C_AZ) / (2.0 + 3.0 * FHe + FZ * (1.0 + C_AZ * 0.5));
+  }
+  k.gm_code = C_G * d->cent_mass / (k.UL * k.UV * k.UV);
+  k.sigma_e = C_sigmaT / C_amu / 1.18;
+  k.krad = d->krad; k.alpharad = d->alpharad;
+  k.lx = d->l_star * d->f_x;
+  k.t_x = d->t_x;
+  k.E_cost = k.UL / k.UD / pow(k.UV, 3.0);
+  return k;
+}
+}  // namespace wind
+}  // namespace pmx
+
 extern "C" {
 
 int pmx_wind_destroy(pmx_wind* w) {
@@ -586,6 +610,8 @@ int pmx_wind_destroy(pmx_wind* w) {
   for (auto& p : w->pend_cool) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
   hipFree(w->V[0]); hipFree(w->V[1]); hipFree(w->U); hipFree(w->xi_s); hipFree(w->tr_s);
   hipFree(w->tab); hipFree(w->lf); hipFree(w->s);
+  hipFree(w->sw.list); hipFree(w->sw.fb); hipFree(w->sw.feat); hipFree(w->sw.yhat);
+  hipFree(w->sw.f0); hipFree(w->sw.cnt);
   if (w->stream) hipStreamDestroy(w->stream);
   delete w;
   return 0;
@@ -610,28 +636,9 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
   w->cfl = d->cfl; w->cfl_max_var = d->cfl_max_var;
   w->sv = (long long)w->nr * w->nt;
   w->step = 0;
+  w->k = make_const(d);
+  const Const& k = w->k;
 
-  // constants (Src/pluto.h:700-716, main.c, the problem's definitions.h)
-  Const& k = w->k;
-  k.gamma = d->gamma;
-  k.gmm1 = d->gamma - 1.0;
-  k.UD = d->unit_density; k.UL = d->unit_length; k.UV = d->unit_velocity;
-  k.UT = (k.UL / k.UV);
-  k.UP = (k.UD * k.UV * k.UV);
-  k.UA = (k.UV * k.UV / k.UL);
-  k.KELVIN = (k.UV * k.UV * C_amu / C_kB);
-  k.mu = d->mu;
-  {
-    const double FHe = 0.2741 / C_AHe * C_AH / 0.7110;           // Src/pluto.h:718-720
-    const double FZ = (1.0 - 0.7110 - 0.2741) / C_AZ * C_AH / 0.7110;
-    k.mu_tab = (C_AH + FHe * C_AHe + FZ * C_AZ) / (2.0 + 3.0 * FHe + FZ * (1.0 + C_AZ * 0.5));
-  }
-  k.gm_code = C_G * d->cent_mass / (k.UL * k.UV * k.UV);
-  k.sigma_e = C_sigmaT / C_amu / 1.18;
-  k.krad = d->krad; k.alpharad = d->alpharad;
-  k.lx = d->l_star * d->f_x;
-  k.t_x = d->t_x;
-  k.E_cost = k.UL / k.UD / pow(k.UV, 3.0);
 
   const int tr = w->nr + 2 * NG, tt = w->nt + 2 * NG, nr = w->nr, nt = w->nt, na = w->na;
   // host tables: radial 6 + 6 PLM rows, polar 9 + 6 PLM rows, centre weights, bin angles
@@ -746,6 +753,8 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
   CK(hipMalloc(&w->tab, sizeof(double) * packed.size()));
   CK(hipMalloc(&w->lf, sizeof(double) * lf.size()));
   CK(hipMalloc(&w->s, sizeof(Scalars)));
+  CK(hipMalloc(&w->sw.cnt, sizeof(unsigned long long) * 8));
+  CK(hipMemset(w->sw.cnt, 0, sizeof(unsigned long long) * 8));
   CK(hipMemcpy(w->tab, packed.data(), sizeof(double) * packed.size(), hipMemcpyHostToDevice));
   CK(hipMemcpy(w->lf, lf.data(), sizeof(double) * lf.size(), hipMemcpyHostToDevice));
   CK(hipMemset(w->V[0], 0, vbytes));
@@ -884,6 +893,41 @@ int pmx_wind_run(pmx_wind* w, int nsteps) {
 int pmx_wind_sync(pmx_wind* w) {
   if (!w) { set_error("null argument"); return -1; }
   PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  return 0;
+}
+
+int pmx_wind_set_surrogate(pmx_wind* w, pmx_mlp* m, int guess) {
+  if (!w) { set_error("null argument"); return -1; }
+  if (m && w->cooling != PMX_COOL_PLAIN) {
+    set_error("the surrogate replaces the plain cooling integrator (PMX_COOL_PLAIN)");
+    return -1;
+  }
+  if (m && !w->sw.list) {
+    const size_t n = (size_t)w->sv;
+    PMX_HIP_CHECK(hipMalloc(&w->sw.list, sizeof(int) * n));
+    PMX_HIP_CHECK(hipMalloc(&w->sw.fb, sizeof(int) * n));
+    PMX_HIP_CHECK(hipMalloc(&w->sw.feat, sizeof(float) * 4 * n));
+    PMX_HIP_CHECK(hipMalloc(&w->sw.yhat, sizeof(float) * n));
+    PMX_HIP_CHECK(hipMalloc(&w->sw.f0, sizeof(double) * n));
+  }
+  w->mlp = m;
+  w->sw.guess = guess;
+  return 0;
+}
+
+int pmx_wind_surrogate_stats(pmx_wind* w, long long* listed, long long* accepted,
+                             long long* fallback, long long* radiat_calls, int reset) {
+  if (!w) { set_error("null argument"); return -1; }
+  unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  {
+    PMX_HIP_CHECK(hipMemcpyAsync(c, w->sw.cnt, sizeof(c), hipMemcpyDeviceToHost, w->stream));
+    PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+    if (reset) PMX_HIP_CHECK(hipMemset(w->sw.cnt + 4, 0, sizeof(unsigned long long) * 4));
+  }
+  if (listed) *listed = (long long)c[4];
+  if (accepted) *accepted = (long long)c[6];
+  if (fallback) *fallback = (long long)c[5];
+  if (radiat_calls) *radiat_calls = (long long)c[7];
   return 0;
 }
 

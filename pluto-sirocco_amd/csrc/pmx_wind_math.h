@@ -10,6 +10,11 @@
 #include <hip/hip_runtime.h>
 #include "pmx_hd_math.h"
 
+// The cooling arithmetic compiles for the host too, so host-side tools can
+// exercise exactly the device code.
+#define PMX_HD __host__ __device__ __forceinline__
+#define PMX_HD_NOINLINE __host__ __device__ inline
+
 namespace pmx {
 namespace wind {
 
@@ -48,12 +53,12 @@ struct Const {
 // ---------------------------------------------------------------------
 // prim <-> cons with the tracer (Src/HD/mappers.c:65-67, :247-249)
 // ---------------------------------------------------------------------
-__device__ __forceinline__ void p2c6(const double* v, double* u, double gmm1) {
+PMX_HD void p2c6(const double* v, double* u, double gmm1) {
   prim_to_cons(v, u, gmm1);
   u[TRC] = v[RHO] * v[TRC];
 }
 
-__device__ __forceinline__ int c2p6(double* u, double* v, double gmm1) {
+PMX_HD int c2p6(double* u, double* v, double gmm1) {
   int f = cons_to_prim(u, v, gmm1);
   v[TRC] = u[TRC] * (1.0 / u[RHO]);
   return f;
@@ -61,7 +66,7 @@ __device__ __forceinline__ int c2p6(double* u, double* v, double gmm1) {
 
 // General-grid van Leer PLM of one cell (Src/States/plm_states.c:141-258
 // with plm_coeffs.h:140-142): c = {cp, cm, wp, wm, dp, dm}.
-__device__ __forceinline__ void plm_vl_general(const double* vm1, const double* v0,
+PMX_HD void plm_vl_general(const double* vm1, const double* v0,
                                                const double* vp1, const double* c,
                                                double* vp, double* vm) {
 #pragma unroll
@@ -81,7 +86,7 @@ __device__ __forceinline__ void plm_vl_general(const double* vm1, const double* 
 // Line force: one angle bin of LineForce (line_connect.c:873-902) for the
 // state (rho_cgs, v_th).  Returns the (1 + M) factor.
 // ---------------------------------------------------------------------
-__device__ __forceinline__ double force_mult(const Const& k, double rho_cgs, double v_th,
+PMX_HD double force_mult(const Const& k, double rho_cgs, double v_th,
                                              double dvds) {
   const double M_max = 4400.;
   double M;
@@ -102,7 +107,7 @@ struct Blondin {
   double xi, sqxi, sqsqxi, ne, nH, n, tx, E, hc_init, dt_share;
 };
 
-__device__ __forceinline__ double ne_rat(double T) {   // cooling.c:210-222
+PMX_HD double ne_rat(double T) {   // cooling.c:210-222
   double ne_ratio;
   if (T < 1.5e4) {
     ne_ratio = 1e-2 + pow(10.0, (-51.59417133 + 12.27740153 * log10(T)));
@@ -116,7 +121,7 @@ __device__ __forceinline__ double ne_rat(double T) {   // cooling.c:210-222
 
 // heatcool(), cooling.c:173-202 with every prefactor equal to 1.0 (the
 // values read_sirocco_heatcool leaves at start-up)
-__device__ __forceinline__ double heatcool(Blondin& b, double T) {
+PMX_HD double heatcool(Blondin& b, double T) {
   double sqT = sqrt(T);
   b.ne = b.nH * ne_rat(T);
   double comp_heat = 1.0 * (8.9e-36 * b.xi * b.tx);
@@ -129,18 +134,18 @@ __device__ __forceinline__ double heatcool(Blondin& b, double T) {
                  b.ne * brem_cool);
 }
 
-__device__ __forceinline__ double zfunc(Blondin& b, double temp) {   // cooling.c:233-237
+PMX_HD double zfunc(Blondin& b, double temp) {   // cooling.c:233-237
   return (temp * b.n * C_kB / (2.0 / 3.0)) - b.E - b.dt_share * (b.hc_init + heatcool(b, temp)) / 2.0;
 }
 
 template <int WHICH>   // 0: zfunc, 1: heatcool
-__device__ __forceinline__ double bfunc(Blondin& b, double x) {
+PMX_HD double bfunc(Blondin& b, double x) {
   return WHICH ? heatcool(b, x) : zfunc(b, x);
 }
 
 // Brent root finder, cooling.c:250-319 (ITMAX 100, EPS 3e-8)
 template <int WHICH>
-__device__ double zbrent(Blondin& bl, double x1, double x2, double tol) {
+PMX_HD_NOINLINE double zbrent(Blondin& bl, double x1, double x2, double tol) {
   const int ITMAX = 100;
   const double EPS = 3.0e-8;
   double a = x1, b = x2, c = x2, d = 0.0, e = 0.0;
@@ -183,7 +188,7 @@ __device__ double zbrent(Blondin& bl, double x1, double x2, double tol) {
 
 // One cell of BlondinCooling (cooling.c:50-165).  Returns the new
 // pressure (code units), or p unchanged below 1e4 K.
-__device__ __forceinline__ double blondin_cell(const Const& k, double rho_code, double p,
+PMX_HD double blondin_cell(const Const& k, double rho_code, double p,
                                                double r_code, double xi_s, double tr_s,
                                                double t, double dt) {
   const double minT = 1.e4;
@@ -240,11 +245,13 @@ struct RadCell {
   double x1;       // code units
   double xi_s, tr_s;
   double t;        // g_time
+  mutable int nev; // Radiat() evaluations so far (cost accounting)
 };
 
 // Radiat(): returns the RHOE rate; may floor *E like the reference.
-__device__ __forceinline__ double radiat(const Const& k, const RadCell& R, double& E) {
+PMX_HD double radiat(const Const& k, const RadCell& R, double& E) {
   const double minT = 1.e4;
+  R.nev++;
   double prs = E * (k.gamma - 1.0);
   if (prs < 0.0) { prs = 1.e-12; E = prs / (k.gamma - 1.0); }
   double T = prs / R.rho * k.KELVIN * k.mu;
@@ -269,7 +276,7 @@ __device__ __forceinline__ double radiat(const Const& k, const RadCell& R, doubl
 }
 
 // SolveODE_RKF12: returns err, writes the 2nd-order result.
-__device__ __forceinline__ double rkf12(const Const& k, const RadCell& R, double E0, double k1,
+PMX_HD double rkf12(const Const& k, const RadCell& R, double E0, double k1,
                                         double& E2nd, double dt) {
   double E1 = E0 + 0.5 * dt * k1;
   double k2 = radiat(k, R, E1);
@@ -285,7 +292,7 @@ __device__ __forceinline__ double rkf12(const Const& k, const RadCell& R, double
 // proposed next step.  E0 and k1 belong to the caller and are not
 // changed.  `nsub` counts accepted sub-steps (the reference aborts the
 // run past 1000; here the caller sees the count and flags the cell).
-__device__ double ck45(const Const& k, const RadCell& R, double E0, double k1, double& E5th,
+PMX_HD_NOINLINE double ck45(const Const& k, const RadCell& R, double E0, double k1, double& E5th,
                        double dt, double tol, int& nsub) {
   const double c1 = 37.0 / 378.0, c2 = 0.0, c3 = 250.0 / 621.0, c4 = 125.0 / 594.0, c5 = 0.0,
                c6 = 512.0 / 1771.0;
@@ -338,12 +345,27 @@ __device__ double ck45(const Const& k, const RadCell& R, double E0, double k1, d
   return dt;
 }
 
+// End of CoolingSource for one cell (cv_idl/cooling_source.c): new
+// pressure from the integrated energy E1, the 1e4 K floor and the cell's
+// dt * maxCoolingRate / |dp/p| candidate for dt_cool.
+PMX_HD double finish_cell(const Const& k, const RadCell& R, double p, double E1,
+                                              double dt, double& dt_cool_cell) {
+  const double minT = 1.e4, maxCoolingRate = 0.1;
+  double T0 = p / R.rho * k.KELVIN * k.mu_tab;
+  double prs = E1 * (k.gamma - 1.0);
+  double T1 = prs / R.rho * k.KELVIN * k.mu_tab;
+  if (prs < 0.0) prs = 1.e-12;
+  if (T1 < minT && T0 > minT) prs = minT * R.rho / (k.KELVIN * k.mu_tab);
+  double e = fabs(prs / p - 1.0);
+  dt_cool_cell = dt * maxCoolingRate / e;
+  return prs;
+}
+
 // One cell of CoolingSource: returns the new pressure and the cell's
 // dt * maxCoolingRate / |dp/p| candidate for dt_cool.
-__device__ __forceinline__ double plain_cell(const Const& k, const RadCell& R, double p,
+PMX_HD double plain_cell(const Const& k, const RadCell& R, double p,
                                              double dt, double& dt_cool_cell, int& bad) {
-  const double min_tol = 2.e-5, minT = 1.e4, maxCoolingRate = 0.1;
-  double T0 = p / R.rho * k.KELVIN * k.mu_tab;
+  const double min_tol = 2.e-5;
   double E0 = p / (k.gamma - 1.0), E1 = E0;
   double k1 = radiat(k, R, E0);
   int stiff = (fabs(k1) > 0.5 / dt ? 1 : 0);
@@ -368,13 +390,281 @@ __device__ __forceinline__ double plain_cell(const Const& k, const RadCell& R, d
       k1 = radiat(k, R, E0);
     } while (!done);
   }
-  double prs = E1 * (k.gamma - 1.0);
-  double T1 = prs / R.rho * k.KELVIN * k.mu_tab;
-  if (prs < 0.0) prs = 1.e-12;
-  if (T1 < minT && T0 > minT) prs = minT * R.rho / (k.KELVIN * k.mu_tab);
-  double e = fabs(prs / p - 1.0);
-  dt_cool_cell = dt * maxCoolingRate / e;
-  return prs;
+  return finish_cell(k, R, p, E1, dt, dt_cool_cell);
+}
+
+// ---------------------------------------------------------------------
+// Refinement of a surrogate guess for a stiff cell (no counterpart in the
+// reference).  Over one cooling step rho and the tracer are fixed, so the
+// energy obeys the scalar ODE dE/dt = f(E) = Radiat(E), whose solution
+// moves monotonically from E0 towards the first zero of f (an equilibrium)
+// or, when cooling, towards the 1e4 K cut below which f = 0.  The exact end
+// state over dt is the E with
+//      t(E) = integral_{E0}^{E} dE' / f(E') = dt,
+// solved here by Newton's method from the network's guess.  The integral
+// uses composite 4-point Gauss-Legendre quadrature, in u = ln E for a path
+// that stays away from an equilibrium, and in w = ln|E_eq - E| towards an
+// equilibrium E_eq (there the integrand tends to the constant 1/lambda of
+// the linearised relaxation, so the quadrature stays accurate).
+// ---------------------------------------------------------------------
+constexpr double GLX[4] = {-0.8611363115940526, -0.3399810435848563, 0.3399810435848563,
+                           0.8611363115940526};
+constexpr double GLW[4] = {0.3478548451374638, 0.6521451548625461, 0.6521451548625461,
+                           0.3478548451374638};
+constexpr int REFINE_ITERS = 10;     // Newton iterations (damped steps included)
+constexpr int REFINE_ROOT_ITERS = 60;
+constexpr int QUAD_STACK = 40;       // adaptive quadrature: bisection depth
+constexpr int QUAD_PANELS = 400;     // adaptive quadrature: panels examined per integral
+constexpr double QUAD_TOL = 1e-6;    // absolute time tolerance, in units of dt
+constexpr double REACH_TOL = 1e-2;   // loose tolerance of the equilibrium reach test
+constexpr double EQ_PROBE = 1e-3;    // relative step beyond the guess probing for f = 0
+
+PMX_HD double rate(const Const& k, const RadCell& R, double E) {
+  double e = E;
+  return radiat(k, R, e);
+}
+
+// Integrand of the time integral in the variable x: in u = ln E (mode 0)
+// dt/du = E / f(E); in w = ln|Eeq - E| with E = Eeq - s0 e^w (mode 1)
+// dt/dw = -s0 e^w / f(E).  False where f does not have the sign s0.
+PMX_HD bool integrand(const Const& k, const RadCell& R, int mode, double Eeq, double s0, double x,
+                      double& g) {
+  double E, dEdx;
+  if (mode == 0) {
+    E = exp(x);
+    dEdx = E;
+  } else {
+    const double ew = exp(x);
+    E = Eeq - s0 * ew;
+    dEdx = -s0 * ew;
+  }
+  const double f = rate(k, R, E);
+  if (!(f * s0 > 0.0)) return false;
+  g = dEdx / f;
+  return true;
+}
+
+// 4-point Gauss-Legendre on one panel
+PMX_HD bool gl4(const Const& k, const RadCell& R, int mode, double Eeq, double s0, double a,
+                double b, double& val, double& xbad) {
+  const double h = b - a, c = 0.5 * (a + b);
+  val = 0.0;
+  for (int q = 0; q < 4; q++) {
+    double g;
+    const double x = c + 0.5 * h * GLX[q];
+    if (!integrand(k, R, mode, Eeq, s0, x, g)) {
+      xbad = x;
+      return false;
+    }
+    val += 0.5 * h * GLW[q] * g;
+  }
+  return true;
+}
+
+// Adaptive integral of the time integrand from a to b: panels are halved
+// until the two halves agree with the whole panel to within the panel's
+// share of the tolerance `tol` (Richardson-corrected sum); false if f
+// changes sign at a node (xbad = that node, else NaN) or the panel budget
+// runs out.
+PMX_HD_NOINLINE bool t_adapt(const Const& k, const RadCell& R, int mode, double Eeq, double s0,
+                             double a, double b, double tol, double& t, double& xbad) {
+  t = 0.0;
+  xbad = NAN;
+  if (a == b) return true;
+  double sa[QUAD_STACK], sb[QUAD_STACK], sw[QUAD_STACK];
+  int sp = 0, panels = 0;
+  const double span = fabs(b - a);
+  if (!gl4(k, R, mode, Eeq, s0, a, b, sw[0], xbad)) return false;
+  sa[0] = a; sb[0] = b; sp = 1;
+  while (sp > 0) {
+    sp--;
+    const double pa = sa[sp], pb = sb[sp], whole = sw[sp], m = 0.5 * (pa + pb);
+    double left, right;
+    if (!gl4(k, R, mode, Eeq, s0, pa, m, left, xbad)) return false;
+    if (!gl4(k, R, mode, Eeq, s0, m, pb, right, xbad)) return false;
+    const double diff = left + right - whole;
+    if (++panels > QUAD_PANELS) return false;
+    if (fabs(diff) <= tol * fabs(pb - pa) / span) {
+      t += left + right + diff / 255.0;
+    } else {
+      if (sp + 2 > QUAD_STACK) return false;
+      sa[sp] = m; sb[sp] = pb; sw[sp] = right; sp++;
+      sa[sp] = pa; sb[sp] = m; sw[sp] = left; sp++;
+    }
+  }
+  return true;
+}
+
+// Surrogate-guided end state of one stiff cell.  E0 > 0, f0 = f(E0) != 0,
+// Eg the guess.  Returns 1 with *Ef set, or a negative code (which check
+// failed) when the cell must take the plain integrator: the path reaches
+// the 1e4 K cut, a quadrature node or iterate crosses an equilibrium where
+// it should not, a root is not bracketed, Newton does not converge, or the
+// final accuracy check fails.
+PMX_HD_NOINLINE int refine_cell(const Const& k, const RadCell& R, double E0, double f0, double dt,
+                           double Eg, double& Ef) {
+  const double s0 = f0 > 0.0 ? 1.0 : -1.0;
+  const double tol = QUAD_TOL * dt;
+  const double Ecut = 1.e4 / (k.KELVIN * k.mu) * R.rho / (k.gamma - 1.0);   // T = 1e4 K
+  const double u0 = log(E0);
+  if (!(Eg > 0.0) || !(Eg - Eg == 0.0)) return -1;   // not positive and finite
+  if (s0 * (Eg - E0) <= 0.0) Eg = E0 * exp(s0 * 1e-3);   // guess behind the start
+  if (s0 < 0.0 && Eg <= Ecut) return -2;
+  double fg = rate(k, R, Eg);
+  double Ea = E0, Eb = 0.0;          // bracket of an equilibrium (f(Ea) s0 > 0 >= f(Eb) s0)
+  bool have_eq = false;
+  if (!(fg * s0 > 0.0)) {
+    Eb = Eg;
+    have_eq = true;
+  } else {
+    // A guess that has (almost) relaxed to an equilibrium: the sign change
+    // of f is right beyond it, and the cheap equilibrium branch applies.
+    const double Et = Eg * exp(s0 * EQ_PROBE);
+    if (!(s0 < 0.0 && Et <= Ecut) && !(rate(k, R, Et) * s0 > 0.0)) {
+      Ea = Eg;
+      Eb = Et;
+      have_eq = true;
+    }
+  }
+  if (!have_eq) {
+    // Newton in u from the guess
+    double u = log(Eg), t, xbad;
+    bool ok0 = t_adapt(k, R, 0, 0.0, s0, u0, u, tol, t, xbad);
+    if (!ok0 && xbad == xbad) {        // an equilibrium before the guess
+      Eb = exp(xbad);
+      have_eq = true;
+    } else if (!ok0) {
+      // the panel budget ran out: the guess sits next to an equilibrium,
+      // where the integrand is nearly singular; look just beyond it
+      const double dl[4] = {1e-4, 1e-3, 1e-2, 1e-1};
+      for (int q = 0; q < 4 && !have_eq; q++) {
+        const double Et = Eg * exp(s0 * dl[q]);
+        if (s0 < 0.0 && Et <= Ecut) break;
+        if (!(rate(k, R, Et) * s0 > 0.0)) {
+          Ea = Eg;
+          Eb = Et;
+          have_eq = true;
+        }
+      }
+      if (!have_eq) return -3;
+    }
+    double fu = fg;
+    for (int it = 0; it < REFINE_ITERS && !have_eq; it++) {
+      const double E = exp(u);
+      double du = -(t - dt) * fu / E;
+      double un = u + du;
+      if (s0 * (un - u0) <= 0.0) un = 0.5 * (u + u0);   // stay ahead of the start
+      // Near an equilibrium the linear step overshoots far past it; never
+      // step across the cut, go half way towards it instead (the sign test
+      // below then finds the equilibrium, if there is one, in between).
+      if (s0 < 0.0 && un <= log(Ecut)) un = 0.5 * (u + log(Ecut));
+      const double En = exp(un);
+      if (s0 < 0.0 && En <= Ecut) return -4;
+      const double fn = rate(k, R, En);
+      if (!(fn * s0 > 0.0)) {          // stepped across an equilibrium
+        Ea = E;
+        Eb = En;
+        have_eq = true;
+        break;
+      }
+      double dtinc;
+      if (!t_adapt(k, R, 0, 0.0, s0, u, un, tol, dtinc, xbad)) {
+        if (!(xbad == xbad)) return -5;
+        Ea = E;                        // an equilibrium between the iterates
+        Eb = exp(xbad);
+        have_eq = true;
+        break;
+      }
+      t += dtinc;
+      u = un;
+      fu = fn;
+      if (fabs(du) <= 1e-7) {
+        Ef = exp(u);
+        return 1;
+      }
+    }
+    if (!have_eq) return -8;
+  }
+  // ---- an equilibrium lies between Ea and Eb: locate it (Illinois) -----
+  if (s0 < 0.0 && Eb <= Ecut) return -9;
+  double fa = rate(k, R, Ea), fb = rate(k, R, Eb);
+  if (fb == 0.0) {
+    double Tb = Eb * (k.gamma - 1.0) / R.rho * k.KELVIN * k.mu;
+    if (Tb < 1.e4) return -10;          // the cut, not an equilibrium
+  }
+  double Eeq = Eb;
+  {
+    double a = Ea, b = Eb;
+    int side = 0;
+    bool found = false;
+    for (int it = 0; it < REFINE_ROOT_ITERS && !found; it++) {
+      double c = (a * fb - b * fa) / (fb - fa);
+      if (!(c > fmin(a, b) && c < fmax(a, b))) c = 0.5 * (a + b);
+      const double fc = rate(k, R, c);
+      if (fc == 0.0) {
+        Eeq = c;
+        found = true;
+      } else if (fc * s0 > 0.0) {
+        a = c; fa = fc;
+        if (side == -1) fb *= 0.5;
+        side = -1;
+      } else {
+        b = c; fb = fc;
+        if (side == 1) fa *= 0.5;
+        side = 1;
+      }
+      if (!found && fabs(b - a) <= 1e-12 * fabs(c)) {
+        Eeq = 0.5 * (a + b);
+        found = true;
+      }
+    }
+    if (!found) return -11;
+  }
+  if (s0 < 0.0 && Eeq <= Ecut * 1.0001) return -12;
+  // ---- reached within dt?  time to come within 1e-6 of E_eq -----------
+  const double w0 = log(fabs(Eeq - E0));
+  const double wend = log(1e-6 * Eeq);
+  if (!(wend < w0)) {                 // started next to the equilibrium
+    Ef = Eeq;
+    return 1;
+  }
+  // Only t_reach <= dt matters here, so a loose tolerance suffices when
+  // the margin is large; otherwise the reach time is recomputed tightly.
+  double treach;
+  double xb;
+  if (t_adapt(k, R, 1, Eeq, s0, w0, wend, REACH_TOL * dt, treach, xb) &&
+      treach <= (1.0 - 2.0 * REACH_TOL) * dt) {
+    Ef = Eeq;
+    return 1;
+  }
+  if (!t_adapt(k, R, 1, Eeq, s0, w0, wend, tol, treach, xb)) return -13;
+  if (treach <= dt) {
+    Ef = Eeq;
+    return 1;
+  }
+  // ---- not reached: Newton in w for t(w) = dt --------------------------
+  double w = (s0 * (Eeq - Eg) > 0.0 && s0 * (Eg - E0) > 0.0) ? log(fabs(Eeq - Eg)) : 0.5 * (w0 + wend);
+  double t;
+  if (!t_adapt(k, R, 1, Eeq, s0, w0, w, tol, t, xb)) return -14;
+  for (int it = 0; it < REFINE_ITERS; it++) {
+    const double ew = exp(w);
+    const double fw = rate(k, R, Eeq - s0 * ew);
+    if (!(fw * s0 > 0.0)) return -15;
+    const double dtdw = (-s0 * ew) / fw;     // < 0
+    double dw = -(t - dt) / dtdw;
+    double wn = w + dw;
+    if (wn > w0) wn = 0.5 * (w + w0);
+    if (wn < wend) wn = 0.5 * (w + wend);
+    double dtinc;
+    if (!t_adapt(k, R, 1, Eeq, s0, w, wn, tol, dtinc, xb)) return -16;
+    t += dtinc;
+    w = wn;
+    if (fabs(dw) <= 1e-7) {
+      Ef = Eeq - s0 * exp(w);
+      return 1;
+    }
+  }
+  return -20;
 }
 
 }  // namespace wind

@@ -36,7 +36,11 @@ EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro
            "pmx_wind_create", "pmx_wind_destroy", "pmx_wind_set_prim", "pmx_wind_get_prim",
            "pmx_wind_get_cons", "pmx_wind_set_heatcool", "pmx_wind_set_time",
            "pmx_wind_get_time", "pmx_wind_set_dts", "pmx_wind_advance_step", "pmx_wind_split_source",
-           "pmx_wind_run", "pmx_wind_sync", "pmx_wind_timing"]
+           "pmx_wind_run", "pmx_wind_sync", "pmx_wind_timing",
+           "pmx_mlp_create", "pmx_mlp_destroy", "pmx_mlp_set_params", "pmx_mlp_get_params",
+           "pmx_mlp_sample", "pmx_mlp_set_dataset", "pmx_mlp_get_dataset",
+           "pmx_mlp_loss_grad", "pmx_mlp_train", "pmx_mlp_predict",
+           "pmx_wind_set_surrogate", "pmx_wind_surrogate_stats", "pmx_mlp_cool_cells"]
 
 
 class PmxError(RuntimeError):
@@ -115,6 +119,24 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     lib.pmx_wind_run.argtypes = [vp, ctypes.c_int]
     lib.pmx_wind_sync.argtypes = [vp]
     lib.pmx_wind_timing.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ip, dp, ip]
+    fp = ctypes.POINTER(ctypes.c_float)
+    ll = ctypes.c_longlong
+    lib.pmx_mlp_create.argtypes = [ctypes.c_ulonglong, ctypes.POINTER(vp)]
+    lib.pmx_mlp_destroy.argtypes = [vp]
+    lib.pmx_mlp_set_params.argtypes = [vp, fp]
+    lib.pmx_mlp_get_params.argtypes = [vp, fp]
+    lib.pmx_mlp_sample.argtypes = [vp, ctypes.POINTER(WindDesc), ll, ctypes.c_ulonglong]
+    lib.pmx_mlp_set_dataset.argtypes = [vp, fp, fp, ll]
+    lib.pmx_mlp_get_dataset.argtypes = [vp, fp, fp, ip, dp, ll]
+    lib.pmx_mlp_loss_grad.argtypes = [vp, ll, fp, fp]
+    lib.pmx_mlp_train.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                  ctypes.c_float, fp]
+    lib.pmx_mlp_predict.argtypes = [vp, fp, ll, fp]
+    llp = ctypes.POINTER(ctypes.c_longlong)
+    lib.pmx_wind_set_surrogate.argtypes = [vp, vp, ctypes.c_int]
+    lib.pmx_mlp_cool_cells.argtypes = [vp, ctypes.POINTER(WindDesc), ll, dp, dp, dp, dp, dp,
+                                       ctypes.c_int, llp, dp]
+    lib.pmx_wind_surrogate_stats.argtypes = [vp, llp, llp, llp, llp, ctypes.c_int]
     for name in EXPORTS:
         if name != "pmx_last_error":
             getattr(lib, name).restype = ctypes.c_int
@@ -236,6 +258,118 @@ class Hydro:
         return ms.value, n.value
 
 
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def wind_desc_scalars(cfg: setups.WindConfig) -> WindDesc:
+    """A descriptor carrying only the scalar physics of a wind setup (units,
+    gamma, mu, T_x, ...), as the surrogate's sampler needs."""
+    d = WindDesc()
+    d.nx1, d.nx2, d.cooling = cfg.nx[0], cfg.nx[1], cfg.cooling
+    for q in range(4):
+        d.bc[q] = cfg.bc[q]
+    for name in ("gamma", "cfl", "cfl_max_var", "mu", "l_star", "f_x", "t_x", "krad",
+                 "alpharad", "cent_mass", "unit_density", "unit_length", "unit_velocity"):
+        setattr(d, name, getattr(cfg, name))
+    d.nangles = cfg.nangles
+    return d
+
+
+class Mlp:
+    """The MLP cooling surrogate (4 -> 64 -> 64 -> 64 -> 1, SiLU, BF16
+    matrix-core operands, FP32 accumulation) trained on the GPU."""
+
+    NP = 8705
+
+    def __init__(self, seed: int = 1):
+        lib = load_library()
+        h = ctypes.c_void_p()
+        _check(lib.pmx_mlp_create(seed, ctypes.byref(h)), "pmx_mlp_create")
+        self._h, self._lib = h, lib
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.pmx_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def get_params(self) -> np.ndarray:
+        P = np.empty(self.NP, dtype=np.float32)
+        _check(self._lib.pmx_mlp_get_params(self._h, _fptr(P)), "get_params")
+        return P
+
+    def set_params(self, P: np.ndarray) -> None:
+        P = np.ascontiguousarray(P, dtype=np.float32)
+        _check(self._lib.pmx_mlp_set_params(self._h, _fptr(P)), "set_params")
+
+    def sample(self, cfg: setups.WindConfig, n: int, seed: int = 7) -> None:
+        d = wind_desc_scalars(cfg)
+        _check(self._lib.pmx_mlp_sample(self._h, ctypes.byref(d), n, seed), "sample")
+
+    def set_dataset(self, X: np.ndarray, Y: np.ndarray) -> None:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        _check(self._lib.pmx_mlp_set_dataset(self._h, _fptr(X), _fptr(Y), len(Y)), "set_dataset")
+
+    def get_dataset(self, n: int):
+        """(X[n][4], Y[n], nev[n], aux[n][4]) of the first n samples."""
+        X = np.empty((n, 4), dtype=np.float32)
+        Y = np.empty(n, dtype=np.float32)
+        nev = np.empty(n, dtype=np.int32)
+        aux = np.empty((n, 4), dtype=np.float64)
+        _check(self._lib.pmx_mlp_get_dataset(self._h, _fptr(X), _fptr(Y),
+                                             nev.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                             _dptr(aux), n), "get_dataset")
+        return X, Y, nev, aux
+
+    def loss_grad(self, count: int):
+        g = np.empty(self.NP, dtype=np.float32)
+        loss = ctypes.c_float()
+        _check(self._lib.pmx_mlp_loss_grad(self._h, count, ctypes.byref(loss), _fptr(g)),
+               "loss_grad")
+        return loss.value, g
+
+    def train(self, steps: int, batch: int = 4096, lr0: float = 3e-3,
+              lr1: float = 1e-5) -> np.ndarray:
+        hist = np.empty(steps, dtype=np.float32)
+        _check(self._lib.pmx_mlp_train(self._h, steps, batch, lr0, lr1, _fptr(hist)), "train")
+        return hist
+
+    def cool_cells(self, cfg: setups.WindConfig, rho, p, xi, dt, mode: int = 1):
+        """One cooling step of independent cells (code units): mode 0 the
+        plain integrator, 1 the surrogate, 2 the surrogate without the
+        network.  Returns (p_new, stats dict, device ms)."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, p, xi, dt)]
+        n = len(arrs[0])
+        out = np.empty(n)
+        st = (ctypes.c_longlong * 4)()
+        ms = ctypes.c_double()
+        d = wind_desc_scalars(cfg)
+        _check(self._lib.pmx_mlp_cool_cells(self._h, ctypes.byref(d), n, *(_dptr(a) for a in arrs),
+                                            _dptr(out), mode, st, ctypes.byref(ms)), "cool_cells")
+        return out, {"listed": st[0], "accepted": st[1], "fallback": st[2],
+                     "radiat_calls": st[3]}, ms.value
+
+    def predict(self, X: np.ndarray) -> np.ndarray:
+        """log10(T_final / T_0) predicted for raw features X[n][4]."""
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.empty(len(X), dtype=np.float32)
+        _check(self._lib.pmx_mlp_predict(self._h, _fptr(X), len(X), _fptr(Y)), "predict")
+        return Y
+
+
 class Wind:
     """The line-driven disk wind (2-D spherical) advanced on the GPU.
 
@@ -338,6 +472,21 @@ class Wind:
 
     def sync(self) -> None:
         _check(self._lib.pmx_wind_sync(self._h), "sync")
+
+    def set_surrogate(self, mlp: "Mlp | None", use_network: bool = True) -> None:
+        """Surrogate-cooled SplitSource; use_network=False is the ablation
+        that starts the same refinement without the network's guess."""
+        self._mlp = mlp   # the context keeps a pointer to it
+        _check(self._lib.pmx_wind_set_surrogate(self._h, mlp._h if mlp else None,
+                                                0 if use_network else 1), "set_surrogate")
+
+    def surrogate_stats(self, reset: bool = False) -> dict:
+        a, b, c, d = (ctypes.c_longlong() for _ in range(4))
+        _check(self._lib.pmx_wind_surrogate_stats(self._h, ctypes.byref(a), ctypes.byref(b),
+                                                  ctypes.byref(c), ctypes.byref(d), int(reset)),
+               "stats")
+        return {"listed": a.value, "accepted": b.value, "fallback": c.value,
+                "radiat_calls": d.value}
 
     def timing(self, enable: bool, reset: bool = True) -> dict:
         sm, cm = ctypes.c_double(), ctypes.c_double()

@@ -1,0 +1,131 @@
+"""GPU tests of the surrogate-cooled SplitSource (MLP guess + Newton
+refinement of the cooling ODE's time integral, plain fallback).
+
+The workload is the one the network is trained for: independent cells drawn
+log-uniformly over rho 1e-16..1e-10 g/cm^3, T 1e4..1e8 K, xi 1..1e6 and
+dt 1e-2..1e2 t_cool.  The plain GPU integrator (the reference's
+CoolingSource) is the judge: the surrogate must agree with it to 1e-3 in
+temperature on at least 99.5% of the cells and be at least 3x faster.
+Where the two disagree, the oracle's exact time integral shows which end
+state solves the cooling ODE.
+"""
+import numpy as np
+import pytest
+
+import setups
+from oracle import refrun
+
+pytestmark = pytest.mark.gpu
+
+CFG = setups.c3_wind(16, 8)
+
+
+@pytest.fixture(scope="module")
+def pmx():
+    import plutomx
+    plutomx.load_library()
+    return plutomx
+
+
+@pytest.fixture(scope="module")
+def trained(pmx):
+    m = pmx.Mlp(seed=11)
+    m.sample(CFG, 500_000, seed=5)
+    hist = m.train(8000, batch=8192, lr0=3e-3, lr1=1e-5)
+    assert hist[-50:].mean() < 1e-3
+    yield m
+    m.close()
+
+
+@pytest.fixture(scope="module")
+def cells(trained):
+    m = trained
+    # a held-out set: the sampler labels it with the plain integrator; the
+    # training set is not needed any more
+    m.sample(CFG, 65536, seed=99)
+    X, Y, nev, aux = m.get_dataset(65536)
+    ok = np.isfinite(Y)
+    return tuple(aux[ok, q].copy() for q in range(4))
+
+
+@pytest.fixture(scope="module")
+def runs(trained, cells):
+    rho, p0, xi, dt = cells
+    out = {}
+    for mode in (0, 1, 2, 0, 1):        # second passes are the timed ones
+        out[mode] = trained.cool_cells(CFG, rho, p0, xi, dt, mode)
+    return out
+
+
+def rel_err(a, b):
+    return np.abs(a / b - 1.0)
+
+
+def test_surrogate_matches_plain_and_is_faster(runs):
+    pp, stp, msp = runs[0]
+    ps, sts, mss = runs[1]
+    e = rel_err(ps, pp)
+    assert np.mean(e <= 1e-3) >= 0.995, np.mean(e <= 1e-3)
+    assert np.quantile(e, 0.99) <= 2e-4, np.quantile(e, 0.99)
+    assert sts["listed"] > 0 and sts["accepted"] >= 0.99 * sts["listed"], sts
+    assert 5 * sts["radiat_calls"] < stp["radiat_calls"], (sts, stp)
+    assert msp / mss >= 3.0, (msp, mss)
+
+
+def ode_residual(rho, p0, xi, dt, p1):
+    """How far p1 is from the exact end state of the cooling ODE over dt:
+    |t(p0 -> p1) / dt - 1| for a state on the way, 0 for an equilibrium
+    that is reached within dt (a point 1e-6 short of it is reached in time
+    and a point 1e-4 beyond it is unreachable), inf if p1 cannot be
+    reached."""
+    s = np.sign(p1 - p0)
+    tol = 1e-10 * dt
+    short = refrun.wind_cool_time(CFG, rho, p0, xi, p1 * (1 - s * 1e-6), tol)
+    beyond = refrun.wind_cool_time(CFG, rho, p0, xi, p1 * (1 + s * 1e-4), tol)
+    if np.isfinite(short) and short <= dt and not np.isfinite(beyond):
+        return 0.0
+    t = refrun.wind_cool_time(CFG, rho, p0, xi, p1, tol)
+    return abs(t / dt - 1.0) if np.isfinite(t) else np.inf
+
+
+def test_disagreements_are_errors_of_the_plain_integrator(runs, cells):
+    """For cells beyond 1e-3, the surrogate's end state solves the cooling
+    ODE (exact time integral, oracle) and the plain integrator's does not,
+    or less well (the integrator may even step over an equilibrium)."""
+    rho, p0, xi, dt = cells
+    pp, ps = runs[0][0], runs[1][0]
+    bad = np.flatnonzero(rel_err(ps, pp) > 1e-3)[:8]
+    for q in bad:
+        rs = ode_residual(rho[q], p0[q], xi[q], dt[q], ps[q])
+        rp = ode_residual(rho[q], p0[q], xi[q], dt[q], pp[q])
+        assert rs <= 2e-5, (q, rs)
+        assert rp > rs, (q, rp, rs)
+
+
+def test_network_gain_over_unguided_refinement(runs):
+    """The same refinement started without the network (explicit step
+    limited to one e-fold) fails more often and has more cells beyond 1e-3."""
+    pp = runs[0][0]
+    (ps, sts, _), (pa, sta, _) = runs[1], runs[2]
+    assert sts["fallback"] <= sta["fallback"], (sts, sta)
+    assert np.sum(rel_err(ps, pp) > 1e-3) <= np.sum(rel_err(pa, pp) > 1e-3)
+
+
+def test_non_stiff_wind_state_is_untouched(pmx, trained):
+    """On the benchmark wind's start state no cell is stiff: the surrogate
+    path hands nothing to the network and reproduces the plain integrator
+    bit for bit."""
+    cfg = setups.c3_wind(128, 64)
+    V0 = setups.wind_ic(cfg)
+    out = []
+    for use in (False, True):
+        with pmx.Wind(cfg) as w:
+            w.set_prim(V0)
+            w.set_time(0.0, 1e-4, 0)
+            if use:
+                w.set_surrogate(trained)
+            w.split_source()
+            w.sync()
+            out.append((w.get_prim(), w.surrogate_stats()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert out[1][1]["listed"] == 0
