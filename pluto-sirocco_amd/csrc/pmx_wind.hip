@@ -92,13 +92,9 @@ __device__ __forceinline__ void face(const double* vL, const double* vR, const C
 // One RK2 stage (Src/Time_Stepping/rk_step.c:117-330 with UpdateStage,
 // Src/Time_Stepping/update_stage.c:35-410, for GEOMETRY SPHERICAL in 2-D).
 template <int STAGE>
-__global__ __launch_bounds__(BX* BY) void wind_stage_kernel(StageArgs a) {
-  const int i = blockIdx.x * BX + threadIdx.x;
-  const int j = blockIdx.y * BY + threadIdx.y;
-  const bool active = i < a.nr && j < a.nt;
-  double cdt = 0.0;
-  int flags = 0;
-  if (active) {
+__device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, double& cdt,
+                                           int& flags) {
+  {
     const Const& k = a.k;
     const Tables& T = a.T;
     const double dt = a.s->dt;
@@ -310,10 +306,15 @@ __global__ __launch_bounds__(BX* BY) void wind_stage_kernel(StageArgs a) {
       cdt += 0.5 * (cm2 + cp2) * (T.inv_dx2[J] * (1.0 / T.x1[I])) * 1.0;
     }
   }
-  // block reductions: max C_dt (stage 1), OR of the cons->prim flags
-  __shared__ double smax[BX * BY / 64];
-  __shared__ int sflag[BX * BY / 64];
-  const int tid = threadIdx.y * BX + threadIdx.x;
+}
+
+// Block reduction of a stage: max C_dt (stage 1) and OR of the cons->prim
+// flags into the grid's scalars.  Every thread of the block calls it.
+template <int STAGE, int NT>
+__device__ __forceinline__ bool block_reduce_stage(Scalars* s, double cdt, int flags) {
+  __shared__ double smax[NT / 64];
+  __shared__ int sflag[NT / 64];
+  const int tid = threadIdx.y * blockDim.x + threadIdx.x;
   for (int off = 32; off > 0; off >>= 1) {
     cdt = dmax(cdt, __shfl_xor(cdt, off));
     flags |= __shfl_xor(flags, off);
@@ -323,10 +324,23 @@ __global__ __launch_bounds__(BX* BY) void wind_stage_kernel(StageArgs a) {
   if (tid == 0) {
     double m = smax[0];
     int f = sflag[0];
-    for (int q = 1; q < BX * BY / 64; q++) { m = dmax(m, smax[q]); f |= sflag[q]; }
+    for (int q = 1; q < NT / 64; q++) { m = dmax(m, smax[q]); f |= sflag[q]; }
     if (STAGE == 1 && m > 0.0)
-      atomicMax(&a.s->cdt_bits, (unsigned long long)__double_as_longlong(m));
-    if (f) atomicOr(&a.s->err, f);
+      atomicMax(&s->cdt_bits, (unsigned long long)__double_as_longlong(m));
+    if (f) atomicOr(&s->err, f);
+  }
+  __syncthreads();   // the shared slots are reused by the next call
+  return tid == 0;
+}
+
+template <int STAGE>
+__global__ __launch_bounds__(BX* BY) void wind_stage_kernel(StageArgs a) {
+  const int i = blockIdx.x * BX + threadIdx.x;
+  const int j = blockIdx.y * BY + threadIdx.y;
+  double cdt = 0.0;
+  int flags = 0;
+  if (i < a.nr && j < a.nt) cell_stage<STAGE>(a, i, j, cdt, flags);
+  if (block_reduce_stage<STAGE, BX * BY>(a.s, cdt, flags)) {
     if (STAGE == 2 && blockIdx.x == 0 && blockIdx.y == 0) {
       // update_stage.c:389-394: invDt = MAX(invDt, C_dt) over the domain,
       // then / DIMENSIONS (stage-1 values, complete when stage 2 runs)
