@@ -189,6 +189,44 @@ int pmx_wind_timing(pmx_wind *w, int enable, int reset, double *stage_ms, int *s
                     double *cool_ms, int *cool_launches);
 
 /* ------------------------------------------------------------------ */
+/* Batched sweep (no counterpart in the reference, which advances one    */
+/* grid per process): many independent wind grids of uneven sizes, each  */
+/* with its own descriptor (sizes, L_star, KRAD, ...), advanced together: */
+/* one persistent kernel per RK stage over the (grid, tile) work list,   */
+/* one batched cooling step, per-grid dt on the device, optional HIP      */
+/* graph replay of the step loop.  All grids share the cooling mode       */
+/* (PMX_COOL_NONE or PMX_COOL_PLAIN) and the step parity.                 */
+/* ------------------------------------------------------------------ */
+typedef struct pmx_wind_batch pmx_wind_batch;
+
+int pmx_wind_batch_create(const pmx_wind_desc *descs, int ngrids, pmx_wind_batch **out);
+int pmx_wind_batch_destroy(pmx_wind_batch *b);
+/* The batch's context of grid g, for pmx_wind_set_prim / get_prim /
+ * get_cons / set_heatcool / set_time / get_time (owned by the batch; do
+ * not destroy it, do not run it on its own). */
+int pmx_wind_batch_grid(pmx_wind_batch *b, int g, pmx_wind **w);
+/* g_stepNumber of the next step (Strang order and the dt cadence). */
+int pmx_wind_batch_set_step(pmx_wind_batch *b, long step);
+/* Surrogate-cooled SplitSource for every grid (see pmx_wind_set_surrogate). */
+int pmx_wind_batch_set_surrogate(pmx_wind_batch *b, pmx_mlp *m, int guess);
+/* Times the stage kernel for each tile shape (64x2, 32x4, 16x8, 128x1,
+ * 64x4, 32x8 cells) and persistent workgroup count (one per tile, or 1, 2,
+ * 4, 8 per compute unit) and keeps the fastest; the state is unchanged. */
+int pmx_wind_batch_autotune(pmx_wind_batch *b, int *shape, int *blocks, double *ms_per_stage);
+/* Sets the tile shape (index as above) and workgroup count (<= 0: one per tile). */
+int pmx_wind_batch_set_shape(pmx_wind_batch *b, int shape, int blocks);
+/* nsteps main-loop iterations of every grid; use_graph != 0 replays a
+ * captured (even, odd) pair of steps.  Asynchronous. */
+int pmx_wind_batch_run(pmx_wind_batch *b, int nsteps, int use_graph);
+int pmx_wind_batch_sync(pmx_wind_batch *b);
+/* Per-kernel timing (stage and cooling launches) while enabled; graph
+ * replay is not used while timing is enabled. */
+int pmx_wind_batch_timing(pmx_wind_batch *b, int enable, int reset, double *stage_ms,
+                          int *stage_launches, double *cool_ms, int *cool_launches);
+/* Interior cells of all grids. */
+int pmx_wind_batch_cells(pmx_wind_batch *b, long long *cells);
+
+/* ------------------------------------------------------------------ */
 /* MLP cooling surrogate (no counterpart in the reference).              */
 /*                                                                      */
 /* 4 -> 64 -> 64 -> 64 -> 1, SiLU, BF16 weights and activations on the   */

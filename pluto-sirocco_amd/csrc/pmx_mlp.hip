@@ -370,6 +370,8 @@ __global__ __launch_bounds__(64) void mlp_predict_kernel(Weights w, const float*
 // ---------------------------------------------------------------------
 using wind::CoolArgs;
 using wind::SurrWork;
+using wind::CoolBatch;
+using wind::find_grid;
 
 // Appends the lanes with `pred` to a list: one atomic per wavefront.
 // Every lane of the wavefront must call it.
@@ -413,6 +415,29 @@ __device__ __forceinline__ RadCell cell_state(const CoolArgs& a, long long cell)
   return R;
 }
 
+// dt_cool candidates and fallback flags of lanes that may belong to
+// different grids: one reduction per wavefront when all its lanes share a
+// grid, per-lane atomics otherwise.
+__device__ __forceinline__ void reduce_dtcool_any(double cand, int bad, wind::Scalars* s) {
+  const unsigned long long key = (unsigned long long)s;
+  const unsigned long long k0 = __shfl(key, 0);
+  if (__all(key == k0)) {
+    for (int off = 32; off > 0; off >>= 1) {
+      cand = dmin(cand, __shfl_xor(cand, off));
+      bad |= __shfl_xor(bad, off);
+    }
+    if ((threadIdx.x & 63) == 0 && s) {
+      if (cand >= 0.0 && cand < 1.0e300)
+        atomicMin(&s->dtcool_bits, (unsigned long long)__double_as_longlong(cand));
+      if (bad) atomicOr(&s->err, 256);
+    }
+  } else if (s) {
+    if (cand >= 0.0 && cand < 1.0e300)
+      atomicMin(&s->dtcool_bits, (unsigned long long)__double_as_longlong(cand));
+    if (bad) atomicOr(&s->err, 256);
+  }
+}
+
 __global__ void surr_reset_kernel(unsigned long long* cnt) {
   if (threadIdx.x < 4) cnt[threadIdx.x] = 0ull;
 }
@@ -431,56 +456,65 @@ __device__ __forceinline__ void count_calls(unsigned long long* ctr, int n) {
 // Non-stiff cells (|dE/dt| <= 0.5 E/dt) take the reference integrator here
 // and keep its exact result; stiff cells inside the training hull are
 // listed for the network, the others queued for the plain integrator.
-__global__ __launch_bounds__(128) void surr_classify_kernel(CoolArgs a, SurrWork w) {
-  const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// Threads run over the global cell index of all grids in `b`.
+__global__ __launch_bounds__(128) void surr_classify_kernel(CoolBatch b, SurrWork w) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double cand = 1.0e300;
   int bad = 0;
   bool to_list = false, to_fb = false;
   float f[4] = {0.f, 0.f, 0.f, 0.f};
   double k1 = 0.0;
   int ncalls = 0;
-  if (cell < a.sv) {
-    const Const& k = a.k;
-    const double dt = a.dtc ? a.dtc[cell] : a.s->dt, t = a.s->t;
-    RadCell R = cell_state(a, cell);
-    const double p = a.V[PRS * a.sv + cell];
-    double E0 = p / (k.gamma - 1.0);
-    k1 = wind::radiat(k, R, E0);
-    if (!(fabs(k1) > 0.5 / dt)) {
-      a.V[PRS * a.sv + cell] = wind::plain_cell(k, R, p, dt, cand, bad);
-      ncalls = R.nev;
-    } else {
-      ncalls = R.nev;
-      double xi, tx;
-      if (t < 3.0) {
-        const double nH = R.rho * k.UD / (1.43 * wind::C_mp), r = R.x1 * k.UL;
-        xi = k.lx / nH / r / r;
-        tx = k.t_x;
+  wind::Scalars* sg = nullptr;
+  if (q < b.total) {
+    const int g = find_grid(b.off, b.G, q);
+    const CoolArgs& a = b.args[g];
+    const long long cell = q - b.off[g];
+    sg = a.s;
+    if (cell < a.sv) {
+      const Const& k = a.k;
+      const double dt = a.dtc ? a.dtc[cell] : a.s->dt, t = a.s->t;
+      RadCell R = cell_state(a, cell);
+      const double p = a.V[PRS * a.sv + cell];
+      double E0 = p / (k.gamma - 1.0);
+      k1 = wind::radiat(k, R, E0);
+      if (!(fabs(k1) > 0.5 / dt)) {
+        a.V[PRS * a.sv + cell] = wind::plain_cell(k, R, p, dt, cand, bad);
+        ncalls = R.nev;
       } else {
-        xi = R.xi_s;
-        tx = R.tr_s;
+        ncalls = R.nev;
+        double xi, tx;
+        if (t < 3.0) {
+          const double nH = R.rho * k.UD / (1.43 * wind::C_mp), r = R.x1 * k.UL;
+          xi = k.lx / nH / r / r;
+          tx = k.t_x;
+        } else {
+          xi = R.xi_s;
+          tx = R.tr_s;
+        }
+        const double T0 = p / R.rho * k.KELVIN * k.mu;
+        const double x[4] = {log10(R.rho * k.UD), log10(T0), log10(xi),
+                             log10(dt / (E0 / fabs(k1)))};
+        bool in = tx == k.t_x && E0 > 0.0;
+        for (int c = 0; c < 4; c++) {
+          in = in && x[c] >= HULL_LO[c] && x[c] <= HULL_HI[c];
+          f[c] = (float)x[c];
+        }
+        to_list = in;
+        to_fb = !in;
       }
-      const double T0 = p / R.rho * k.KELVIN * k.mu;
-      const double x[4] = {log10(R.rho * k.UD), log10(T0), log10(xi), log10(dt / (E0 / fabs(k1)))};
-      bool in = tx == k.t_x && E0 > 0.0;
-      for (int q = 0; q < 4; q++) {
-        in = in && x[q] >= HULL_LO[q] && x[q] <= HULL_HI[q];
-        f[q] = (float)x[q];
-      }
-      to_list = in;
-      to_fb = !in;
     }
   }
   const long long li = wave_append(&w.cnt[0], to_list);
   const long long fi = wave_append(&w.cnt[1], to_fb);
   if (to_list) {
-    w.list[li] = (int)cell;
-    for (int q = 0; q < 4; q++) w.feat[4 * li + q] = f[q];
+    w.list[li] = q;
+    for (int c = 0; c < 4; c++) w.feat[4 * li + c] = f[c];
     w.f0[li] = k1;
   }
-  if (to_fb) w.fb[fi] = (int)cell;
+  if (to_fb) w.fb[fi] = q;
   count_calls(&w.cnt[3], ncalls);
-  reduce_dtcool(cand, bad, a.s);
+  reduce_dtcool_any(cand, bad, sg);
 }
 
 // Network forward pass over the listed cells, 16 per wavefront.
@@ -515,24 +549,29 @@ __global__ __launch_bounds__(64) void surr_mlp_kernel(Weights wt, SurrWork w) {
 }
 
 // Newton refinement of the network's guess; failures join the queue.
-__global__ __launch_bounds__(128) void surr_refine_kernel(CoolArgs a, SurrWork w) {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(128) void surr_refine_kernel(CoolBatch b, SurrWork w) {
+  const long long qi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long n = (long long)w.cnt[0];
   if ((long long)blockIdx.x * blockDim.x >= n) return;   // uniform over the workgroup
   double cand = 1.0e300;
   bool fail = false, ok = false;
-  long long cell = 0;
+  long long q = 0;
   int ncalls = 0;
-  if (q < n) {
+  wind::Scalars* sg = nullptr;
+  if (qi < n) {
+    q = w.list[qi];
+    const int g = find_grid(b.off, b.G, q);
+    const CoolArgs& a = b.args[g];
+    const long long cell = q - b.off[g];
+    sg = a.s;
     const Const& k = a.k;
-    cell = w.list[q];
     RadCell R = cell_state(a, cell);
     const double p = a.V[PRS * a.sv + cell], dt = a.dtc ? a.dtc[cell] : a.s->dt;
     const double E0 = p / (k.gamma - 1.0);
     // guess: the network's, or (ablation) the explicit step limited to one
     // e-fold, which is what a solver without the network starts from
-    const double f0 = w.f0[q];
-    const double Eg = w.guess == 0 ? E0 * pow(10.0, (double)w.yhat[q])
+    const double f0 = w.f0[qi];
+    const double Eg = w.guess == 0 ? E0 * pow(10.0, (double)w.yhat[qi])
                                    : E0 * exp(fmax(-1.0, fmin(1.0, f0 * dt / E0)));
     double Ef;
     if (wind::refine_cell(k, R, E0, f0, dt, Eg, Ef) > 0) {
@@ -544,43 +583,56 @@ __global__ __launch_bounds__(128) void surr_refine_kernel(CoolArgs a, SurrWork w
     ncalls = R.nev;
   }
   const long long fi = wave_append(&w.cnt[1], fail);
-  if (fail) w.fb[fi] = (int)cell;
+  if (fail) w.fb[fi] = q;
   wave_append(&w.cnt[2], ok);
   count_calls(&w.cnt[3], ncalls);
-  reduce_dtcool(cand, 0, a.s);
+  reduce_dtcool_any(cand, 0, sg);
 }
 
 // The plain integrator for the queued cells.
-__global__ __launch_bounds__(128) void surr_fallback_kernel(CoolArgs a, SurrWork w) {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(128) void surr_fallback_kernel(CoolBatch b, SurrWork w) {
+  const long long qi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long n = (long long)w.cnt[1];
   if ((long long)blockIdx.x * blockDim.x >= n) return;   // uniform over the workgroup
   double cand = 1.0e300;
   int bad = 0, ncalls = 0;
-  if (q < n) {
-    const long long cell = w.fb[q];
+  wind::Scalars* sg = nullptr;
+  if (qi < n) {
+    const long long q = w.fb[qi];
+    const int g = find_grid(b.off, b.G, q);
+    const CoolArgs& a = b.args[g];
+    const long long cell = q - b.off[g];
+    sg = a.s;
     RadCell R = cell_state(a, cell);
     const double p = a.V[PRS * a.sv + cell];
     a.V[PRS * a.sv + cell] = wind::plain_cell(a.k, R, p, a.dtc ? a.dtc[cell] : a.s->dt, cand, bad);
     ncalls = R.nev;
   }
   count_calls(&w.cnt[3], ncalls);
-  reduce_dtcool(cand, bad, a.s);
+  reduce_dtcool_any(cand, bad, sg);
 }
 
-// The plain integrator on every cell (independent cells with their own dt).
-__global__ __launch_bounds__(128) void plain_cells_kernel(CoolArgs a, unsigned long long* calls) {
-  const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// The plain integrator on every cell of every grid in `b`.
+__global__ __launch_bounds__(128) void plain_cells_kernel(CoolBatch b, unsigned long long* calls) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double cand = 1.0e300;
   int bad = 0, ncalls = 0;
-  if (cell < a.sv) {
-    RadCell R = cell_state(a, cell);
-    const double p = a.V[PRS * a.sv + cell];
-    a.V[PRS * a.sv + cell] = wind::plain_cell(a.k, R, p, a.dtc ? a.dtc[cell] : a.s->dt, cand, bad);
-    ncalls = R.nev;
+  wind::Scalars* sg = nullptr;
+  if (q < b.total) {
+    const int g = find_grid(b.off, b.G, q);
+    const CoolArgs& a = b.args[g];
+    const long long cell = q - b.off[g];
+    sg = a.s;
+    if (cell < a.sv) {
+      RadCell R = cell_state(a, cell);
+      const double p = a.V[PRS * a.sv + cell];
+      a.V[PRS * a.sv + cell] =
+          wind::plain_cell(a.k, R, p, a.dtc ? a.dtc[cell] : a.s->dt, cand, bad);
+      ncalls = R.nev;
+    }
   }
   count_calls(calls, ncalls);
-  reduce_dtcool(cand, bad, a.s);
+  reduce_dtcool_any(cand, bad, sg);
 }
 
 }  // namespace mlp
@@ -638,14 +690,22 @@ int ensure_dataset(pmx_mlp* m, long long n) {
 
 namespace pmx {
 namespace mlp {
-int surrogate_cool(pmx_mlp* m, const wind::CoolArgs& a, const wind::SurrWork& w,
+int plain_cool(const wind::CoolBatch& b, unsigned long long* calls, hipStream_t st) {
+  hipLaunchKernelGGL(plain_cells_kernel, dim3((unsigned)((b.total + 127) / 128)), dim3(128), 0, st,
+                     b, calls);
+  PMX_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+int surrogate_cool(pmx_mlp* m, const wind::CoolBatch& b, const wind::SurrWork& w,
                    hipStream_t st) {
-  const unsigned nb = (unsigned)((a.sv + 127) / 128);
+  const unsigned nb = (unsigned)((b.total + 127) / 128);
   hipLaunchKernelGGL(surr_reset_kernel, dim3(1), dim3(64), 0, st, w.cnt);
-  hipLaunchKernelGGL(surr_classify_kernel, dim3(nb), dim3(128), 0, st, a, w);
-  hipLaunchKernelGGL(surr_mlp_kernel, dim3((unsigned)((a.sv + 15) / 16)), dim3(64), 0, st, m->w, w);
-  hipLaunchKernelGGL(surr_refine_kernel, dim3(nb), dim3(128), 0, st, a, w);
-  hipLaunchKernelGGL(surr_fallback_kernel, dim3(nb), dim3(128), 0, st, a, w);
+  hipLaunchKernelGGL(surr_classify_kernel, dim3(nb), dim3(128), 0, st, b, w);
+  hipLaunchKernelGGL(surr_mlp_kernel, dim3((unsigned)((b.total + 15) / 16)), dim3(64), 0, st,
+                     m->w, w);
+  hipLaunchKernelGGL(surr_refine_kernel, dim3(nb), dim3(128), 0, st, b, w);
+  hipLaunchKernelGGL(surr_fallback_kernel, dim3(nb), dim3(128), 0, st, b, w);
   hipLaunchKernelGGL(surr_totals_kernel, dim3(1), dim3(64), 0, st, w.cnt);
   PMX_HIP_CHECK(hipGetLastError());
   return 0;
@@ -852,12 +912,16 @@ int pmx_mlp_cool_cells(pmx_mlp* m, const pmx_wind_desc* d, long long n, const do
   A((void**)&ddt, sizeof(double) * n);
   A((void**)&dx1, sizeof(double) * 4);
   A((void**)&ds, sizeof(wind::Scalars));
-  A((void**)&sw.list, sizeof(int) * n);
-  A((void**)&sw.fb, sizeof(int) * n);
+  A((void**)&sw.list, sizeof(long long) * n);
+  A((void**)&sw.fb, sizeof(long long) * n);
   A((void**)&sw.feat, sizeof(float) * 4 * n);
   A((void**)&sw.yhat, sizeof(float) * n);
   A((void**)&sw.f0, sizeof(double) * n);
   A((void**)&sw.cnt, sizeof(unsigned long long) * 8);
+  wind::CoolArgs* dargs = nullptr;
+  long long* doff = nullptr;
+  A((void**)&dargs, sizeof(wind::CoolArgs));
+  A((void**)&doff, sizeof(long long) * 2);
   wind::Scalars s0;
   memset(&s0, 0, sizeof(s0));
   s0.t = 5.0;    // g_time > 3: Radiat takes xi and T_r from the per-cell arrays
@@ -879,13 +943,18 @@ int pmx_mlp_cool_cells(pmx_mlp* m, const pmx_wind_desc* d, long long n, const do
     wind::CoolArgs a;
     a.V = dV; a.nr = 1; a.nt = (int)n; a.sv = n; a.k = k; a.x1 = dx1; a.xi_s = dxi; a.tr_s = dtr;
     a.s = ds; a.dtc = ddt;
+    const long long off[2] = {0, n};
+    e = hipMemcpy(dargs, &a, sizeof(a), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(doff, off, sizeof(off), hipMemcpyHostToDevice);
+    wind::CoolBatch b;
+    b.args = dargs; b.off = doff; b.G = 1; b.total = n;
     sw.guess = mode == 2 ? 1 : 0;
-    e = hipEventRecord(e0, m->stream);
+    if (e == hipSuccess) e = hipEventRecord(e0, m->stream);
     if (e == hipSuccess) {
       if (mode == 0)
         hipLaunchKernelGGL(plain_cells_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), 0,
-                           m->stream, a, sw.cnt + 7);
-      else if (surrogate_cool(m, a, sw, m->stream))
+                           m->stream, b, sw.cnt + 7);
+      else if (surrogate_cool(m, b, sw, m->stream))
         e = hipErrorUnknown;
     }
     if (e == hipSuccess) e = hipGetLastError();
@@ -899,7 +968,7 @@ int pmx_mlp_cool_cells(pmx_mlp* m, const pmx_wind_desc* d, long long n, const do
   if (e == hipSuccess) e = hipMemcpy(c, sw.cnt, sizeof(c), hipMemcpyDeviceToHost);
   hipFree(dV); hipFree(dxi); hipFree(dtr); hipFree(ddt); hipFree(dx1); hipFree(ds);
   hipFree(sw.list); hipFree(sw.fb); hipFree(sw.feat); hipFree(sw.yhat); hipFree(sw.f0);
-  hipFree(sw.cnt);
+  hipFree(sw.cnt); hipFree(dargs); hipFree(doff);
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);
   if (e != hipSuccess) { set_error(hipGetErrorString(e)); return -1; }

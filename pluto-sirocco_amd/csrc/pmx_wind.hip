@@ -462,6 +462,8 @@ struct pmx_wind {
   int stage_n, cool_n;
   pmx_mlp* mlp;        // surrogate for the stiff cells (PMX_COOL_PLAIN), or null
   SurrWork sw;
+  CoolArgs* cargs;     // device copy of this grid's cooling arguments (1 entry)
+  long long* coff;     // {0, sv}
 };
 
 namespace {
@@ -537,8 +539,7 @@ int advance(pmx_wind* w) {
   return timed_end(w, e, false);
 }
 
-int cool(pmx_wind* w) {
-  if (w->cooling == PMX_COOL_NONE) return 0;
+CoolArgs cool_args(pmx_wind* w) {
   CoolArgs a;
   a.V = w->V[0];
   a.nr = w->nr; a.nt = w->nt;
@@ -549,11 +550,19 @@ int cool(pmx_wind* w) {
   a.tr_s = w->tr_s;
   a.s = w->s;
   a.dtc = nullptr;
+  return a;
+}
+
+int cool(pmx_wind* w) {
+  if (w->cooling == PMX_COOL_NONE) return 0;
+  CoolArgs a = cool_args(w);
   dim3 grid((unsigned)((w->sv + 127) / 128)), block(128);
   hipEvent_t e[2];
   if (timed_begin(w, e)) return -1;
   if (w->mlp) {
-    if (mlp::surrogate_cool(w->mlp, a, w->sw, w->stream)) return -1;
+    CoolBatch b;
+    b.args = w->cargs; b.off = w->coff; b.G = 1; b.total = w->sv;
+    if (mlp::surrogate_cool(w->mlp, b, w->sw, w->stream)) return -1;
   } else if (w->cooling == PMX_COOL_BLONDIN)
     hipLaunchKernelGGL(wind_cool_kernel<PMX_COOL_BLONDIN>, grid, block, 0, w->stream, a,
                        w->sw.cnt + 7);
@@ -625,7 +634,7 @@ int pmx_wind_destroy(pmx_wind* w) {
   hipFree(w->V[0]); hipFree(w->V[1]); hipFree(w->U); hipFree(w->xi_s); hipFree(w->tr_s);
   hipFree(w->tab); hipFree(w->lf); hipFree(w->s);
   hipFree(w->sw.list); hipFree(w->sw.fb); hipFree(w->sw.feat); hipFree(w->sw.yhat);
-  hipFree(w->sw.f0); hipFree(w->sw.cnt);
+  hipFree(w->sw.f0); hipFree(w->sw.cnt); hipFree(w->cargs); hipFree(w->coff);
   if (w->stream) hipStreamDestroy(w->stream);
   delete w;
   return 0;
@@ -769,6 +778,8 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
   CK(hipMalloc(&w->s, sizeof(Scalars)));
   CK(hipMalloc(&w->sw.cnt, sizeof(unsigned long long) * 8));
   CK(hipMemset(w->sw.cnt, 0, sizeof(unsigned long long) * 8));
+  CK(hipMalloc(&w->cargs, sizeof(CoolArgs)));
+  CK(hipMalloc(&w->coff, sizeof(long long) * 2));
   CK(hipMemcpy(w->tab, packed.data(), sizeof(double) * packed.size(), hipMemcpyHostToDevice));
   CK(hipMemcpy(w->lf, lf.data(), sizeof(double) * lf.size(), hipMemcpyHostToDevice));
   CK(hipMemset(w->V[0], 0, vbytes));
@@ -778,6 +789,12 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
   CK(hipMemset(w->tr_s, 0, sizeof(double) * w->sv));
   for (size_t q = 0; q < parts.size(); q++) *parts[q].first = w->tab + offs[q];
   w->T.lf = w->lf;
+  {
+    CoolArgs ca = cool_args(w);
+    const long long off[2] = {0, w->sv};
+    CK(hipMemcpy(w->cargs, &ca, sizeof(ca), hipMemcpyHostToDevice));
+    CK(hipMemcpy(w->coff, off, sizeof(off), hipMemcpyHostToDevice));
+  }
   Scalars s0;
   memset(&s0, 0, sizeof(s0));
   s0.dt_cool = 1.e38;
@@ -918,8 +935,8 @@ int pmx_wind_set_surrogate(pmx_wind* w, pmx_mlp* m, int guess) {
   }
   if (m && !w->sw.list) {
     const size_t n = (size_t)w->sv;
-    PMX_HIP_CHECK(hipMalloc(&w->sw.list, sizeof(int) * n));
-    PMX_HIP_CHECK(hipMalloc(&w->sw.fb, sizeof(int) * n));
+    PMX_HIP_CHECK(hipMalloc(&w->sw.list, sizeof(long long) * n));
+    PMX_HIP_CHECK(hipMalloc(&w->sw.fb, sizeof(long long) * n));
     PMX_HIP_CHECK(hipMalloc(&w->sw.feat, sizeof(float) * 4 * n));
     PMX_HIP_CHECK(hipMalloc(&w->sw.yhat, sizeof(float) * n));
     PMX_HIP_CHECK(hipMalloc(&w->sw.f0, sizeof(double) * n));
@@ -955,6 +972,434 @@ int pmx_wind_timing(pmx_wind* w, int enable, int reset, double* stage_ms, int* s
   if (cool_launches) *cool_launches = w->cool_n;
   if (reset) { w->stage_ms = w->cool_ms = 0.0; w->stage_n = w->cool_n = 0; }
   w->timing = enable != 0;
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------
+// Batched sweep: many independent wind grids of uneven sizes advanced
+// together (no counterpart in the reference, which runs one grid per
+// process).  One persistent kernel per RK stage walks the list of (grid,
+// tile) work items; the step loop is captured in a HIP graph; the tile
+// shape and the number of persistent workgroups are chosen by timing.
+// ---------------------------------------------------------------------
+namespace pmx {
+namespace wind {
+
+__device__ __forceinline__ int find_item(const int* off, int G, int q) {
+  int lo = 0, hi = G - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= q) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <int STAGE, int TBX, int TBY>
+__global__ __launch_bounds__(TBX* TBY) void wind_batch_stage_kernel(const StageArgs* args,
+                                                                    const int* tile_off, int G,
+                                                                    int total) {
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int g = find_item(tile_off, G, item);
+    const StageArgs& a = args[g];
+    const int tx_n = (a.nr + TBX - 1) / TBX;
+    const int local = item - tile_off[g];
+    const int ty = local / tx_n, tx = local - ty * tx_n;
+    const int i = tx * TBX + threadIdx.x, j = ty * TBY + threadIdx.y;
+    double cdt = 0.0;
+    int flags = 0;
+    if (i < a.nr && j < a.nt) cell_stage<STAGE>(a, i, j, cdt, flags);
+    block_reduce_stage<STAGE, TBX * TBY>(a.s, cdt, flags);
+  }
+}
+
+// Per grid: fold the stage-1 C_dt into invDt (update_stage.c:389-394),
+// t += dt, and NextTimeStep + reset when `next` (see wind_end_kernel).
+__global__ void wind_batch_end_kernel(const StageArgs* args, int G, int next, const double* cfl,
+                                      const double* cfl_max_var) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  Scalars* s = args[g].s;
+  double inv = s->inv_dt_hyp;
+  inv = dmax(inv, __longlong_as_double((long long)s->cdt_bits));
+  inv /= (double)2;
+  s->inv_dt_hyp = inv;
+  s->cdt_bits = 0ull;
+  const double dt = s->dt;
+  s->t += dt;
+  const double dt_cool = __longlong_as_double((long long)s->dtcool_bits);
+  s->dt_cool = dt_cool;
+  if (next) {
+    double dt_hyp = 1.0 / s->inv_dt_hyp;
+    dt_hyp *= cfl[g];
+    double dtn = dt_hyp;
+    dtn = dmin(dtn, dt_cool);
+    dtn = dmin(dtn, cfl_max_var[g] * dt);
+    if (cfl_max_var[g] != 1.0) s->dt = dtn;
+    s->inv_dt_hyp = 0.0;
+    s->dtcool_bits = (unsigned long long)__double_as_longlong(1.e38);
+  }
+}
+
+}  // namespace wind
+}  // namespace pmx
+
+struct pmx_wind_batch {
+  int G;
+  int cooling;
+  std::vector<pmx_wind*> grids;
+  StageArgs* args1;       // device arrays, G entries: stage 1 (V0 -> V1), stage 2 (V1 -> V0)
+  StageArgs* args2;
+  int* tile_off;          // device, G + 1, for the current tile shape
+  int total_tiles;
+  double* cfl;            // device, G
+  double* cfl_max_var;
+  CoolArgs* cargs;        // device, G
+  long long* cell_off;    // device, G + 1 (multiples of 128)
+  long long total_cells;  // padded
+  long long cells;        // interior cells of all grids
+  int shape;              // index into the tile-shape table
+  int blocks;             // persistent workgroups per stage launch
+  hipStream_t stream;
+  hipGraphExec_t graph;
+  bool have_graph;
+  pmx_mlp* mlp;
+  SurrWork sw;
+  unsigned long long* calls;
+  long step;
+  bool timing;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pend_stage, pend_cool;
+  double stage_ms, cool_ms;
+  int stage_n, cool_n;
+};
+
+namespace {
+
+struct TileShape { int bx, by; };
+constexpr TileShape SHAPES[] = {{64, 2}, {32, 4}, {16, 8}, {128, 1}, {64, 4}, {32, 8}};
+constexpr int NSHAPES = sizeof(SHAPES) / sizeof(SHAPES[0]);
+
+template <int STAGE>
+void launch_batch_stage(pmx_wind_batch* b) {
+  const StageArgs* args = STAGE == 1 ? b->args1 : b->args2;
+  const dim3 grid(b->blocks);
+#define PMX_SHAPE(q, X, Y)                                                                   \
+  case q:                                                                                    \
+    hipLaunchKernelGGL((wind_batch_stage_kernel<STAGE, X, Y>), grid, dim3(X, Y), 0, b->stream, \
+                       args, b->tile_off, b->G, b->total_tiles);                             \
+    break;
+  switch (b->shape) {
+    PMX_SHAPE(0, 64, 2)
+    PMX_SHAPE(1, 32, 4)
+    PMX_SHAPE(2, 16, 8)
+    PMX_SHAPE(3, 128, 1)
+    PMX_SHAPE(4, 64, 4)
+    PMX_SHAPE(5, 32, 8)
+  }
+#undef PMX_SHAPE
+}
+
+int batch_set_shape(pmx_wind_batch* b, int shape, int blocks) {
+  std::vector<int> off(b->G + 1, 0);
+  for (int g = 0; g < b->G; g++) {
+    const pmx_wind* w = b->grids[g];
+    const int tx = (w->nr + SHAPES[shape].bx - 1) / SHAPES[shape].bx;
+    const int ty = (w->nt + SHAPES[shape].by - 1) / SHAPES[shape].by;
+    off[g + 1] = off[g] + tx * ty;
+  }
+  PMX_HIP_CHECK(hipMemcpy(b->tile_off, off.data(), sizeof(int) * (b->G + 1), hipMemcpyHostToDevice));
+  b->total_tiles = off[b->G];
+  b->shape = shape;
+  b->blocks = blocks > 0 ? std::min(blocks, b->total_tiles) : b->total_tiles;
+  if (b->have_graph) {
+    hipGraphExecDestroy(b->graph);
+    b->have_graph = false;
+  }
+  return 0;
+}
+
+int batch_timed(pmx_wind_batch* b, hipEvent_t* e, bool begin, bool cool) {
+  if (!b->timing) return 0;
+  if (begin) {
+    if (b->ev_pool.size() < 2) {
+      for (int q = 0; q < 64; q++) {
+        hipEvent_t ev;
+        PMX_HIP_CHECK(hipEventCreate(&ev));
+        b->ev_pool.push_back(ev);
+      }
+    }
+    e[0] = b->ev_pool.back(); b->ev_pool.pop_back();
+    e[1] = b->ev_pool.back(); b->ev_pool.pop_back();
+    PMX_HIP_CHECK(hipEventRecord(e[0], b->stream));
+  } else {
+    PMX_HIP_CHECK(hipEventRecord(e[1], b->stream));
+    (cool ? b->pend_cool : b->pend_stage).push_back({e[0], e[1]});
+  }
+  return 0;
+}
+
+int batch_advance(pmx_wind_batch* b) {
+  hipEvent_t e[2];
+  if (batch_timed(b, e, true, false)) return -1;
+  launch_batch_stage<1>(b);
+  PMX_HIP_CHECK(hipGetLastError());
+  if (batch_timed(b, e, false, false)) return -1;
+  if (batch_timed(b, e, true, false)) return -1;
+  launch_batch_stage<2>(b);
+  PMX_HIP_CHECK(hipGetLastError());
+  return batch_timed(b, e, false, false);
+}
+
+int batch_cool(pmx_wind_batch* b) {
+  if (b->cooling == PMX_COOL_NONE) return 0;
+  CoolBatch cb;
+  cb.args = b->cargs; cb.off = b->cell_off; cb.G = b->G; cb.total = b->total_cells;
+  hipEvent_t e[2];
+  if (batch_timed(b, e, true, true)) return -1;
+  if (b->mlp) {
+    if (mlp::surrogate_cool(b->mlp, cb, b->sw, b->stream)) return -1;
+  } else if (mlp::plain_cool(cb, b->calls, b->stream)) {
+    return -1;
+  }
+  return batch_timed(b, e, false, true);
+}
+
+// One main-loop iteration of every grid (the same step parity for all).
+int batch_step(pmx_wind_batch* b, long st) {
+  if (st % 2 == 0) {
+    if (batch_advance(b) || batch_cool(b)) return -1;
+  } else {
+    if (batch_cool(b) || batch_advance(b)) return -1;
+  }
+  const int next = (b->cooling == PMX_COOL_NONE) || (st % 2 == 1);
+  hipLaunchKernelGGL(wind_batch_end_kernel, dim3((b->G + 63) / 64), dim3(64), 0, b->stream,
+                     b->args1, b->G, next, b->cfl, b->cfl_max_var);
+  PMX_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmx_wind_batch_destroy(pmx_wind_batch* b) {
+  if (!b) return 0;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->have_graph) hipGraphExecDestroy(b->graph);
+  for (auto e : b->ev_pool) hipEventDestroy(e);
+  for (auto& p : b->pend_stage) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
+  for (auto& p : b->pend_cool) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
+  for (auto w : b->grids) pmx_wind_destroy(w);
+  hipFree(b->args1); hipFree(b->args2); hipFree(b->tile_off); hipFree(b->cfl);
+  hipFree(b->cfl_max_var); hipFree(b->cargs); hipFree(b->cell_off); hipFree(b->calls);
+  hipFree(b->sw.list); hipFree(b->sw.fb); hipFree(b->sw.feat); hipFree(b->sw.yhat);
+  hipFree(b->sw.f0); hipFree(b->sw.cnt);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+int pmx_wind_batch_create(const pmx_wind_desc* descs, int ngrids, pmx_wind_batch** out) {
+  if (!descs || ngrids <= 0 || !out) { set_error("bad argument"); return -1; }
+  pmx_wind_batch* b = new pmx_wind_batch();
+  b->G = ngrids;
+  b->cooling = descs[0].cooling;
+  for (int g = 0; g < ngrids; g++) {
+    if (descs[g].cooling != b->cooling || descs[g].cooling == PMX_COOL_BLONDIN) {
+      set_error("batched grids share one cooling mode: PMX_COOL_NONE or PMX_COOL_PLAIN");
+      pmx_wind_batch_destroy(b);
+      return -1;
+    }
+    pmx_wind* w = nullptr;
+    if (pmx_wind_create(&descs[g], &w)) { pmx_wind_batch_destroy(b); return -1; }
+    b->grids.push_back(w);
+  }
+#define CK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) { \
+    set_error(std::string(#x) + ": " + hipGetErrorString(e_)); pmx_wind_batch_destroy(b); return -1; } } while (0)
+  CK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+  const int G = ngrids;
+  std::vector<StageArgs> a1(G), a2(G);
+  std::vector<CoolArgs> ca(G);
+  std::vector<double> cfl(G), cmv(G);
+  std::vector<long long> coff(G + 1, 0);
+  b->cells = 0;
+  for (int g = 0; g < G; g++) {
+    pmx_wind* w = b->grids[g];
+    a1[g] = stage_args(w, w->V[0], w->V[1]);
+    a2[g] = stage_args(w, w->V[1], w->V[0]);
+    ca[g] = cool_args(w);
+    cfl[g] = w->cfl;
+    cmv[g] = w->cfl_max_var;
+    coff[g + 1] = coff[g] + (w->sv + 127) / 128 * 128;
+    b->cells += w->sv;
+  }
+  b->total_cells = coff[G];
+  CK(hipMalloc(&b->args1, sizeof(StageArgs) * G));
+  CK(hipMalloc(&b->args2, sizeof(StageArgs) * G));
+  CK(hipMalloc(&b->tile_off, sizeof(int) * (G + 1)));
+  CK(hipMalloc(&b->cfl, sizeof(double) * G));
+  CK(hipMalloc(&b->cfl_max_var, sizeof(double) * G));
+  CK(hipMalloc(&b->cargs, sizeof(CoolArgs) * G));
+  CK(hipMalloc(&b->cell_off, sizeof(long long) * (G + 1)));
+  CK(hipMalloc(&b->calls, sizeof(unsigned long long)));
+  CK(hipMemset(b->calls, 0, sizeof(unsigned long long)));
+  CK(hipMemcpy(b->args1, a1.data(), sizeof(StageArgs) * G, hipMemcpyHostToDevice));
+  CK(hipMemcpy(b->args2, a2.data(), sizeof(StageArgs) * G, hipMemcpyHostToDevice));
+  CK(hipMemcpy(b->cfl, cfl.data(), sizeof(double) * G, hipMemcpyHostToDevice));
+  CK(hipMemcpy(b->cfl_max_var, cmv.data(), sizeof(double) * G, hipMemcpyHostToDevice));
+  CK(hipMemcpy(b->cargs, ca.data(), sizeof(CoolArgs) * G, hipMemcpyHostToDevice));
+  CK(hipMemcpy(b->cell_off, coff.data(), sizeof(long long) * (G + 1), hipMemcpyHostToDevice));
+#undef CK
+  if (batch_set_shape(b, 0, 0)) { pmx_wind_batch_destroy(b); return -1; }
+  *out = b;
+  return 0;
+}
+
+int pmx_wind_batch_grid(pmx_wind_batch* b, int g, pmx_wind** w) {
+  if (!b || g < 0 || g >= b->G || !w) { set_error("bad argument"); return -1; }
+  *w = b->grids[g];
+  return 0;
+}
+
+int pmx_wind_batch_set_step(pmx_wind_batch* b, long step) {
+  if (!b) { set_error("null argument"); return -1; }
+  b->step = step;
+  return 0;
+}
+
+int pmx_wind_batch_set_surrogate(pmx_wind_batch* b, pmx_mlp* m, int guess) {
+  if (!b) { set_error("null argument"); return -1; }
+  if (m && b->cooling != PMX_COOL_PLAIN) { set_error("the surrogate needs PMX_COOL_PLAIN"); return -1; }
+  if (m && !b->sw.list) {
+    const size_t n = (size_t)b->total_cells;
+    PMX_HIP_CHECK(hipMalloc(&b->sw.list, sizeof(long long) * n));
+    PMX_HIP_CHECK(hipMalloc(&b->sw.fb, sizeof(long long) * n));
+    PMX_HIP_CHECK(hipMalloc(&b->sw.feat, sizeof(float) * 4 * n));
+    PMX_HIP_CHECK(hipMalloc(&b->sw.yhat, sizeof(float) * n));
+    PMX_HIP_CHECK(hipMalloc(&b->sw.f0, sizeof(double) * n));
+    PMX_HIP_CHECK(hipMalloc(&b->sw.cnt, sizeof(unsigned long long) * 8));
+    PMX_HIP_CHECK(hipMemset(b->sw.cnt, 0, sizeof(unsigned long long) * 8));
+  }
+  b->mlp = m;
+  b->sw.guess = guess;
+  if (b->have_graph) { hipGraphExecDestroy(b->graph); b->have_graph = false; }
+  return 0;
+}
+
+// Times stage-1 launches (they read V^n and write only V^(1), U and the
+// step reductions, which are saved and restored) for every tile shape and
+// persistent workgroup count, and keeps the fastest.
+int pmx_wind_batch_autotune(pmx_wind_batch* b, int* shape_out, int* blocks_out, double* ms_out) {
+  if (!b) { set_error("null argument"); return -1; }
+  int dev = 0, ncu = 0;
+  PMX_HIP_CHECK(hipGetDevice(&dev));
+  PMX_HIP_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  std::vector<Scalars> saved(b->G);
+  for (int g = 0; g < b->G; g++)
+    PMX_HIP_CHECK(hipMemcpy(&saved[g], b->grids[g]->s, sizeof(Scalars), hipMemcpyDeviceToHost));
+  hipEvent_t e0, e1;
+  PMX_HIP_CHECK(hipEventCreate(&e0));
+  PMX_HIP_CHECK(hipEventCreate(&e1));
+  double best = 1e300;
+  int bs = 0, bb = 0;
+  const int mult[] = {0, 1, 2, 4, 8};   // 0: one workgroup per tile
+  for (int sh = 0; sh < NSHAPES; sh++) {
+    for (int mq : mult) {
+      if (batch_set_shape(b, sh, mq * ncu)) return -1;
+      launch_batch_stage<1>(b);                 // warm-up
+      PMX_HIP_CHECK(hipEventRecord(e0, b->stream));
+      for (int r = 0; r < 3; r++) launch_batch_stage<1>(b);
+      PMX_HIP_CHECK(hipEventRecord(e1, b->stream));
+      PMX_HIP_CHECK(hipGetLastError());
+      PMX_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      PMX_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) { best = ms; bs = sh; bb = mq * ncu; }
+    }
+  }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  for (int g = 0; g < b->G; g++)
+    PMX_HIP_CHECK(hipMemcpy(b->grids[g]->s, &saved[g], sizeof(Scalars), hipMemcpyHostToDevice));
+  if (batch_set_shape(b, bs, bb)) return -1;
+  if (shape_out) *shape_out = bs;
+  if (blocks_out) *blocks_out = b->blocks;
+  if (ms_out) *ms_out = best / 3.0;
+  return 0;
+}
+
+int pmx_wind_batch_set_shape(pmx_wind_batch* b, int shape, int blocks) {
+  if (!b || shape < 0 || shape >= NSHAPES) { set_error("bad argument"); return -1; }
+  return batch_set_shape(b, shape, blocks);
+}
+
+int pmx_wind_batch_run(pmx_wind_batch* b, int nsteps, int use_graph) {
+  if (!b || nsteps < 0) { set_error("bad argument"); return -1; }
+  int n = 0;
+  if (use_graph && !b->timing) {
+    // a captured pair of steps (even, odd) replayed
+    if (b->step % 2 == 1 && n < nsteps) {
+      if (batch_step(b, b->step)) return -1;
+      b->step++;
+      n++;
+    }
+    if (nsteps - n >= 2 && !b->have_graph) {
+      hipGraph_t gph;
+      PMX_HIP_CHECK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+      int rc = batch_step(b, 0) | batch_step(b, 1);
+      hipError_t e = hipStreamEndCapture(b->stream, &gph);
+      if (rc || e != hipSuccess) { set_error("graph capture failed"); return -1; }
+      PMX_HIP_CHECK(hipGraphInstantiate(&b->graph, gph, nullptr, nullptr, 0));
+      PMX_HIP_CHECK(hipGraphDestroy(gph));
+      b->have_graph = true;
+    }
+    for (; nsteps - n >= 2; n += 2) {
+      PMX_HIP_CHECK(hipGraphLaunch(b->graph, b->stream));
+      b->step += 2;
+    }
+  }
+  for (; n < nsteps; n++) {
+    if (batch_step(b, b->step)) return -1;
+    b->step++;
+  }
+  return 0;
+}
+
+int pmx_wind_batch_sync(pmx_wind_batch* b) {
+  if (!b) { set_error("null argument"); return -1; }
+  PMX_HIP_CHECK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int pmx_wind_batch_timing(pmx_wind_batch* b, int enable, int reset, double* stage_ms,
+                          int* stage_launches, double* cool_ms, int* cool_launches) {
+  if (!b) { set_error("null argument"); return -1; }
+  for (int c = 0; c < 2; c++) {
+    auto& v = c ? b->pend_cool : b->pend_stage;
+    for (auto& p : v) {
+      float ms = 0.f;
+      PMX_HIP_CHECK(hipEventSynchronize(p.second));
+      PMX_HIP_CHECK(hipEventElapsedTime(&ms, p.first, p.second));
+      if (c) { b->cool_ms += ms; b->cool_n++; } else { b->stage_ms += ms; b->stage_n++; }
+      b->ev_pool.push_back(p.first);
+      b->ev_pool.push_back(p.second);
+    }
+    v.clear();
+  }
+  if (stage_ms) *stage_ms = b->stage_ms;
+  if (stage_launches) *stage_launches = b->stage_n;
+  if (cool_ms) *cool_ms = b->cool_ms;
+  if (cool_launches) *cool_launches = b->cool_n;
+  if (reset) { b->stage_ms = b->cool_ms = 0.0; b->stage_n = b->cool_n = 0; }
+  b->timing = enable != 0;
+  return 0;
+}
+
+int pmx_wind_batch_cells(pmx_wind_batch* b, long long* cells) {
+  if (!b || !cells) { set_error("bad argument"); return -1; }
+  *cells = b->cells;
   return 0;
 }
 

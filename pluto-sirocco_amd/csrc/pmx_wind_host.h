@@ -30,10 +30,29 @@ struct CoolArgs {
   const double* dtc;     // per-cell dt (independent cells), or null: s->dt
 };
 
+// Several grids' cooling arguments at once: grid g owns the global cell
+// indices [off[g], off[g] + args[g].sv); off[g] are multiples of 128 so a
+// 128-thread workgroup never straddles two grids.  `total` = off[G].
+struct CoolBatch {
+  const CoolArgs* args;   // device array, G entries
+  const long long* off;   // device array, G + 1 entries
+  int G;
+  long long total;
+};
+
+__device__ __forceinline__ int find_grid(const long long* off, int G, long long q) {
+  int lo = 0, hi = G - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (off[mid] <= q) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
 // Work buffers of the surrogate-cooled SplitSource (one entry per cell).
 struct SurrWork {
-  int* list;             // stiff cells that take the network + refinement
-  int* fb;               // cells that take the plain integrator
+  long long* list;       // stiff cells (global index) that take the network + refinement
+  long long* fb;         // cells that take the plain integrator
   float* feat;           // [cell-list][4] network inputs
   float* yhat;
   double* f0;            // initial rate of each listed cell
@@ -48,7 +67,9 @@ Const make_const(const pmx_wind_desc* d);
 
 namespace mlp {
 // Launches the surrogate-cooled SplitSource on `stream` (see pmx_mlp.hip).
-int surrogate_cool(pmx_mlp* m, const wind::CoolArgs& a, const wind::SurrWork& w,
+int surrogate_cool(pmx_mlp* m, const wind::CoolBatch& b, const wind::SurrWork& w,
                    hipStream_t stream);
+// The plain integrator on every cell of the batch.
+int plain_cool(const wind::CoolBatch& b, unsigned long long* calls, hipStream_t stream);
 }  // namespace mlp
 }  // namespace pmx

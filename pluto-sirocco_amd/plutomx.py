@@ -40,7 +40,11 @@ EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro
            "pmx_mlp_create", "pmx_mlp_destroy", "pmx_mlp_set_params", "pmx_mlp_get_params",
            "pmx_mlp_sample", "pmx_mlp_set_dataset", "pmx_mlp_get_dataset",
            "pmx_mlp_loss_grad", "pmx_mlp_train", "pmx_mlp_predict",
-           "pmx_wind_set_surrogate", "pmx_wind_surrogate_stats", "pmx_mlp_cool_cells"]
+           "pmx_wind_set_surrogate", "pmx_wind_surrogate_stats", "pmx_mlp_cool_cells",
+           "pmx_wind_batch_create", "pmx_wind_batch_destroy", "pmx_wind_batch_grid",
+           "pmx_wind_batch_set_step", "pmx_wind_batch_set_surrogate", "pmx_wind_batch_autotune",
+           "pmx_wind_batch_set_shape", "pmx_wind_batch_run", "pmx_wind_batch_sync",
+           "pmx_wind_batch_timing", "pmx_wind_batch_cells"]
 
 
 class PmxError(RuntimeError):
@@ -134,6 +138,18 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     lib.pmx_mlp_predict.argtypes = [vp, fp, ll, fp]
     llp = ctypes.POINTER(ctypes.c_longlong)
     lib.pmx_wind_set_surrogate.argtypes = [vp, vp, ctypes.c_int]
+    lib.pmx_wind_batch_create.argtypes = [ctypes.POINTER(WindDesc), ctypes.c_int,
+                                          ctypes.POINTER(vp)]
+    lib.pmx_wind_batch_destroy.argtypes = [vp]
+    lib.pmx_wind_batch_grid.argtypes = [vp, ctypes.c_int, ctypes.POINTER(vp)]
+    lib.pmx_wind_batch_set_step.argtypes = [vp, ctypes.c_long]
+    lib.pmx_wind_batch_set_surrogate.argtypes = [vp, vp, ctypes.c_int]
+    lib.pmx_wind_batch_autotune.argtypes = [vp, ip, ip, dp]
+    lib.pmx_wind_batch_set_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.pmx_wind_batch_run.argtypes = [vp, ctypes.c_int, ctypes.c_int]
+    lib.pmx_wind_batch_sync.argtypes = [vp]
+    lib.pmx_wind_batch_timing.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ip, dp, ip]
+    lib.pmx_wind_batch_cells.argtypes = [vp, llp]
     lib.pmx_mlp_cool_cells.argtypes = [vp, ctypes.POINTER(WindDesc), ll, dp, dp, dp, dp, dp,
                                        ctypes.c_int, llp, dp]
     lib.pmx_wind_surrogate_stats.argtypes = [vp, llp, llp, llp, llp, ctypes.c_int]
@@ -370,6 +386,25 @@ class Mlp:
         return Y
 
 
+def wind_desc(cfg: setups.WindConfig, fluxes=None, grid=None, keep=None) -> WindDesc:
+    """Full descriptor of a wind grid; the arrays it points to are appended
+    to `keep` (they must outlive the create call)."""
+    gr, gt = grid if grid is not None else setups.wind_grid(cfg)
+    fr, ft, fp = fluxes if fluxes is not None else setups.wind_fluxes(cfg)
+    keep = [] if keep is None else keep
+
+    def arr(a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        keep.append(a)
+        return _dptr(a)
+
+    d = wind_desc_scalars(cfg)
+    d.x1, d.x1l, d.x1r, d.dx1 = arr(gr["x"]), arr(gr["xl"]), arr(gr["xr"]), arr(gr["dx"])
+    d.x2, d.x2l, d.x2r, d.dx2 = arr(gt["x"]), arr(gt["xl"]), arr(gt["xr"]), arr(gt["dx"])
+    d.flux_r, d.flux_t, d.flux_p = arr(fr), arr(ft), arr(fp)
+    return d
+
+
 class Wind:
     """The line-driven disk wind (2-D spherical) advanced on the GPU.
 
@@ -377,8 +412,16 @@ class Wind:
     ``split_source()`` its SplitSource (cooling), ``run(n)`` n iterations of
     the main loop with cooling, all on the device."""
 
+    @classmethod
+    def borrowed(cls, cfg: setups.WindConfig, handle) -> "Wind":
+        """View of a context owned elsewhere (a batch's grid)."""
+        w = cls.__new__(cls)
+        w.cfg, w._h, w._lib, w._owned = cfg, handle, load_library(), False
+        return w
+
     def __init__(self, cfg: setups.WindConfig, fluxes=None, grid=None):
         self.cfg = cfg
+        self._owned = True
         lib = load_library()
         gr, gt = grid if grid is not None else setups.wind_grid(cfg)
         fr, ft, fp = fluxes if fluxes is not None else setups.wind_fluxes(cfg)
@@ -408,7 +451,8 @@ class Wind:
 
     def close(self) -> None:
         if getattr(self, "_h", None):
-            self._lib.pmx_wind_destroy(self._h)
+            if self._owned:
+                self._lib.pmx_wind_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -494,5 +538,81 @@ class Wind:
         _check(self._lib.pmx_wind_timing(self._h, int(enable), int(reset), ctypes.byref(sm),
                                          ctypes.byref(sn), ctypes.byref(cm), ctypes.byref(cn)),
                "timing")
+        return {"stage_ms": sm.value, "stage_launches": sn.value, "cool_ms": cm.value,
+                "cool_launches": cn.value}
+
+
+class WindBatch:
+    """Independent wind grids advanced together (batched persistent stage
+    kernel, batched cooling, optional HIP graph replay)."""
+
+    def __init__(self, cfgs):
+        lib = load_library()
+        self.cfgs = list(cfgs)
+        keep = []
+        descs = (WindDesc * len(self.cfgs))()
+        for g, cfg in enumerate(self.cfgs):
+            descs[g] = wind_desc(cfg, keep=keep)
+        h = ctypes.c_void_p()
+        _check(lib.pmx_wind_batch_create(descs, len(self.cfgs), ctypes.byref(h)), "batch_create")
+        self._h, self._lib = h, lib
+        self.grids = []
+        for g, cfg in enumerate(self.cfgs):
+            wh = ctypes.c_void_p()
+            _check(lib.pmx_wind_batch_grid(h, g, ctypes.byref(wh)), "batch_grid")
+            self.grids.append(Wind.borrowed(cfg, wh))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.pmx_wind_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def cells(self) -> int:
+        c = ctypes.c_longlong()
+        _check(self._lib.pmx_wind_batch_cells(self._h, ctypes.byref(c)), "cells")
+        return c.value
+
+    def set_step(self, step: int) -> None:
+        _check(self._lib.pmx_wind_batch_set_step(self._h, step), "set_step")
+
+    def set_surrogate(self, mlp: "Mlp | None", use_network: bool = True) -> None:
+        self._mlp = mlp
+        _check(self._lib.pmx_wind_batch_set_surrogate(self._h, mlp._h if mlp else None,
+                                                      0 if use_network else 1), "set_surrogate")
+
+    def autotune(self) -> dict:
+        sh, bl, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(self._lib.pmx_wind_batch_autotune(self._h, ctypes.byref(sh), ctypes.byref(bl),
+                                                 ctypes.byref(ms)), "autotune")
+        return {"shape": sh.value, "blocks": bl.value, "ms_per_stage": ms.value}
+
+    def set_shape(self, shape: int, blocks: int = 0) -> None:
+        _check(self._lib.pmx_wind_batch_set_shape(self._h, shape, blocks), "set_shape")
+
+    def run(self, nsteps: int, graph: bool = False) -> None:
+        _check(self._lib.pmx_wind_batch_run(self._h, nsteps, int(graph)), "run")
+
+    def sync(self) -> None:
+        _check(self._lib.pmx_wind_batch_sync(self._h), "sync")
+
+    def timing(self, enable: bool, reset: bool = True) -> dict:
+        sm, cm = ctypes.c_double(), ctypes.c_double()
+        sn, cn = ctypes.c_int(), ctypes.c_int()
+        _check(self._lib.pmx_wind_batch_timing(self._h, int(enable), int(reset), ctypes.byref(sm),
+                                               ctypes.byref(sn), ctypes.byref(cm),
+                                               ctypes.byref(cn)), "timing")
         return {"stage_ms": sm.value, "stage_launches": sn.value, "cool_ms": cm.value,
                 "cool_launches": cn.value}

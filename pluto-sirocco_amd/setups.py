@@ -418,3 +418,22 @@ def write_flux_files(cfg: WindConfig, directory, fluxes=None) -> None:
 
 def c3_wind(nr: int = 512, nt: int = 256, cooling: int = COOL_PLAIN) -> WindConfig:
     return WindConfig(nx=(nr, nt), cooling=cooling)
+
+
+def c5_batch(n_grids: int = 64, seed: int = 2024, cooling: int = COOL_PLAIN,
+             size_range=((128, 64), (768, 384))) -> list:
+    """Parameter sweep of independent wind grids: N_r uniform over
+    [128, 768] and N_theta uniform over [64, 384] (multiples of 8), L_x log-
+    uniform in 1e36..1e38 erg/s (L_star = L_x / f_x) and the force-multiplier
+    k uniform in 0.1..0.4, all drawn from the seeded counter-based generator."""
+    (r0, t0), (r1, t1) = size_range
+    idx = np.arange(4 * n_grids, dtype=np.uint64)
+    u = uniform01(seed, idx).reshape(n_grids, 4)
+    cfgs = []
+    for g in range(n_grids):
+        nr = int(r0 + (r1 - r0) * u[g, 0]) // 8 * 8
+        nt = int(t0 + (t1 - t0) * u[g, 1]) // 8 * 8
+        lx = 10.0 ** (36.0 + 2.0 * u[g, 2])
+        cfg = WindConfig(nx=(max(nr, r0), max(nt, t0)), cooling=cooling)
+        cfgs.append(dataclasses.replace(cfg, l_star=lx / cfg.f_x, krad=0.1 + 0.3 * u[g, 3]))
+    return cfgs
