@@ -63,6 +63,94 @@ def cpu_baseline(n_sample: int = 128, steps: int = 3) -> dict | None:
                       f"{os.cpu_count()} host cores ({wall:.2f} s)"}
 
 
+def wind_legs(steps: int = 200) -> dict:
+    """The wind workloads: C3 (512 x 256 spherical wind, plain sub-cycled
+    cooling), C4 (the same with the MLP surrogate, trained here on the GPU
+    on 2e6 samples of the plain integrator), the cooling step alone on
+    cells drawn over the surrogate's training hull (where cooling is stiff),
+    and C5 (64 uneven grids advanced as one batch)."""
+    import plutomx
+    out = {}
+    cfg = setups.c3_wind(512, 256, setups.COOL_PLAIN)
+    V0 = setups.wind_ic(cfg)
+
+    def run_wind(mlp=None):
+        with plutomx.Wind(cfg) as w:
+            w.set_prim(V0)
+            w.set_time(0.0, cfg.first_dt, 0)
+            if mlp is not None:
+                w.set_surrogate(mlp)
+            w.run(2)
+            w.sync()
+            w.timing(True, reset=True)
+            w.surrogate_stats(reset=True)
+            t0 = time.perf_counter()
+            w.run(steps)
+            w.sync()
+            el = time.perf_counter() - t0
+            tm = w.timing(False)
+            st = w.surrogate_stats()
+            T = w.get_prim()
+        return {"Mcell_updates_per_s": round(cfg.ncells * steps / el / 1e6, 2),
+                "ms_per_step": round(1e3 * el / steps, 4),
+                "stage_ms_per_launch": round(tm["stage_ms"] / max(tm["stage_launches"], 1), 4),
+                "cool_ms_per_call": round(tm["cool_ms"] / max(tm["cool_launches"], 1), 4),
+                "surrogate": st}, T
+
+    plain, Tp = run_wind()
+    out["c3_plain"] = dict(plain, workload=f"wind 512x256, PLM+HLLC+RK2, line force, plain "
+                                            f"cooling, {steps} steps")
+    m = plutomx.Mlp(seed=11)
+    t0 = time.perf_counter()
+    m.sample(cfg, 2_000_000, seed=5)
+    t1 = time.perf_counter()
+    hist = m.train(20000, batch=8192, lr0=3e-3, lr1=1e-5)
+    t2 = time.perf_counter()
+    surr, Ts = run_wind(m)
+    out["c4_surrogate"] = dict(surr, workload="C3 with the surrogate cooling (cells outside the "
+                                              "training hull take the plain integrator)",
+                               training={"samples": 2_000_000, "sample_s": round(t1 - t0, 2),
+                                         "steps": 20000, "batch": 8192,
+                                         "train_s": round(t2 - t1, 2),
+                                         "final_loss": float(np.mean(hist[-50:]))},
+                               speedup_vs_plain=round(plain["ms_per_step"] / surr["ms_per_step"], 3))
+    # cooling alone over the training hull (dt 1e-2..1e2 t_cool)
+    m.sample(cfg, 131072, seed=99)
+    X, Y, nev, aux = m.get_dataset(131072)
+    ok = np.isfinite(Y)
+    rho, p0, xi, dt = (aux[ok, q].copy() for q in range(4))
+    res = {}
+    for mode in (0, 1, 0, 1):
+        res[mode] = m.cool_cells(cfg, rho, p0, xi, dt, mode)
+    e = np.abs(res[1][0] / res[0][0] - 1.0)
+    out["cooling_on_training_hull"] = {
+        "cells": int(ok.sum()), "plain_ms": round(res[0][2], 3),
+        "surrogate_ms": round(res[1][2], 3), "speedup": round(res[0][2] / res[1][2], 2),
+        "frac_T_err_le_1e-3": float(np.mean(e <= 1e-3)), "T_err_p99": float(np.quantile(e, 0.99)),
+        "stats": res[1][1], "plain_radiat_calls": res[0][1]["radiat_calls"]}
+    m.close()
+    # C5: batched parameter sweep
+    cfgs = setups.c5_batch()
+    with plutomx.WindBatch(cfgs) as b:
+        for w, c in zip(b.grids, cfgs):
+            w.set_prim(setups.wind_ic(c))
+            w.set_time(0.0, c.first_dt, 0)
+        b.set_step(0)
+        tune = b.autotune()
+        b.run(2, graph=True)
+        b.sync()
+        t0 = time.perf_counter()
+        b.run(steps, graph=True)
+        b.sync()
+        el = time.perf_counter() - t0
+        out["c5_batch"] = {"grids": len(cfgs), "cells": b.cells,
+                           "Mcell_updates_per_s": round(b.cells * steps / el / 1e6, 2),
+                           "ms_per_step": round(1e3 * el / steps, 3), "autotune": tune,
+                           "workload": "64 wind grids 128..768 x 64..384, L_x 1e36..1e38, "
+                                       f"k 0.1..0.4, plain cooling, graph replay, {steps} steps"}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +159,10 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--graph", action="store_true", help="replay a captured HIP graph")
+    ap.add_argument("--no-wind", action="store_true", help="skip the wind workloads")
+    ap.add_argument("--wind-only", action="store_true",
+                    help="only the wind workloads (for profiling them)")
+    ap.add_argument("--wind-steps", type=int, default=200)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB megabytes (counter calibration)")
     args = ap.parse_args()
@@ -81,6 +173,9 @@ def main() -> int:
         return 2
 
     import plutomx
+    if args.wind_only:
+        print(json.dumps({"wind": wind_legs(args.wind_steps)}), flush=True)
+        return 0
     if args.copy_probe:
         nbytes = args.copy_probe * 1 << 20
         ms = plutomx.copy_probe(nbytes, reps=5)
@@ -130,6 +225,8 @@ def main() -> int:
                       "c2p_flags": flags},
            "roofline": roofline,
            "cpu_baseline": None if args.no_cpu else cpu_baseline()}
+    if not args.no_wind:
+        out["wind"] = wind_legs(args.wind_steps)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -117,6 +117,7 @@ int pmx_hydro_device_state(pmx_hydro *h, double **V, double **U);
 #define PMX_COOL_PLAIN   2   /* cv_idl CoolingSource: RKF12 / Cash-Karp 45 */
 
 typedef struct pmx_wind pmx_wind;
+typedef struct pmx_mlp pmx_mlp;           /* the cooling surrogate, see below */
 
 /* Mirrors the reference's set-up of the wind problem: definitions.h
  * (UNIT_*), pluto.ini ([Time] CFL / CFL_max_var, [Boundary], [Parameters]
@@ -236,8 +237,6 @@ int pmx_wind_batch_cells(pmx_wind_batch *b, long long *cells);
 /* Parameters are a float32 vector of 8705 entries: W1[64][4] b1[64]     */
 /* W2[64][64] b2[64] W3[64][64] b3[64] w4[64] b4.                        */
 /* ------------------------------------------------------------------ */
-typedef struct pmx_mlp pmx_mlp;
-
 int pmx_mlp_create(unsigned long long seed, pmx_mlp **out);
 int pmx_mlp_destroy(pmx_mlp *m);
 int pmx_mlp_set_params(pmx_mlp *m, const float *P);   /* also resets Adam's moments */

@@ -136,6 +136,9 @@ __device__ __forceinline__ void phys_flux(const double* v, const double* u, doub
 // MaxSignalSpeed eigenv.c:45-65, SoundSpeed2 EOS/Ideal/eos.c:33.
 // Only the star state on the side that is selected is formed: the other
 // one never reaches the output.
+#ifndef PMX_HLLC_BRANCHLESS
+#define PMX_HLLC_BRANCHLESS 0
+#endif
 template <int DIR>
 __device__ __forceinline__ void hllc(const double* vL, const double* vR, double gamma,
                                      double gmm1, double* flux, double& press, double& cmax) {
@@ -149,6 +152,42 @@ __device__ __forceinline__ void hllc(const double* vL, const double* vR, double 
   double SL = dmin(vL[n] - aL, vR[n] - aR);
   double SR = dmax(vL[n] + aL, vR[n] + aR);
   cmax = dmax(fabs(SL), fabs(SR));
+  if (PMX_HLLC_BRANCHLESS) {
+    // Same operations on every path, selected at the end: a supersonic
+    // interface takes the upwind physical flux (hllc.c:75-84), the others
+    // the star flux of the side the contact leaves behind.
+    const bool supL = SL > 0.0, supR = !supL && SR < 0.0;
+    double qL = vL[PRS] + uL[n] * (vL[n] - SL);
+    double qR = vR[PRS] + uR[n] * (vR[n] - SR);
+    double wL = vL[RHO] * (vL[n] - SL);
+    double wR = vR[RHO] * (vR[n] - SR);
+    double vs = (qR - qL) / (wR - wL);
+    const bool left = supL || (!supR && vs >= 0.0);
+    double v[NV], u[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      v[k] = left ? vL[k] : vR[k];
+      u[k] = left ? uL[k] : uR[k];
+    }
+    double S = left ? SL : SR;
+    double vx = v[n];
+    double f[NV];
+    phys_flux<DIR>(v, u, f);
+    double us[NV];
+    us[RHO] = u[RHO] * (S - vx) / (S - vs);
+#pragma unroll
+    for (int k = VX1; k <= VX3; k++) us[k] = us[RHO] * (k == n ? vs : v[k]);
+    us[ENG] = u[ENG] / v[RHO] + (vs - vx) * (vs + v[PRS] / (v[RHO] * (S - vx)));
+    us[ENG] *= us[RHO];
+    const bool sup = supL || supR;
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+      double fs = f[k] + S * (us[k] - u[k]);
+      flux[k] = sup ? f[k] : fs;
+    }
+    press = v[PRS];
+    return;
+  }
   if (SL > 0.0) {
     phys_flux<DIR>(vL, uL, flux);
     press = vL[PRS];

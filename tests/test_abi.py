@@ -37,3 +37,17 @@ def test_python_binding_covers_exports():
 def test_missing_library_fails_loudly(tmp_path):
     with pytest.raises(plutomx.PmxError):
         plutomx.load_library(tmp_path / "nope.so")
+
+
+def test_header_compiles_as_c():
+    """include/plutomx.h is a plain C header (C callers such as PLUTO's
+    driver include it directly)."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    hdr = pathlib.Path(__file__).resolve().parents[1] / "include" / "plutomx.h"
+    r = subprocess.run([cc, "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-x", "c", str(hdr)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
