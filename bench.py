@@ -128,26 +128,32 @@ def wind_legs(steps: int = 200) -> dict:
         "surrogate_ms": round(res[1][2], 3), "speedup": round(res[0][2] / res[1][2], 2),
         "frac_T_err_le_1e-3": float(np.mean(e <= 1e-3)), "T_err_p99": float(np.quantile(e, 0.99)),
         "stats": res[1][1], "plain_radiat_calls": res[0][1]["radiat_calls"]}
-    m.close()
-    # C5: batched parameter sweep
+    # C5: batched parameter sweep, with the surrogate cooling (the network
+    # trained above; the hull does not depend on L_x or k) and with the
+    # plain integrator for comparison
     cfgs = setups.c5_batch()
-    with plutomx.WindBatch(cfgs) as b:
-        for w, c in zip(b.grids, cfgs):
-            w.set_prim(setups.wind_ic(c))
-            w.set_time(0.0, c.first_dt, 0)
-        b.set_step(0)
-        tune = b.autotune()
-        b.run(2, graph=True)
-        b.sync()
-        t0 = time.perf_counter()
-        b.run(steps, graph=True)
-        b.sync()
-        el = time.perf_counter() - t0
-        out["c5_batch"] = {"grids": len(cfgs), "cells": b.cells,
-                           "Mcell_updates_per_s": round(b.cells * steps / el / 1e6, 2),
-                           "ms_per_step": round(1e3 * el / steps, 3), "autotune": tune,
-                           "workload": "64 wind grids 128..768 x 64..384, L_x 1e36..1e38, "
-                                       f"k 0.1..0.4, plain cooling, graph replay, {steps} steps"}
+    for name, mlp in (("c5_batch", m), ("c5_batch_plain", None)):
+        with plutomx.WindBatch(cfgs) as b:
+            for w, c in zip(b.grids, cfgs):
+                w.set_prim(setups.wind_ic(c))
+                w.set_time(0.0, c.first_dt, 0)
+            b.set_step(0)
+            if mlp is not None:
+                b.set_surrogate(mlp)
+            tune = b.autotune()
+            b.run(2, graph=True)
+            b.sync()
+            t0 = time.perf_counter()
+            b.run(steps, graph=True)
+            b.sync()
+            el = time.perf_counter() - t0
+            cooling = "surrogate cooling with plain fallback" if mlp is not None else "plain cooling"
+            out[name] = {"grids": len(cfgs), "cells": b.cells,
+                         "Mcell_updates_per_s": round(b.cells * steps / el / 1e6, 2),
+                         "ms_per_step": round(1e3 * el / steps, 3), "autotune": tune,
+                         "workload": "64 wind grids 128..768 x 64..384, L_x 1e36..1e38, "
+                                     f"k 0.1..0.4, {cooling}, graph replay, {steps} steps"}
+    m.close()
     return out
 
 

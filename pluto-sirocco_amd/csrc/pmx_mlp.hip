@@ -517,12 +517,8 @@ __global__ __launch_bounds__(128) void surr_classify_kernel(CoolBatch b, SurrWor
   reduce_dtcool_any(cand, bad, sg);
 }
 
-// Network forward pass over the listed cells, 16 per wavefront.
-__global__ __launch_bounds__(64) void surr_mlp_kernel(Weights wt, SurrWork w) {
-  __shared__ FwdLds S;
-  const long long n = (long long)w.cnt[0];
-  const long long s0 = (long long)blockIdx.x * 16;
-  if (s0 >= n) return;   // uniform over the workgroup
+__device__ __forceinline__ void surr_mlp_group(const Weights& wt, const SurrWork& w, FwdLds& S,
+                                               long long n, long long s0) {
   const int l = threadIdx.x, r = l & 15, kq = (l >> 4) * 4;
   const long long sidx = s0 + r;
   const bool valid = sidx < n;
@@ -546,13 +542,21 @@ __global__ __launch_bounds__(64) void surr_mlp_kernel(Weights wt, SurrWork w) {
   __syncthreads();
   float y = output_unit(S.H3, wt, l);
   if (l < 16 && valid) w.yhat[sidx] = y * YS;
+  __syncthreads();   // S is reused by the next group
+}
+
+// Network forward pass over the listed cells, 16 per wavefront; the grid
+// strides over the list, whose length is known only on the device.
+__global__ __launch_bounds__(64) void surr_mlp_kernel(Weights wt, SurrWork w) {
+  __shared__ FwdLds S;
+  const long long n = (long long)w.cnt[0];
+  for (long long s0 = (long long)blockIdx.x * 16; s0 < n; s0 += (long long)gridDim.x * 16)
+    surr_mlp_group(wt, w, S, n, s0);
 }
 
 // Newton refinement of the network's guess; failures join the queue.
-__global__ __launch_bounds__(128) void surr_refine_kernel(CoolBatch b, SurrWork w) {
-  const long long qi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long n = (long long)w.cnt[0];
-  if ((long long)blockIdx.x * blockDim.x >= n) return;   // uniform over the workgroup
+__device__ __forceinline__ void surr_refine_cell(const CoolBatch& b, const SurrWork& w,
+                                                 long long qi, long long n) {
   double cand = 1.0e300;
   bool fail = false, ok = false;
   long long q = 0;
@@ -589,11 +593,19 @@ __global__ __launch_bounds__(128) void surr_refine_kernel(CoolBatch b, SurrWork 
   reduce_dtcool_any(cand, 0, sg);
 }
 
+// The grid strides over the list (length known only on the device); the
+// loop bound is uniform over each workgroup, so every wavefront takes part
+// in the wavefront-wide appends and reductions of every pass.
+__global__ __launch_bounds__(128) void surr_refine_kernel(CoolBatch b, SurrWork w) {
+  const long long n = (long long)w.cnt[0];
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < n;
+       base += (long long)gridDim.x * blockDim.x)
+    surr_refine_cell(b, w, base + threadIdx.x, n);
+}
+
 // The plain integrator for the queued cells.
-__global__ __launch_bounds__(128) void surr_fallback_kernel(CoolBatch b, SurrWork w) {
-  const long long qi = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long n = (long long)w.cnt[1];
-  if ((long long)blockIdx.x * blockDim.x >= n) return;   // uniform over the workgroup
+__device__ __forceinline__ void surr_fallback_cell(const CoolBatch& b, const SurrWork& w,
+                                                   long long qi, long long n) {
   double cand = 1.0e300;
   int bad = 0, ncalls = 0;
   wind::Scalars* sg = nullptr;
@@ -610,6 +622,13 @@ __global__ __launch_bounds__(128) void surr_fallback_kernel(CoolBatch b, SurrWor
   }
   count_calls(&w.cnt[3], ncalls);
   reduce_dtcool_any(cand, bad, sg);
+}
+
+__global__ __launch_bounds__(128) void surr_fallback_kernel(CoolBatch b, SurrWork w) {
+  const long long n = (long long)w.cnt[1];
+  for (long long base = (long long)blockIdx.x * blockDim.x; base < n;
+       base += (long long)gridDim.x * blockDim.x)
+    surr_fallback_cell(b, w, base + threadIdx.x, n);
 }
 
 // The plain integrator on every cell of every grid in `b`.
@@ -697,15 +716,20 @@ int plain_cool(const wind::CoolBatch& b, unsigned long long* calls, hipStream_t 
   return 0;
 }
 
+constexpr int SURR_GRID = 2048;
+
 int surrogate_cool(pmx_mlp* m, const wind::CoolBatch& b, const wind::SurrWork& w,
                    hipStream_t st) {
   const unsigned nb = (unsigned)((b.total + 127) / 128);
+  // the list and queue lengths are known only on the device: those kernels
+  // stride over them with at most SURR_GRID workgroups
+  const unsigned nr = std::min(nb, (unsigned)SURR_GRID);
+  const unsigned nm = (unsigned)std::min((b.total + 15) / 16, (long long)SURR_GRID);
   hipLaunchKernelGGL(surr_reset_kernel, dim3(1), dim3(64), 0, st, w.cnt);
   hipLaunchKernelGGL(surr_classify_kernel, dim3(nb), dim3(128), 0, st, b, w);
-  hipLaunchKernelGGL(surr_mlp_kernel, dim3((unsigned)((b.total + 15) / 16)), dim3(64), 0, st,
-                     m->w, w);
-  hipLaunchKernelGGL(surr_refine_kernel, dim3(nb), dim3(128), 0, st, b, w);
-  hipLaunchKernelGGL(surr_fallback_kernel, dim3(nb), dim3(128), 0, st, b, w);
+  hipLaunchKernelGGL(surr_mlp_kernel, dim3(nm), dim3(64), 0, st, m->w, w);
+  hipLaunchKernelGGL(surr_refine_kernel, dim3(nr), dim3(128), 0, st, b, w);
+  hipLaunchKernelGGL(surr_fallback_kernel, dim3(nr), dim3(128), 0, st, b, w);
   hipLaunchKernelGGL(surr_totals_kernel, dim3(1), dim3(64), 0, st, w.cnt);
   PMX_HIP_CHECK(hipGetLastError());
   return 0;

@@ -83,21 +83,35 @@ PMX_HD void plm_vl_general(const double* vm1, const double* v0,
 }
 
 // ---------------------------------------------------------------------
-// Line force: one angle bin of LineForce (line_connect.c:873-902) for the
-// state (rho_cgs, v_th).  Returns the (1 + M) factor.
+// Line force: the force multiplier of one angle bin of LineForce
+// (line_connect.c:873-902), 1 + M with M = krad t^alpha capped at 4400,
+// t = sigma_e rho v_th / dvds, and M = 0 where dvds <= 0 (or the -999 flag).
+//
+// Both LineForce calls of a cell (x1 state and cell centre) see the same
+// dvds per bin, so t^alpha is evaluated as A * dvds^-alpha with the
+// per-call factor A = (sigma_e rho v_th)^alpha computed once per cell:
+// one pow per bin serves both calls and the division by dvds goes away.
+// The product differs from pow(t, alpha) by a few ulp (two pow roundings,
+// one product rounding, and the rounding of t that the reference's form
+// has and this one has not); with krad = 0 both forms give M = 0 exactly.
 // ---------------------------------------------------------------------
-PMX_HD double force_mult(const Const& k, double rho_cgs, double v_th,
-                                             double dvds) {
+PMX_HD double force_factor(const Const& k, double rho_cgs, double v_th) {
+  return pow(k.sigma_e * rho_cgs * v_th, k.alpharad);
+}
+
+PMX_HD void force_mult_pair(const Const& k, double A1, double A2, double dvds,
+                            double& f1, double& f2) {
   const double M_max = 4400.;
-  double M;
+  double M1 = 0.0, M2 = 0.0;
   if (dvds > 0.0) {
-    double t_UV = k.sigma_e * rho_cgs * v_th / dvds;
-    M = k.krad * pow(t_UV, k.alpharad);
-    if (M > M_max) M = M_max;
-  } else {
-    M = 0.0;
+    double P = pow(dvds, -k.alpharad);
+    M1 = k.krad * (A1 * P);
+    M2 = k.krad * (A2 * P);
+    if (M1 > M_max) M1 = M_max;
+    if (M2 > M_max) M2 = M_max;
   }
-  return 1.0 + M;
+  f1 = 1.0 + M1;
+  f2 = 1.0 + M2;
 }
 
 // ---------------------------------------------------------------------
