@@ -30,6 +30,11 @@ extern "C" {
 #define PMX_BC_PERIODIC   0
 #define PMX_BC_OUTFLOW    1
 #define PMX_BC_REFLECTIVE 2
+/* The wind's userdef sides of the production problem
+ * (Test_Problems/LineDrivenWind/cv_idl/init.c): */
+#define PMX_BC_WIND_INNER 3   /* X1-beg: copy of the first cell, v_r = MIN(v_r, 0) (:318-329) */
+#define PMX_BC_WIND_POLE  4   /* X2-beg: reflective for the velocities and the tracer, rho and
+                                 p copied from the first cell (:343-359) */
 
 typedef struct pmx_hydro pmx_hydro;
 
@@ -121,7 +126,7 @@ typedef struct pmx_mlp pmx_mlp;           /* the cooling surrogate, see below */
 
 /* Mirrors the reference's set-up of the wind problem: definitions.h
  * (UNIT_*), pluto.ini ([Time] CFL / CFL_max_var, [Boundary], [Parameters]
- * MU, L_star, f_x, T_x, KRAD, ALPHARAD, CENT_MASS, GAMMA_EOS) and the
+ * MU, L_star, f_x, T_x, KRAD, ALPHARAD, CENT_MASS, GAMMA, DFLOOR) and the
  * Grid arrays x, xl, xr, dx of both directions WITH 2 ghost cells per side
  * (nx + 4 entries, Src/structs.h:133-187).  flux_r / flux_t / flux_p are
  * the SIROCCO directional fluxes (cgs) [nangles][nx2][nx1] that
@@ -137,6 +142,13 @@ typedef struct {
   const double *x2, *x2l, *x2r, *dx2;
   int nangles;                  /* 1 .. 64 (the reference's files hold 36) */
   const double *flux_r, *flux_t, *flux_p;
+  /* > 0: INTERNAL_BOUNDARY YES with the production problem's
+   * UserDefBoundary(side 0) (cv_idl/init.c:196-273) at the start of every
+   * stage: density floor DFLOOR (g/cm^3, velocities scaled to keep the
+   * momentum, pressure to keep the sound speed), temperature floor 500 K,
+   * pressure floor DFLOOR*500/(KELVIN mu), Uc of changed cells from
+   * PrimToCons.  0: no internal boundary. */
+  double dfloor;
 } pmx_wind_desc;
 
 /* Create a context: uploads geometry, PLM weights and the line-force

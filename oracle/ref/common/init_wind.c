@@ -31,7 +31,11 @@ static double wall_now(void)
 void Init (double *v, double x1, double x2, double x3)
 {
   int nv;
+#ifdef GAMMA_EOS
   g_gamma = g_inputParam[GAMMA_EOS];
+#else
+  g_gamma = g_inputParam[GAMMA];      /* the cv_idl parameter set */
+#endif
   for (nv = 0; nv < NVAR; nv++) v[nv] = 0.0;
   v[RHO] = 1.0;
   v[PRS] = 1.0;
@@ -112,9 +116,11 @@ void Analysis (const Data *d, Grid *grid)
   fclose(fp);
 }
 
+#ifndef PMX_CVIDL_BOUNDARY
 void UserDefBoundary (const Data *d, RBox *box, int side, Grid *grid)
 {
 }
+#endif
 
 void BodyForceVector(double *v, double *g, double x1, double x2, double x3)
 {

@@ -185,7 +185,8 @@ class _OrcWindCfg(ctypes.Structure):
                 ("x1", _dp), ("x1l", _dp), ("x1r", _dp), ("dx1", _dp),
                 ("x2", _dp), ("x2l", _dp), ("x2r", _dp), ("dx2", _dp),
                 ("nangles", ctypes.c_int),
-                ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp)]
+                ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp),
+                ("dfloor", ctypes.c_double)]
 
 
 def _wind_struct(cfg, fluxes=None, grid=None):
@@ -224,6 +225,7 @@ def _wind_struct(cfg, fluxes=None, grid=None):
     c.x2, c.x2l, c.x2r, c.dx2 = arr(gt["x"]), arr(gt["xl"]), arr(gt["xr"]), arr(gt["dx"])
     c.nangles = cfg.nangles
     c.flux_r, c.flux_t, c.flux_p = arr(fr), arr(ft), arr(fp)
+    c.dfloor = cfg.dfloor
     return c, keep
 
 
@@ -266,7 +268,7 @@ def run_wind_cool_oracle(cfg, V0: np.ndarray, dt: float, t: float = 0.0, xi=None
 
 WIND_COUNTERS = ("rkf12_ok", "ck45_calls", "ck45_sub", "ck45_reject", "bl_solve",
                  "bl_widen", "bl_second", "below_minT", "dt_cfl", "dt_cool", "dt_maxvar",
-                 "c2p_fail", "floor_T")
+                 "c2p_fail", "floor_T", "floor_cells")
 
 
 def wind_counters() -> dict:

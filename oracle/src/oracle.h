@@ -22,7 +22,9 @@ enum { ORC_PLM = 0, ORC_PPM = 1 };
 enum { ORC_LIM_DEFAULT = 0, ORC_LIM_VANLEER = 1, ORC_LIM_MC = 2,
        ORC_LIM_MINMOD = 3, ORC_LIM_FLAT = 4 };
 enum { ORC_HLLC = 0, ORC_ROE = 1 };
-enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW = 1, ORC_BC_REFLECTIVE = 2 };
+enum { ORC_BC_PERIODIC = 0, ORC_BC_OUTFLOW = 1, ORC_BC_REFLECTIVE = 2,
+       ORC_BC_WIND_INNER = 3,   /* cv_idl X1-beg userdef (cv_idl/init.c:320-330) */
+       ORC_BC_WIND_POLE = 4 };  /* cv_idl X2-beg userdef (cv_idl/init.c:346-359) */
 
 typedef struct {
   int    dims;          /* 2 or 3                                        */
@@ -75,6 +77,7 @@ typedef struct {
   const double *x2, *x2l, *x2r, *dx2;
   int    nangles;             /* 36 */
   const double *flux_r, *flux_t, *flux_p;   /* [nangles][nx2][nx1]            */
+  double dfloor;              /* > 0: cv_idl internal-boundary floors (g/cm^3) */
 } orc_wind_cfg;
 
 /* Advance V ([6][nx2][nx1], in/out) by nsteps steps of the reference's
@@ -101,7 +104,8 @@ double orc_wind_cool_time(const orc_wind_cfg *cfg, double rho, double p0, double
 /* Branch counters of the last orc_wind_run / orc_wind_cool: rkf12 accepted, ck45 calls,
  * ck45 substeps, ck45 rejections, BLONDIN solves, bracket widenings,
  * second (heat-cool root) solves, cells below the 1e4 K cut, dt limited by
- * CFL / cooling / CFL_max_var, cons->prim failures, temperature floors.
+ * CFL / cooling / CFL_max_var, cons->prim failures, temperature floors,
+ * cells changed by the internal-boundary floors.
  * Copies min(n, count) entries; returns the count. */
 int orc_wind_counters(long *out, int n);
 

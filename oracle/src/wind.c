@@ -42,7 +42,7 @@
 /* path counters, for tests that check which branches a case exercises */
 enum { CNT_RKF12_OK, CNT_CK45_CALLS, CNT_CK45_SUB, CNT_CK45_REJECT, CNT_BL_SOLVE,
        CNT_BL_WIDEN, CNT_BL_SECOND, CNT_BELOW_MINT, CNT_DT_CFL, CNT_DT_COOL,
-       CNT_DT_MAXVAR, CNT_C2P_FAIL, CNT_FLOOR_T, CNT_N };
+       CNT_DT_MAXVAR, CNT_C2P_FAIL, CNT_FLOOR_T, CNT_FLOOR_D, CNT_N };
 static long cnt[CNT_N];
 
 int orc_wind_counters(long *out, int n)
@@ -171,11 +171,68 @@ static void geometry(W *w)
 }
 
 /* ------------------------------------------------------------------ */
+/* cv_idl UserDefBoundary(side 0), Test_Problems/LineDrivenWind/cv_idl/  */
+/* init.c:196-273: density, pressure and temperature floors on every    */
+/* cell (TOT_LOOP), momentum kept by scaling the velocities, the sound   */
+/* speed kept by the new pressure; PrimToCons of the changed cells.     */
+/* The midplane assignment of the same pass (:276-311) writes the first */
+/* ghost row beyond theta_max, which the X2-end reflective fill that    */
+/* follows overwrites, so it is not restated.                           */
+/* ------------------------------------------------------------------ */
+static int floor_cell(const W *w, double *v)
+{
+  const orc_wind_cfg *c = w->c;
+  const double dfloor = c->dfloor/w->UD, tfloor = 5.e2;
+  const double pfloor = dfloor*tfloor/(w->KELVIN*c->mu);
+  int changed = 0;
+  if (v[RHO] < dfloor) {
+    double cs, dfact, temp;
+    if (v[RHO] < 0.0) v[RHO] = dfloor;
+    cs = sqrt(c->gamma*v[PRS]/v[RHO]);
+    dfact = v[RHO]/dfloor;
+    v[RHO] = dfloor;
+    v[VX1] = dfact*v[VX1];
+    v[VX2] = dfact*v[VX2];
+    v[VX3] = dfact*v[VX3];
+    v[PRS] = pow(cs, 2)*v[RHO]/c->gamma;
+    temp = v[PRS]/v[RHO]*w->KELVIN*c->mu;
+    if (temp < tfloor) {
+      temp = tfloor;
+      v[PRS] = v[RHO]*temp/(w->KELVIN*c->mu);
+    }
+    v[TRC] = 0.0;
+    changed = 1;
+  }
+  if (v[PRS] < pfloor) {
+    v[PRS] = pfloor;
+    changed = 1;
+  }
+  return changed;
+}
+
+static void internal_floors(W *w)
+{
+  int i, j, nv;
+  if (w->c->dfloor <= 0.0) return;
+  for (j = 0; j < w->t2; j++)
+    for (i = 0; i < w->t1; i++) {
+      double v[NVW];
+      for (nv = 0; nv < NVW; nv++) v[nv] = w->V[VI(w, nv, j, i)];
+      if (floor_cell(w, v)) {
+        for (nv = 0; nv < NVW; nv++) w->V[VI(w, nv, j, i)] = v[nv];
+        p2c(w->c->gamma, v, w->U + NVW*CI(w, j, i));
+        cnt[CNT_FLOOR_D]++;
+      }
+    }
+}
+
+/* ------------------------------------------------------------------ */
 /* boundaries (Src/boundary.c), sides in order x1beg, x1end, x2beg, x2end */
 /* ------------------------------------------------------------------ */
 static void boundary(W *w)
 {
   int side, nv;
+  internal_floors(w);                 /* boundary.c:126-128, INTERNAL_BOUNDARY */
   for (side = 0; side < 4; side++) {
     int d = side/2, left = (side%2 == 0), type = w->c->bc[side];
     int n = d ? w->n2 : w->n1, b = NG, e = NG + n - 1;
@@ -183,14 +240,21 @@ static void boundary(W *w)
     for (nv = 0; nv < NVW; nv++) {
       double sgn = (type == ORC_BC_REFLECTIVE && nv == VX1 + d) ? -1.0 : 1.0;
       int a0 = 0, a1 = d ? w->t1 - 1 : w->t2 - 1, a, g;
+      if (type == ORC_BC_WIND_POLE && nv == VX2) sgn = -1.0;
       for (a = a0; a <= a1; a++)
       for (g = lo; g <= hi; g++) {
         int src;
         if (type == ORC_BC_PERIODIC) src = left ? g + n : g - n;
-        else if (type == ORC_BC_OUTFLOW) src = left ? b : e;
+        else if (type == ORC_BC_OUTFLOW || type == ORC_BC_WIND_INNER) src = left ? b : e;
+        else if (type == ORC_BC_WIND_POLE && (nv == RHO || nv == PRS)) src = left ? b : e;
         else src = left ? 2*b - 1 - g : 2*e + 1 - g;
         if (d == 0) w->V[VI(w, nv, a, g)] = sgn*w->V[VI(w, nv, a, src)];
         else w->V[VI(w, nv, g, a)] = sgn*w->V[VI(w, nv, src, a)];
+        /* cv_idl X1-beg: inflow only (v_r <= 0) */
+        if (type == ORC_BC_WIND_INNER && nv == VX1) {
+          size_t q = d == 0 ? VI(w, nv, a, g) : VI(w, nv, g, a);
+          w->V[q] = MIND(w->V[q], 0.0);
+        }
       }
     }
   }

@@ -62,15 +62,44 @@ struct StageArgs {
   Scalars* s;
 };
 
-__device__ __forceinline__ void ld6(const StageArgs& a, int jj, int ii, double* v) {
+// The state of cell (jj, ii) as the reference's Boundary() leaves it at
+// the start of a stage (Src/boundary.c:56-480): ghost cells are never
+// stored; an out-of-domain index maps to the interior cell the boundary
+// rule copies from.  With the internal boundary on, that cell's floors
+// (floor_cell) apply first, as UserDefBoundary(side 0) runs before the
+// sides are filled; then the side rules in the reference's order (x1
+// sides, then x2 sides over the full x1 range, so corners see both):
+// reflective flips the normal velocity; WIND_INNER caps v_r at 0; WIND_POLE
+// reflects the velocities and takes rho and p from the first interior row.
+// Returns 1 when the floors changed an interior cell (used for the cell's
+// own Uc).
+__device__ __forceinline__ int ld6(const StageArgs& a, const double* __restrict__ V, int jj,
+                                   int ii, double* v) {
   bool fr = false, ft = false;
   int im = map_bc(ii, a.nr, a.bc[0], a.bc[1], fr);
   int jm = map_bc(jj, a.nt, a.bc[2], a.bc[3], ft);
-  const double* p = a.Vin + (long long)jm * a.nr + im;
+  const double* p = V + (long long)jm * a.nr + im;
 #pragma unroll
   for (int nv = 0; nv < NVW; nv++) v[nv] = p[nv * a.sv];
+  int changed = 0;
+  if (a.k.dfloor > 0.0) changed = floor_cell(a.k, v);
   if (fr) v[VX1] = -v[VX1];
+  if (ii < 0 && a.bc[0] == BC_WIND_INNER) v[VX1] = dmin(v[VX1], 0.0);
   if (ft) v[VX2] = -v[VX2];
+  if (jj < 0 && a.bc[2] == BC_WIND_POLE) {
+    const double* q = V + im;          // row 0 of the same column
+    double w[NVW];
+#pragma unroll
+    for (int nv = 0; nv < NVW; nv++) w[nv] = q[nv * a.sv];
+    if (a.k.dfloor > 0.0) floor_cell(a.k, w);
+    v[RHO] = w[RHO];
+    v[PRS] = w[PRS];
+  }
+  return changed;
+}
+
+__device__ __forceinline__ void ld6(const StageArgs& a, int jj, int ii, double* v) {
+  ld6(a, a.Vin, jj, ii, v);
 }
 
 __device__ __forceinline__ void ld_vel(const StageArgs& a, int jj, int ii, double& v1,
@@ -81,7 +110,18 @@ __device__ __forceinline__ void ld_vel(const StageArgs& a, int jj, int ii, doubl
   const double* p = a.Vin + (long long)jm * a.nr + im;
   v1 = p[VX1 * a.sv];
   v2 = p[VX2 * a.sv];
+  if (a.k.dfloor > 0.0) {
+    // floor_cell's velocity scaling, which needs only rho
+    double rho = p[RHO * a.sv];
+    if (rho < a.k.dfloor) {
+      if (rho < 0.0) rho = a.k.dfloor;
+      const double dfact = rho / a.k.dfloor;
+      v1 = dfact * v1;
+      v2 = dfact * v2;
+    }
+  }
   if (fr) v1 = -v1;
+  if (ii < 0 && a.bc[0] == BC_WIND_INNER) v1 = dmin(v1, 0.0);
   if (ft) v2 = -v2;
 }
 
@@ -112,7 +152,7 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
 
     // ---- x1 stencil and reconstruction -------------------------------
     double vc[NVW], w1[NVW], e1[NVW];
-    ld6(a, j, i, vc);
+    const int floored = ld6(a, a.Vin, j, i, vc);
     ld6(a, j, i - 1, w1);
     ld6(a, j, i + 1, e1);
     double c[6];
@@ -287,7 +327,9 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
 
     // ---- update, RK2 combination, cons->prim ---------------------------
     double u[NVW];
-    if (STAGE == 1) {
+    if (STAGE == 1 || floored) {
+      // stage 1: U0 = PrimToCons of the bounded state (rk_step.c:124-125);
+      // stage 2: a cell the floors changed gets PrimToCons of its new state
       p2c6(vc, u, k.gmm1);
     } else {
 #pragma unroll
@@ -301,6 +343,7 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       double vn[NVW], u0[NVW];
 #pragma unroll
       for (int nv = 0; nv < NVW; nv++) vn[nv] = a.Vn[nv * a.sv + cell];
+      if (k.dfloor > 0.0) floor_cell(k, vn);   // U0 was formed after the floors
       p2c6(vn, u0, k.gmm1);
 #pragma unroll
       for (int nv = 0; nv < NVW; nv++) u[nv] = 0.5 * u0[nv] + 0.5 * u[nv];
@@ -630,6 +673,9 @@ Const make_const(const pmx_wind_desc* d) {
   k.lx = d->l_star * d->f_x;
   k.t_x = d->t_x;
   k.E_cost = k.UL / k.UD / pow(k.UV, 3.0);
+  // cv_idl/init.c:184-188: dfloor = DFLOOR / UNIT_DENSITY, pfloor with T = 500 K
+  k.dfloor = d->dfloor / k.UD;
+  k.pfloor = k.dfloor * 5.e2 / (k.KELVIN * d->mu);
   return k;
 }
 }  // namespace wind
@@ -660,7 +706,13 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
     set_error("unknown cooling mode"); return -1;
   }
   for (int q = 0; q < 4; q++)
-    if (d->bc[q] < 0 || d->bc[q] > 2) { set_error("unknown boundary type"); return -1; }
+    if (d->bc[q] < 0 || d->bc[q] > 4) { set_error("unknown boundary type"); return -1; }
+  if (d->bc[0] == PMX_BC_WIND_POLE || d->bc[2] == PMX_BC_WIND_INNER || d->bc[1] > 2 ||
+      d->bc[3] > 2) {
+    set_error("PMX_BC_WIND_INNER is an r-beg and PMX_BC_WIND_POLE a theta-beg boundary");
+    return -1;
+  }
+  if (!(d->dfloor >= 0.0)) { set_error("dfloor must be >= 0"); return -1; }
   if (!d->x1 || !d->x1l || !d->x1r || !d->dx1 || !d->x2 || !d->x2l || !d->x2r || !d->dx2 ||
       !d->flux_r || !d->flux_t || !d->flux_p) {
     set_error("missing grid or flux arrays"); return -1;

@@ -48,7 +48,43 @@ struct Const {
   double lx;                               // L_star * f_x
   double t_x;                              // T_x
   double E_cost;                           // UL / UD / UV^3, cv_idl/radiat.c:47
+  double dfloor, pfloor;                   // internal-boundary floors (code units); 0: off
 };
+
+// ---------------------------------------------------------------------
+// The production problem's internal boundary, UserDefBoundary(side 0)
+// (Test_Problems/LineDrivenWind/cv_idl/init.c:196-273), on one cell's
+// primitive state: a density below DFLOOR is raised to it with the
+// velocities scaled by rho/DFLOOR (momentum kept) and the pressure set to
+// keep the sound speed, then held above 500 K; the tracer is cleared; the
+// pressure is held above pfloor.  Returns 1 if the cell changed (its Uc
+// is then PrimToCons of the new state).  cs^2 is formed as cs*cs (the
+// correctly rounded square) where the reference calls pow(cs, 2); glibc
+// 2.35's pow(x, 2) differs from x*x by one ulp on ~0.09% of random inputs.
+// ---------------------------------------------------------------------
+PMX_HD int floor_cell(const Const& k, double* v) {
+  const double tfloor = 5.e2;
+  int changed = 0;
+  if (v[RHO] < k.dfloor) {
+    if (v[RHO] < 0.0) v[RHO] = k.dfloor;
+    const double cs = sqrt(k.gamma * v[PRS] / v[RHO]);
+    const double dfact = v[RHO] / k.dfloor;
+    v[RHO] = k.dfloor;
+    v[VX1] = dfact * v[VX1];
+    v[VX2] = dfact * v[VX2];
+    v[VX3] = dfact * v[VX3];
+    v[PRS] = (cs * cs) * v[RHO] / k.gamma;
+    const double temp = v[PRS] / v[RHO] * k.KELVIN * k.mu;
+    if (temp < tfloor) v[PRS] = v[RHO] * tfloor / (k.KELVIN * k.mu);
+    v[TRC] = 0.0;
+    changed = 1;
+  }
+  if (v[PRS] < k.pfloor) {
+    v[PRS] = k.pfloor;
+    changed = 1;
+  }
+  return changed;
+}
 
 // ---------------------------------------------------------------------
 // prim <-> cons with the tracer (Src/HD/mappers.c:65-67, :247-249)

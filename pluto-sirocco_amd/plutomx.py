@@ -75,7 +75,8 @@ class WindDesc(ctypes.Structure):
                 ("x1", _dp), ("x1l", _dp), ("x1r", _dp), ("dx1", _dp),
                 ("x2", _dp), ("x2l", _dp), ("x2r", _dp), ("dx2", _dp),
                 ("nangles", ctypes.c_int),
-                ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp)]
+                ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp),
+                ("dfloor", ctypes.c_double)]
 
 
 _lib = None
@@ -289,6 +290,7 @@ def wind_desc_scalars(cfg: setups.WindConfig) -> WindDesc:
                  "alpharad", "cent_mass", "unit_density", "unit_length", "unit_velocity"):
         setattr(d, name, getattr(cfg, name))
     d.nangles = cfg.nangles
+    d.dfloor = cfg.dfloor
     return d
 
 
@@ -423,26 +425,8 @@ class Wind:
         self.cfg = cfg
         self._owned = True
         lib = load_library()
-        gr, gt = grid if grid is not None else setups.wind_grid(cfg)
-        fr, ft, fp = fluxes if fluxes is not None else setups.wind_fluxes(cfg)
         self._keep = []
-
-        def arr(a):
-            a = np.ascontiguousarray(a, dtype=np.float64)
-            self._keep.append(a)
-            return _dptr(a)
-
-        d = WindDesc()
-        d.nx1, d.nx2, d.cooling = cfg.nx[0], cfg.nx[1], cfg.cooling
-        for q in range(4):
-            d.bc[q] = cfg.bc[q]
-        for name in ("gamma", "cfl", "cfl_max_var", "mu", "l_star", "f_x", "t_x", "krad",
-                     "alpharad", "cent_mass", "unit_density", "unit_length", "unit_velocity"):
-            setattr(d, name, getattr(cfg, name))
-        d.x1, d.x1l, d.x1r, d.dx1 = arr(gr["x"]), arr(gr["xl"]), arr(gr["xr"]), arr(gr["dx"])
-        d.x2, d.x2l, d.x2r, d.dx2 = arr(gt["x"]), arr(gt["xl"]), arr(gt["xr"]), arr(gt["dx"])
-        d.nangles = cfg.nangles
-        d.flux_r, d.flux_t, d.flux_p = arr(fr), arr(ft), arr(fp)
+        d = wind_desc(cfg, fluxes, grid, keep=self._keep)
         h = ctypes.c_void_p()
         _check(lib.pmx_wind_create(ctypes.byref(d), ctypes.byref(h)), "pmx_wind_create")
         self._keep = []

@@ -24,9 +24,12 @@ PLM, PPM = 0, 1
 LIM_DEFAULT, LIM_VANLEER, LIM_MC, LIM_MINMOD, LIM_FLAT = range(5)
 HLLC, ROE = 0, 1
 BC_PERIODIC, BC_OUTFLOW, BC_REFLECTIVE = range(3)
+# the production wind problem's userdef sides (cv_idl/init.c): X1-beg outflow
+# with v_r <= 0, X2-beg reflective for velocities and outflow for rho and p
+BC_WIND_INNER, BC_WIND_POLE = 3, 4
 
 _BC_NAMES = {BC_PERIODIC: "periodic", BC_OUTFLOW: "outflow",
-             BC_REFLECTIVE: "reflective"}
+             BC_REFLECTIVE: "reflective", BC_WIND_INNER: "userdef", BC_WIND_POLE: "userdef"}
 _SOLVER_NAMES = {HLLC: "hllc", ROE: "roe"}
 
 
@@ -233,6 +236,9 @@ class WindConfig:
     alpharad: float = -0.6
     cent_mass: float = 0.6 * 2.e33
     nangles: int = 36
+    # > 0: the production problem's internal boundary (UserDefBoundary side 0:
+    # density / pressure / temperature floors), density floor in g/cm^3
+    dfloor: float = 0.0
     # units of the problem (definitions.h)
     unit_density: float = 1e-14
     unit_length: float = 1e9
@@ -250,6 +256,10 @@ class WindConfig:
         return self.nx[0] * self.nx[1]
 
     def reference_problem(self) -> str:
+        if self.dfloor > 0.0:
+            if self.cooling != COOL_PLAIN:
+                raise ValueError("the cv_idl boundary build uses the plain cooling")
+            return "wind_cvidl"
         return {COOL_BLONDIN: "wind_blondin", COOL_PLAIN: "wind_plain"}[self.cooling]
 
     def pluto_ini(self) -> str:
@@ -276,12 +286,24 @@ class WindConfig:
                   "vtk       -1.0  -1   single_file",
                   "tab       -1.0  -1", "ppm       -1.0  -1", "png       -1.0  -1",
                   "log        1000000", "analysis  -1.0  1",
-                  "", "[Parameters]", "",
-                  f"MU             {self.mu!r}", f"T_ISO          {self.t_iso!r}",
-                  f"L_star         {self.l_star!r}", f"f_x            {self.f_x!r}",
-                  f"f_uv           {self.f_uv!r}", f"T_x            {self.t_x!r}",
-                  f"KRAD           {self.krad!r}", f"ALPHARAD       {self.alpharad!r}",
-                  f"CENT_MASS      {self.cent_mass!r}", f"GAMMA_EOS      {self.gamma!r}", ""]
+                  "", "[Parameters]", ""]
+        if self.dfloor > 0.0:
+            # the cv_idl parameter list, in its definitions.h order; RHO_0,
+            # R_0, RHO_ALPHA and DISK_MDOT only enter the midplane ghost row
+            # that the X2-end reflective fill overwrites
+            lines += [f"MU             {self.mu!r}", "RHO_0          1e-09", "R_0            1.0",
+                      "RHO_ALPHA      2.0", f"CENT_MASS      {self.cent_mass!r}",
+                      "DISK_MDOT      6.3e17", f"T_ISO          {self.t_iso!r}",
+                      f"L_star         {self.l_star!r}", f"f_x            {self.f_x!r}",
+                      f"f_uv           {self.f_uv!r}", f"T_x            {self.t_x!r}",
+                      f"KRAD           {self.krad!r}", f"ALPHARAD       {self.alpharad!r}",
+                      f"DFLOOR         {self.dfloor!r}", f"GAMMA          {self.gamma!r}", ""]
+        else:
+            lines += [f"MU             {self.mu!r}", f"T_ISO          {self.t_iso!r}",
+                      f"L_star         {self.l_star!r}", f"f_x            {self.f_x!r}",
+                      f"f_uv           {self.f_uv!r}", f"T_x            {self.t_x!r}",
+                      f"KRAD           {self.krad!r}", f"ALPHARAD       {self.alpharad!r}",
+                      f"CENT_MASS      {self.cent_mass!r}", f"GAMMA_EOS      {self.gamma!r}", ""]
         return "\n".join(lines)
 
 
@@ -418,6 +440,15 @@ def write_flux_files(cfg: WindConfig, directory, fluxes=None) -> None:
 
 def c3_wind(nr: int = 512, nt: int = 256, cooling: int = COOL_PLAIN) -> WindConfig:
     return WindConfig(nx=(nr, nt), cooling=cooling)
+
+
+def cvidl_wind(nr: int = 512, nt: int = 256, dfloor: float = 1.e-15,
+               cooling: int = COOL_PLAIN) -> WindConfig:
+    """The wind with the production problem's boundaries
+    (Test_Problems/LineDrivenWind/cv_idl: X1-beg and X2-beg userdef,
+    X1-end outflow, X2-end reflective) and its internal-boundary floors."""
+    return WindConfig(nx=(nr, nt), cooling=cooling, dfloor=dfloor,
+                      bc=(BC_WIND_INNER, BC_OUTFLOW, BC_WIND_POLE, BC_REFLECTIVE))
 
 
 def c5_batch(n_grids: int = 64, seed: int = 2024, cooling: int = COOL_PLAIN,

@@ -2,7 +2,7 @@
 
 Run in a container that holds the reference sources (/root/reference):
 
-    python tests/golden/make_golden.py [hydro] [wind]
+    python tests/golden/make_golden.py [hydro] [wind] [cvidl]
 
 For every case below the unmodified PLUTO reference (oracle/_ref/<problem>/
 pluto, built by oracle/ref/Makefile) is run from a seeded initial condition;
@@ -74,6 +74,16 @@ def wind_cases():
     yield "wind_plain_mixed_24x12", cfg, setups.wind_ic_mixed(cfg, rho0_cgs=1.e-9), 8
 
 
+def cvidl_cases():
+    """The wind with the production problem's internal boundary (density,
+    pressure and temperature floors on every stage) and userdef sides
+    (X1-beg v_r <= 0, X2-beg hybrid pole): a mixed IC with cells below the
+    density floor and below 500 K."""
+    cfg = dataclasses.replace(setups.cvidl_wind(32, 16, dfloor=1.e-15), first_dt=1.e-3)
+    yield ("wind_cvidl_mixed_32x16", cfg,
+           setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7)), 8)
+
+
 def wind_inputs(cfg):
     """Grid and flux arrays exactly as handed to the reference."""
     gr, gt = setups.wind_grid(cfg)
@@ -83,9 +93,11 @@ def wind_inputs(cfg):
             "flux_r": fr, "flux_t": ft, "flux_p": fp}
 
 
-def main(which=("hydro", "wind")):
+def main(which=("hydro", "wind", "cvidl")):
     refrun.build(with_reference=True)
-    for name, cfg, V0, n in (wind_cases() if "wind" in which else ()):
+    wind = list(wind_cases()) if "wind" in which else []
+    wind += list(cvidl_cases()) if "cvidl" in which else []
+    for name, cfg, V0, n in wind:
         inp = wind_inputs(cfg)
         fl = (inp["flux_r"], inp["flux_t"], inp["flux_p"])
         r = refrun.run_reference(cfg, V0, n, fluxes=fl)
@@ -104,4 +116,4 @@ def main(which=("hydro", "wind")):
 
 
 if __name__ == "__main__":
-    main(tuple(sys.argv[1:]) or ("hydro", "wind"))
+    main(tuple(sys.argv[1:]) or ("hydro", "wind", "cvidl"))

@@ -51,3 +51,38 @@ def test_header_compiles_as_c():
     r = subprocess.run([cc, "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-x", "c", str(hdr)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_descriptor_layout_matches_header(tmp_path):
+    """The ctypes mirror of pmx_wind_desc has the C struct's size and the
+    offset of its last field."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "plutomx.h"\n'
+                   'int main(void){printf("%zu %zu\\n", sizeof(pmx_wind_desc),'
+                   ' offsetof(pmx_wind_desc, dfloor)); return 0;}\n')
+    exe = tmp_path / "sz"
+    subprocess.run([cc, "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    size, off = map(int, subprocess.run([str(exe)], capture_output=True, text=True,
+                                        check=True).stdout.split())
+    assert ctypes.sizeof(plutomx.WindDesc) == size
+    assert plutomx.WindDesc.dfloor.offset == off
+
+
+def test_wind_descriptor_validation_without_gpu():
+    """pmx_wind_create checks the descriptor before it touches the device,
+    and the Python wrapper passes every field (dfloor, boundary types)."""
+    import dataclasses
+    import setups
+    bad = [dataclasses.replace(setups.cvidl_wind(16, 8), dfloor=-1.0),
+           dataclasses.replace(setups.cvidl_wind(16, 8),
+                               bc=(setups.BC_WIND_POLE, 1, setups.BC_WIND_INNER, 2)),
+           dataclasses.replace(setups.cvidl_wind(16, 8), bc=(9, 1, 2, 2))]
+    msgs = ["dfloor must be >= 0", "is an r-beg", "unknown boundary type"]
+    for cfg, msg in zip(bad, msgs):
+        with pytest.raises(plutomx.PmxError, match=msg):
+            plutomx.Wind(cfg)

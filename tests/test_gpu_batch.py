@@ -97,6 +97,20 @@ def test_every_tile_shape(pmx, shape):
         assert np.array_equal(Vr, Vg)
 
 
+def test_batch_with_production_boundaries_matches_single_grids(pmx):
+    """A grid with the production problem's floors and userdef sides in a
+    batch with two plain grids: every grid as on its own, bit for bit."""
+    cv = dataclasses.replace(setups.cvidl_wind(40, 24, dfloor=1.e-15), first_dt=1.e-3)
+    cfgs = small_batch()[:2] + [cv]
+    V0s = [setups.wind_ic_mixed(c, seed=50 + g, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
+           for g, c in enumerate(cfgs)]
+    batch = run_batch(pmx, cfgs, V0s, 6, graph=True)
+    for cfg, V0, (Vb, Ub, tb) in zip(cfgs, V0s, batch):
+        Vs, Us, ts = run_single(pmx, cfg, V0, 6)
+        assert np.array_equal(Vb, Vs) and np.array_equal(Ub, Us)
+        assert tb["dt"] == ts["dt"]
+
+
 def test_batch_surrogate_matches_single_grids(pmx):
     """Stiff cells (dense gas, long dt) take the surrogate path; an
     untrained network still yields certified (or fallback) results, and the

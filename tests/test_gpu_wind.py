@@ -6,7 +6,8 @@ the device math library's pow / exp / log10 / cosh, which can differ from
 the host libm in the last bit.  They enter through the force multiplier
 M = k t^alpha of the line force and through the cooling rates.  So:
 
-* with k = 0 (no force multiplier) and no cooling the GPU step must be
+* with k = 0 (no force multiplier), no cooling and no internal-boundary
+  floors (whose pow(cs, 2) the device forms as cs*cs) the GPU step must be
   bit-identical to the oracle;
 * otherwise every variable must satisfy |gpu - ref| <= 1e-12 |ref| +
   1e-12 max|ref| (relative error, with the variable's own scale as the
@@ -93,6 +94,22 @@ def test_wind_c3_grid_matches_oracle(pmx, cooling):
     V0 = setups.wind_ic(cfg)
     o = refrun.run_wind_oracle(cfg, V0, 4)
     V, U, dts, tm = gpu_run(pmx, cfg, V0, 4)
+    assert_close(V, o["V"], "V")
+    assert_close(U, o["U"], "U")
+    np.testing.assert_allclose(dts, o["dt_seq"], rtol=RTOL, atol=0)
+
+
+def test_wind_production_boundaries_match_oracle(pmx):
+    """The production problem's internal-boundary floors and userdef sides
+    on a 256 x 128 grid whose mixed state puts ~5% of the cells below the
+    density floor and some below 500 K, 6 steps with plain cooling."""
+    import dataclasses
+    cfg = dataclasses.replace(setups.cvidl_wind(256, 128, dfloor=1.e-15), first_dt=1.e-3)
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
+    assert np.mean(V0[0] * cfg.unit_density < cfg.dfloor) > 0.02
+    o = refrun.run_wind_oracle(cfg, V0, 6)
+    assert refrun.wind_counters()["floor_cells"] > 0
+    V, U, dts, tm = gpu_run(pmx, cfg, V0, 6)
     assert_close(V, o["V"], "V")
     assert_close(U, o["U"], "U")
     np.testing.assert_allclose(dts, o["dt_seq"], rtol=RTOL, atol=0)

@@ -1,7 +1,9 @@
 """CPU tests pinning the line-driven wind restatement (oracle/src/wind.c).
 
 The golden fixtures were written by the compiled, unmodified reference
-(problems wind_blondin / wind_plain, tests/golden/make_golden.py) from the
+(problems wind_blondin / wind_plain, and wind_cvidl with the production
+problem's UserDefBoundary from Test_Problems/LineDrivenWind/cv_idl/init.c;
+tests/golden/make_golden.py) from the
 grid and flux arrays stored beside them; the oracle must reproduce V, Uc,
 the dt sequence, t and the next dt bit for bit.  Each fixture also states
 which branches of the cooling and time-step logic it reaches, so a fixture
@@ -47,6 +49,7 @@ COVERAGE = {
     "wind_blondin_mixed_24x12": ["bl_solve", "bl_widen", "below_minT", "c2p_fail"],
     "wind_blondin_mixed_cfl_24x12": ["bl_widen", "dt_cfl", "c2p_fail"],
     "wind_plain_mixed_24x12": ["rkf12_ok", "ck45_calls", "ck45_reject", "dt_cool"],
+    "wind_cvidl_mixed_32x16": ["floor_cells", "ck45_calls", "c2p_fail"],
 }
 
 
@@ -55,6 +58,28 @@ def test_wind_fixture_branch_coverage(name):
     _, _, _, cnt = run_fixture(name)
     missing = [k for k in COVERAGE[name] if cnt[k] == 0]
     assert not missing, (missing, cnt)
+
+
+def test_cvidl_fixture_pins_floors_and_sides():
+    """The internal-boundary floors and the X1-beg userdef side change the
+    result of the cv_idl fixture: dropping either one (in the oracle) breaks
+    the bitwise match, so the fixture pins both.  The X2-beg hybrid side
+    differs from a reflective one only in rho and p of the second ghost row,
+    which no result can show with PLM + van Leer: the first ghost row equals
+    the first interior row under both rules, so that row's slope difference
+    is zero and the limiter returns a zero slope either way."""
+    import dataclasses
+    cfg, g = golden_io.load("wind_cvidl_mixed_32x16")
+    grid = ({"x": g["x1"], "xl": g["x1l"], "xr": g["x1r"], "dx": g["dx1"]},
+            {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
+    fl = (g["flux_r"], g["flux_t"], g["flux_p"])
+    for variant in (dataclasses.replace(cfg, dfloor=0.0),
+                    dataclasses.replace(cfg, bc=(setups.BC_OUTFLOW,) + cfg.bc[1:])):
+        r = refrun.run_wind_oracle(variant, g["V0"], int(g["nsteps"]), fluxes=fl, grid=grid)
+        assert not np.array_equal(r["V"], g["V"])
+    pole = dataclasses.replace(cfg, bc=cfg.bc[:2] + (setups.BC_REFLECTIVE,) + cfg.bc[3:])
+    r = refrun.run_wind_oracle(pole, g["V0"], int(g["nsteps"]), fluxes=fl, grid=grid)
+    assert np.array_equal(r["V"], g["V"])
 
 
 def test_wind_fixture_inputs_match_generators():
