@@ -35,6 +35,11 @@ import setups  # noqa: E402
 
 METRIC = "Mcell-updates/s per RK step; surrogate vs plain cooling at <=1e-3 T error"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per stage-kernel launch at 256^3, from two rocprofv3 --pmc passes
+# (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a plain copy of known
+# size reads on this device), mean over the three RK3 stage launches; the
+# commands and per-launch values are in profiles/stage_kernel_sedov256.md.
+TRAFFIC_BYTES_PER_LAUNCH_256 = 5.2627e9
 
 
 def algorithmic_bytes_per_cell(rk: int) -> list:
@@ -204,6 +209,15 @@ def main() -> int:
     t1 = time.perf_counter()
     stage_ms, launches = h.stage_timing(False, reset=True)
     t, dt, inv, flags = h.get_time()
+    # the same step with the state handed over in host memory (upload V,
+    # one step, download V): what a driver pays that keeps Vc on the host
+    Vh = h.get_prim()
+    t2 = time.perf_counter()
+    for _ in range(2):
+        h.set_prim(Vh)
+        h.run(1)
+        Vh = h.get_prim()
+    host_ms = 1e3 * (time.perf_counter() - t2) / 2
     h.close()
 
     copy_ms = plutomx.copy_probe(1 << 30, reps=5)
@@ -215,8 +229,9 @@ def main() -> int:
     bytes_per_launch = cfg.ncells * sum(per_stage) / len(per_stage)
     mean_launch_ms = stage_ms / max(launches, 1)
     achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
+    traffic = TRAFFIC_BYTES_PER_LAUNCH_256 if args.n == 256 else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "stage_kernel (fused PPM+HLLC stage, all directions)",
                 "mean_launch_ms": round(mean_launch_ms, 4),
                 "algorithmic_bytes_per_launch": bytes_per_launch,
@@ -230,6 +245,10 @@ def main() -> int:
                       "cells": cfg.ncells, "sim_time_after": t, "dt_next": dt,
                       "c2p_flags": flags},
            "roofline": roofline,
+           "host_roundtrip": {"ms_per_step": round(host_ms, 3),
+                              "Mcell_updates_per_s": round(cfg.ncells / host_ms / 1e3, 2),
+                              "what": "set_prim + 1 step + get_prim of the full state from "
+                                      "pageable host memory (not the headline value)"},
            "cpu_baseline": None if args.no_cpu else cpu_baseline()}
     if not args.no_wind:
         out["wind"] = wind_legs(args.wind_steps)
