@@ -18,10 +18,7 @@ void set_error(const std::string& msg);
     }                                                                              \
   } while (0)
 
-enum { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECTIVE = 2,
-       BC_WIND_INNER = 3,     // index map of outflow (the wind stage adds v_r <= 0)
-       BC_WIND_POLE = 4 };    // index map of reflective (the wind stage takes rho, p
-                              // from the first cell)
+enum { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECTIVE = 2 };
 
 // Ghost-index mapping of the reference's Boundary() for one direction
 // (Src/boundary.c:617-763): periodic wraps, outflow copies the first/last
@@ -32,11 +29,11 @@ enum { BC_PERIODIC = 0, BC_OUTFLOW = 1, BC_REFLECTIVE = 2,
 __device__ __forceinline__ int map_bc(int g, int n, int bcl, int bcr, bool& flip) {
   if (g < 0) {
     if (bcl == BC_PERIODIC) g += n;
-    else if (bcl == BC_OUTFLOW || bcl == BC_WIND_INNER) g = 0;
+    else if (bcl == BC_OUTFLOW) g = 0;
     else { g = -1 - g; flip = !flip; }
   } else if (g >= n) {
     if (bcr == BC_PERIODIC) g -= n;
-    else if (bcr == BC_OUTFLOW || bcr == BC_WIND_INNER) g = n - 1;
+    else if (bcr == BC_OUTFLOW) g = n - 1;
     else { g = 2 * n - 1 - g; flip = !flip; }
   }
   return g < 0 ? 0 : (g >= n ? n - 1 : g);

@@ -103,10 +103,34 @@ struct pmx_hydro {
 
 namespace {
 
+// The boundary kind of a domain: one rule on every side of the active
+// directions is compiled in (BCK_OUTFLOW, BCK_PERIODIC), else BCK_ANY.
+int boundary_kind(const pmx_hydro_desc& d) {
+  const int ns = 2 * d.dims;
+  bool out = true, per = true;
+  for (int q = 0; q < ns; q++) {
+    out = out && d.bc[q] == PMX_BC_OUTFLOW;
+    per = per && d.bc[q] == PMX_BC_PERIODIC;
+  }
+  return out ? BCK_OUTFLOW : (per ? BCK_PERIODIC : BCK_ANY);
+}
+
 template <int DIMS, int REC, int SOLVER, int MODE>
 void launch_stage(pmx_hydro* h, const StageArgs& a, int nblocks) {
-  hipLaunchKernelGGL((stage_kernel<DIMS, REC, SOLVER, MODE, TILE_X, TILE_Y, NTHREADS>),
-                     dim3(nblocks), dim3(NTHREADS), 0, h->stream, a);
+  const dim3 g(nblocks), b(NTHREADS);
+  switch (boundary_kind(h->d)) {
+    case BCK_OUTFLOW:
+      hipLaunchKernelGGL((stage_kernel<DIMS, REC, SOLVER, MODE, BCK_OUTFLOW, TILE_X, TILE_Y,
+                                       NTHREADS>), g, b, 0, h->stream, a);
+      break;
+    case BCK_PERIODIC:
+      hipLaunchKernelGGL((stage_kernel<DIMS, REC, SOLVER, MODE, BCK_PERIODIC, TILE_X, TILE_Y,
+                                       NTHREADS>), g, b, 0, h->stream, a);
+      break;
+    default:
+      hipLaunchKernelGGL((stage_kernel<DIMS, REC, SOLVER, MODE, BCK_ANY, TILE_X, TILE_Y,
+                                       NTHREADS>), g, b, 0, h->stream, a);
+  }
 }
 
 template <int DIMS, int REC, int SOLVER>

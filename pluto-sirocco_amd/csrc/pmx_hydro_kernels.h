@@ -31,7 +31,8 @@ namespace pmx {
 enum { MODE_FIRST = 0, MODE_WEIGHTED = 1, MODE_THIRD = 2 };
 
 #ifndef PMX_WAVES_PER_EU
-#define PMX_WAVES_PER_EU 3   // occupancy the register allocation targets (3 measured fastest)
+#define PMX_WAVES_PER_EU 2   // occupancy the register allocation targets: 2 fits the stage
+                             // without spills (~215 VGPRs) and measured 3% faster than 3
 #endif
 #ifndef PMX_PREFETCH_U
 #define PMX_PREFETCH_U 0   // load U and V0 of the cell at the top of the plane loop
@@ -60,15 +61,32 @@ template <int REC> struct RecTraits {
   static constexpr int H = (REC == REC_PPM) ? 3 : 2;  // ghost width, Src/get_nghost.c:19-42
 };
 
+// Boundary kinds known at compile time.  The stage kernel sits at its
+// register limit, and the run-time boundary switch of map_bc costs it
+// about 40% more spilled registers; a domain with the same rule on every
+// side takes a specialised load instead.
+enum { BCK_ANY = 0, BCK_OUTFLOW = 1, BCK_PERIODIC = 2 };
+
+template <int BCK>
+__device__ __forceinline__ int map_bck(int g, int n, int bcl, int bcr, bool& flip) {
+  if (BCK == BCK_OUTFLOW) return min(max(g, 0), n - 1);
+  if (BCK == BCK_PERIODIC) {
+    g += g < 0 ? n : 0;
+    g -= g >= n ? n : 0;
+    return min(max(g, 0), n - 1);
+  }
+  return map_bc(g, n, bcl, bcr, flip);
+}
+
 // Load the 5 primitive variables of global cell (i, j, k) with boundary
 // mapping; flip the sign of the normal velocity of a reflected direction.
-template <int DIMS>
+template <int DIMS, int BCK>
 __device__ __forceinline__ void load_cell(const StageArgs& a, const double* __restrict__ V,
                                           int i, int j, int k, double* q) {
   bool fx = false, fy = false, fz = false;
-  int ii = map_bc(i, a.nx, a.bc[0], a.bc[1], fx);
-  int jj = map_bc(j, a.ny, a.bc[2], a.bc[3], fy);
-  int kk = (DIMS == 3) ? map_bc(k, a.nz, a.bc[4], a.bc[5], fz) : 0;
+  int ii = map_bck<BCK>(i, a.nx, a.bc[0], a.bc[1], fx);
+  int jj = map_bck<BCK>(j, a.ny, a.bc[2], a.bc[3], fy);
+  int kk = (DIMS == 3) ? map_bck<BCK>(k, a.nz, a.bc[4], a.bc[5], fz) : 0;
   long long o = ((long long)kk * a.ny + jj) * a.nx + ii;
 #pragma unroll
   for (int nv = 0; nv < NV; nv++) q[nv] = V[o + nv * a.sv];
@@ -111,7 +129,7 @@ __device__ __forceinline__ void halo_item(int it, int& r, int& c) {
   }
 }
 
-template <int DIMS, int REC, int SOLVER, int MODE, int TX, int TY, int NT>
+template <int DIMS, int REC, int SOLVER, int MODE, int BCK, int TX, int TY, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PER_EU, PMX_WAVES_PER_EU))) void stage_kernel(StageArgs a) {
   constexpr int H = RecTraits<REC>::H;
   constexpr int NC = TX * TY;          // cells of a tile plane
@@ -167,7 +185,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
     if (REC == REC_PPM) {
       double p[6][NV];  // v[k0-3 .. k0+2]
 #pragma unroll
-      for (int s = 0; s < 6; s++) load_cell<DIMS>(a, a.Vin, i, j, k0 - 3 + s, p[s]);
+      for (int s = 0; s < 6; s++) load_cell<DIMS, BCK>(a, a.Vin, i, j, k0 - 3 + s, p[s]);
       double vm0[NV], vpm[NV];
 #pragma unroll
       for (int nv = 0; nv < NV; nv++) {
@@ -186,7 +204,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
     } else {
       double p[4][NV];  // v[k0-2 .. k0+1]
 #pragma unroll
-      for (int s = 0; s < 4; s++) load_cell<DIMS>(a, a.Vin, i, j, k0 - 2 + s, p[s]);
+      for (int s = 0; s < 4; s++) load_cell<DIMS, BCK>(a, a.Vin, i, j, k0 - 2 + s, p[s]);
       double vpm[NV], vmm[NV], vm0[NV];
       plm_states<REC>(p[0], p[1], p[2], vpm, vmm);   // cell k0-1
       plm_states<REC>(p[1], p[2], p[3], vpz, vm0);   // cell k0
@@ -205,7 +223,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
     int it = tid + q * NT, r, c;
     if (it < NHALO) {
       halo_item<H, TX, TY>(it, r, c);
-      load_cell<DIMS>(a, a.Vin, i0 + c - H, j0 + r - H, k0, hq[q]);
+      load_cell<DIMS, BCK>(a, a.Vin, i0 + c - H, j0 + r - H, k0, hq[q]);
     }
   }
 
@@ -213,11 +231,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
     double vc[NV];  // own cell value at plane k
     if (owner) {
       if (DIMS == 3) {
-        load_cell<DIMS>(a, a.Vin, i, j, k + WZ - 1, w[WZ - 1]);
+        load_cell<DIMS, BCK>(a, a.Vin, i, j, k + WZ - 1, w[WZ - 1]);
 #pragma unroll
         for (int nv = 0; nv < NV; nv++) vc[nv] = w[0][nv];
       } else {
-        load_cell<DIMS>(a, a.Vin, i, j, 0, vc);
+        load_cell<DIMS, BCK>(a, a.Vin, i, j, 0, vc);
       }
     }
     // U of this stage and V at the start of the step, needed by the update
@@ -243,7 +261,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
         halo_item<H, TX, TY>(it, r, c);
 #pragma unroll
         for (int nv = 0; nv < NV; nv++) sPlane[(nv * LH + r) * LW + c] = hq[q][nv];
-        if (k + 1 < k1) load_cell<DIMS>(a, a.Vin, i0 + c - H, j0 + r - H, k + 1, hq[q]);
+        if (k + 1 < k1) load_cell<DIMS, BCK>(a, a.Vin, i0 + c - H, j0 + r - H, k + 1, hq[q]);
       }
     }
     __syncthreads();

@@ -62,6 +62,16 @@ struct StageArgs {
   Scalars* s;
 };
 
+// The wind's userdef sides: r-beg maps like outflow (plus v_r <= 0), theta-
+// beg like reflective (plus rho, p from the first row).  Kept out of the
+// shared map_bc: one more compare there costs the Cartesian stage kernel,
+// which sits at its register limit, twice its spills.
+enum { BC_WIND_INNER = PMX_BC_WIND_INNER, BC_WIND_POLE = PMX_BC_WIND_POLE };
+
+__device__ __forceinline__ int map_bc_wind(int g, int n, int bcl, int bcr, bool& flip) {
+  return map_bc(g, n, bcl == BC_WIND_INNER ? BC_OUTFLOW : bcl, bcr, flip);
+}
+
 // The state of cell (jj, ii) as the reference's Boundary() leaves it at
 // the start of a stage (Src/boundary.c:56-480): ghost cells are never
 // stored; an out-of-domain index maps to the interior cell the boundary
@@ -76,8 +86,8 @@ struct StageArgs {
 __device__ __forceinline__ int ld6(const StageArgs& a, const double* __restrict__ V, int jj,
                                    int ii, double* v) {
   bool fr = false, ft = false;
-  int im = map_bc(ii, a.nr, a.bc[0], a.bc[1], fr);
-  int jm = map_bc(jj, a.nt, a.bc[2], a.bc[3], ft);
+  int im = map_bc_wind(ii, a.nr, a.bc[0], a.bc[1], fr);
+  int jm = map_bc_wind(jj, a.nt, a.bc[2], a.bc[3], ft);
   const double* p = V + (long long)jm * a.nr + im;
 #pragma unroll
   for (int nv = 0; nv < NVW; nv++) v[nv] = p[nv * a.sv];
@@ -105,8 +115,8 @@ __device__ __forceinline__ void ld6(const StageArgs& a, int jj, int ii, double* 
 __device__ __forceinline__ void ld_vel(const StageArgs& a, int jj, int ii, double& v1,
                                        double& v2) {
   bool fr = false, ft = false;
-  int im = map_bc(ii, a.nr, a.bc[0], a.bc[1], fr);
-  int jm = map_bc(jj, a.nt, a.bc[2], a.bc[3], ft);
+  int im = map_bc_wind(ii, a.nr, a.bc[0], a.bc[1], fr);
+  int jm = map_bc_wind(jj, a.nt, a.bc[2], a.bc[3], ft);
   const double* p = a.Vin + (long long)jm * a.nr + im;
   v1 = p[VX1 * a.sv];
   v2 = p[VX2 * a.sv];
