@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a plain copy of known
 # size reads on this device), mean over the three RK3 stage launches; the
 # commands and per-launch values are in profiles/stage_kernel_sedov256.md.
-TRAFFIC_BYTES_PER_LAUNCH_256 = 5.2627e9
+TRAFFIC_BYTES_PER_LAUNCH_256 = 3.8061e9
 
 
 def algorithmic_bytes_per_cell(rk: int) -> list:
