@@ -149,6 +149,13 @@ typedef struct {
    * pressure floor DFLOOR*500/(KELVIN mu), Uc of changed cells from
    * PrimToCons.  0: no internal boundary. */
   double dfloor;
+  /* 1: SHOCK_FLATTENING MULTID (Src/flag_shock.c:81-253): at the first
+   * stage of each step, zones with div v < 0 and a pressure jump above 5x
+   * the local minimum take the HLL flux (Src/HD/hllc.c:93-104) and, with
+   * their neighbours, the minmod slope (Src/States/plm_states.c:177-190).
+   * The reference then runs with 3 ghost zones; the test covers the first
+   * two ghost layers. */
+  int shock_flattening;
 } pmx_wind_desc;
 
 /* Create a context: uploads geometry, PLM weights and the line-force

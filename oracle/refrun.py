@@ -186,7 +186,7 @@ class _OrcWindCfg(ctypes.Structure):
                 ("x2", _dp), ("x2l", _dp), ("x2r", _dp), ("dx2", _dp),
                 ("nangles", ctypes.c_int),
                 ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp),
-                ("dfloor", ctypes.c_double)]
+                ("dfloor", ctypes.c_double), ("shock_flattening", ctypes.c_int)]
 
 
 def _wind_struct(cfg, fluxes=None, grid=None):
@@ -226,6 +226,7 @@ def _wind_struct(cfg, fluxes=None, grid=None):
     c.nangles = cfg.nangles
     c.flux_r, c.flux_t, c.flux_p = arr(fr), arr(ft), arr(fp)
     c.dfloor = cfg.dfloor
+    c.shock_flattening = int(cfg.shock_flattening)
     return c, keep
 
 
@@ -268,7 +269,7 @@ def run_wind_cool_oracle(cfg, V0: np.ndarray, dt: float, t: float = 0.0, xi=None
 
 WIND_COUNTERS = ("rkf12_ok", "ck45_calls", "ck45_sub", "ck45_reject", "bl_solve",
                  "bl_widen", "bl_second", "below_minT", "dt_cfl", "dt_cool", "dt_maxvar",
-                 "c2p_fail", "floor_T", "floor_cells")
+                 "c2p_fail", "floor_T", "floor_cells", "shock_zones")
 
 
 def wind_counters() -> dict:

@@ -78,6 +78,7 @@ typedef struct {
   int    nangles;             /* 36 */
   const double *flux_r, *flux_t, *flux_p;   /* [nangles][nx2][nx1]            */
   double dfloor;              /* > 0: cv_idl internal-boundary floors (g/cm^3) */
+  int    shock_flattening;    /* 1: SHOCK_FLATTENING MULTID (flag_shock.c)     */
 } orc_wind_cfg;
 
 /* Advance V ([6][nx2][nx1], in/out) by nsteps steps of the reference's
@@ -105,7 +106,7 @@ double orc_wind_cool_time(const orc_wind_cfg *cfg, double rho, double p0, double
  * ck45 substeps, ck45 rejections, BLONDIN solves, bracket widenings,
  * second (heat-cool root) solves, cells below the 1e4 K cut, dt limited by
  * CFL / cooling / CFL_max_var, cons->prim failures, temperature floors,
- * cells changed by the internal-boundary floors.
+ * cells changed by the internal-boundary floors, zones flagged as shocked.
  * Copies min(n, count) entries; returns the count. */
 int orc_wind_counters(long *out, int n);
 

@@ -42,7 +42,7 @@
 /* path counters, for tests that check which branches a case exercises */
 enum { CNT_RKF12_OK, CNT_CK45_CALLS, CNT_CK45_SUB, CNT_CK45_REJECT, CNT_BL_SOLVE,
        CNT_BL_WIDEN, CNT_BL_SECOND, CNT_BELOW_MINT, CNT_DT_CFL, CNT_DT_COOL,
-       CNT_DT_MAXVAR, CNT_C2P_FAIL, CNT_FLOOR_T, CNT_FLOOR_D, CNT_N };
+       CNT_DT_MAXVAR, CNT_C2P_FAIL, CNT_FLOOR_T, CNT_FLOOR_D, CNT_SHOCK, CNT_N };
 static long cnt[CNT_N];
 
 int orc_wind_counters(long *out, int n)
@@ -87,7 +87,10 @@ typedef struct {
   double UD, UL, UV, UT, UP, UA, KELVIN;
   double invDt_hyp, dt_cool, t;
   int stage;
+  unsigned char *flag;                /* zone flags [j][i] (FLAG_MINMOD, FLAG_HLL) */
 } W;
+
+enum { FLAG_MINMOD = 1, FLAG_HLL = 4 };   /* Src/pluto.h:212-214 */
 
 #define VI(w, nv, j, i) (((size_t)(nv)*(w)->t2 + (j))*(w)->t1 + (i))
 #define CI(w, j, i) ((size_t)(j)*(w)->t1 + (i))
@@ -224,6 +227,125 @@ static void internal_floors(W *w)
         cnt[CNT_FLOOR_D]++;
       }
     }
+}
+
+/* ------------------------------------------------------------------ */
+/* FlagShock with SHOCK_FLATTENING MULTID (Src/flag_shock.c:81-253),    */
+/* 2-D spherical: a zone with div v < 0 and a total pressure jump       */
+/* |dp_1| + |dp_2| > 5 min(p of the zone and its 4 neighbours) gets     */
+/* FLAG_HLL and FLAG_MINMOD, its 4 neighbours FLAG_MINMOD.  MULTID      */
+/* raises nghost to 3 (Src/get_nghost.c:48-57) and the test runs from   */
+/* 1 to NX_TOT - 2, i.e. over both of the ghost layers this grid        */
+/* stores; their outer neighbours (the third ghost layer) come from the */
+/* same boundary rules, through ghost3().                               */
+/* ------------------------------------------------------------------ */
+
+/* Variable nv of zone (j, i) for j in [-1, t2], i in [-1, t1] (at most one
+ * index outside the stored range): the boundary rule of that side. */
+static double ghost3(const W *w, int nv, int j, int i)
+{
+  int d = (i < 0 || i >= w->t1) ? 0 : ((j < 0 || j >= w->t2) ? 1 : -1);
+  if (d < 0) return w->V[VI(w, nv, j, i)];
+  {
+    int g = d ? j : i, n = d ? w->n2 : w->n1, b = NG, e = NG + n - 1, left = g < 0;
+    int type = w->c->bc[2*d + (left ? 0 : 1)], src;
+    double sgn = 1.0, val;
+    if (type == ORC_BC_PERIODIC) src = left ? g + n : g - n;
+    else if (type == ORC_BC_OUTFLOW || type == ORC_BC_WIND_INNER) src = left ? b : e;
+    else if (type == ORC_BC_WIND_POLE && (nv == RHO || nv == PRS)) src = left ? b : e;
+    else {
+      src = left ? 2*b - 1 - g : 2*e + 1 - g;
+      if (nv == VX1 + d) sgn = -1.0;
+    }
+    val = sgn*(d ? w->V[VI(w, nv, src, i)] : w->V[VI(w, nv, j, src)]);
+    if (type == ORC_BC_WIND_INNER && nv == VX1) val = MIND(val, 0.0);
+    return val;
+  }
+}
+
+static void flag_shock(W *w)
+{
+  int i, j;
+  const double eps = 5.0;                  /* EPS_PSHOCK_FLATTEN */
+  for (j = 0; j < w->t2; j++)
+    for (i = 0; i < w->t1; i++) {
+      size_t c = CI(w, j, i);
+      /* face areas left / below the zone: set_geometry.c:141-186 (i = -1,
+       * j = -1 use the left edge of the first stored zone) */
+      double dmu = fabs(cos(w->x2l[j]) - cos(w->x2r[j]));
+      double A1m = i > 0 ? w->A1[c - 1] : w->x1l[0]*w->x1l[0]*dmu;
+      double A2m = j > 0 ? w->A2[c - w->t1]
+                         : fabs(w->x1[i])*w->dx1[i]*fabs(sin(w->x2l[0]));
+      double dvx1, dvx2, divv;
+      dvx1 = w->A1[c]*(ghost3(w, VX1, j, i + 1) + ghost3(w, VX1, j, i))
+           - A1m*(ghost3(w, VX1, j, i - 1) + ghost3(w, VX1, j, i));
+      dvx2 = w->A2[c]*(ghost3(w, VX2, j + 1, i) + ghost3(w, VX2, j, i))
+           - A2m*(ghost3(w, VX2, j - 1, i) + ghost3(w, VX2, j, i));
+      divv = (dvx1 + dvx2)/w->dV[c];
+      if (divv < 0.0) {
+        double pc = ghost3(w, PRS, j, i);
+        double pe = ghost3(w, PRS, j, i + 1), pw = ghost3(w, PRS, j, i - 1);
+        double pn = ghost3(w, PRS, j + 1, i), ps = ghost3(w, PRS, j - 1, i);
+        double pt_min = pc, pt_min1, pt_min2, dpx1, dpx2, gradp;
+        pt_min1 = MIND(pe, pw);
+        pt_min2 = MIND(pn, ps);
+        pt_min = MIND(pt_min, pt_min1);
+        pt_min = MIND(pt_min, pt_min2);
+        dpx1 = fabs(pe - pw);
+        dpx2 = fabs(pn - ps);
+        gradp = dpx1 + dpx2;
+        if (gradp > eps*pt_min) {
+          w->flag[c] |= FLAG_HLL | FLAG_MINMOD;
+          if (i + 1 < w->t1) w->flag[c + 1] |= FLAG_MINMOD;
+          if (i > 0) w->flag[c - 1] |= FLAG_MINMOD;
+          if (j > 0) w->flag[c - w->t1] |= FLAG_MINMOD;
+          if (j + 1 < w->t2) w->flag[c + w->t1] |= FLAG_MINMOD;
+          cnt[CNT_SHOCK]++;
+        }
+      }
+    }
+}
+
+/* HLLC with the FLAG_HLL switch of SHOCK_FLATTENING MULTID
+ * (Src/HD/hllc.c:57-104): the same speeds and supersonic cases, then the
+ * HLL flux for a subsonic interface. */
+static void riemann_face(double gamma, int n, int hll, const double *vL, const double *uL,
+                         const double *vR, const double *uR, double *flux, double *press,
+                         double *cmax)
+{
+  double mach = 0.0;
+  if (!hll) {
+    orc_hllc(gamma, n, vL, uL, vR, uR, flux, press, cmax, &mach);
+    return;
+  }
+  {
+    double a2L = gamma*vL[PRS]/vL[RHO], a2R = gamma*vR[PRS]/vR[RHO];
+    double fL[NVW], fR[NVW], aL, aR, SL, SR, scrh;   /* the NFLX = 5 HD fluxes */
+    int nv;
+    orc_phys_flux(n, vL, uL, fL);
+    orc_phys_flux(n, vR, uR, fR);
+    aL = sqrt(a2L);
+    aR = sqrt(a2R);
+    SL = MIND(vL[VX1 + n] - aL, vR[VX1 + n] - aR);        /* hll_speed.c:76-91 */
+    SR = MAXD(vL[VX1 + n] + aL, vR[VX1 + n] + aR);
+    *cmax = MAXD(fabs(SL), fabs(SR));
+    if (SL > 0.0) {
+      for (nv = 0; nv < TRC; nv++) flux[nv] = fL[nv];
+      *press = vL[PRS];
+      return;
+    }
+    if (SR < 0.0) {
+      for (nv = 0; nv < TRC; nv++) flux[nv] = fR[nv];
+      *press = vR[PRS];
+      return;
+    }
+    scrh = 1.0/(SR - SL);
+    for (nv = 0; nv < TRC; nv++) {
+      flux[nv] = SL*SR*(uR[nv] - uL[nv]) + SR*fL[nv] - SL*fR[nv];
+      flux[nv] *= scrh;
+    }
+    *press = (SR*vL[PRS] - SL*vR[PRS])*scrh;
+  }
 }
 
 /* ------------------------------------------------------------------ */
@@ -418,20 +540,27 @@ static void update_stage(W *w, double dt)
       for (ip = nbeg - 2; ip <= nend + 1; ip++)
         for (nv = 0; nv < NVW; nv++) dv[ip*NVW + nv] = v[ip + 1][nv] - v[ip][nv];
       for (ip = nbeg - 1; ip <= nend + 1; ip++) {
+        /* plm_states.c:177-190: a FLAG_MINMOD zone takes the minmod slope */
+        int mm = (d ? w->flag[CI(w, ip, i)] : w->flag[CI(w, j, ip)]) & FLAG_MINMOD;
         for (nv = 0; nv < NVW; nv++) {
           double dvp = dv[ip*NVW + nv]*wp[ip], dvm = dv[(ip - 1)*NVW + nv]*wm[ip];
-          double dl = (dvp*dvm > 0.0 ? (dvp)*(dvm)*(cp[ip]*(dvm) + cm[ip]*(dvp))
-                       /((dvp)*(dvp) + (dvm)*(dvm) + (cp[ip] + cm[ip] - 2.0)*(dvp)*(dvm)) : 0.0);
+          double dl;
+          if (mm) dl = (dvp*dvm > 0.0 ? (fabs(dvp) < fabs(dvm) ? dvp : dvm) : 0.0);
+          else dl = (dvp*dvm > 0.0 ? (dvp)*(dvm)*(cp[ip]*(dvm) + cm[ip]*(dvp))
+                     /((dvp)*(dvp) + (dvm)*(dvm) + (cp[ip] + cm[ip] - 2.0)*(dvp)*(dvm)) : 0.0);
           vp[ip][nv] = v[ip][nv] + dl*dp[ip];
           vm[ip][nv] = v[ip][nv] - dl*dm[ip];
         }
       }
       /* Riemann + tracer flux (adv_flux.c) */
       for (ip = nbeg - 1; ip <= nend; ip++) {
-        double uL[NVW], uR[NVW], mach = 0.0;
+        double uL[NVW], uR[NVW];
+        int f0 = d ? w->flag[CI(w, ip, i)] : w->flag[CI(w, j, ip)];
+        int f1 = d ? w->flag[CI(w, ip + 1, i)] : w->flag[CI(w, j, ip + 1)];
         p2c(gamma, vp[ip], uL);
         p2c(gamma, vm[ip + 1], uR);
-        orc_hllc(gamma, d, vp[ip], uL, vm[ip + 1], uR, flux[ip], &press[ip], &cmax[ip], &mach);
+        riemann_face(gamma, d, (f0 | f1) & FLAG_HLL, vp[ip], uL, vm[ip + 1], uR, flux[ip],
+                     &press[ip], &cmax[ip]);
         flux[ip][TRC] = flux[ip][RHO]*(flux[ip][RHO] > 0.0 ? vp[ip][TRC] : vm[ip + 1][TRC]);
       }
       /* TotalFlux (rhs.c:466-560) */
@@ -510,8 +639,10 @@ static void advance_step(W *w, double dt)
 {
   int i, j, nv;
   size_t nc = (size_t)w->t1*w->t2;
+  memset(w->flag, 0, nc);             /* main.c:258-261, once per step */
   w->stage = 1;
   boundary(w);
+  if (w->c->shock_flattening) flag_shock(w);   /* rk_step.c:123-125 */
   for (j = NG; j < NG + w->n2; j++)
     for (i = NG; i < NG + w->n1; i++) {
       double v[NVW];
@@ -900,6 +1031,7 @@ static void w_init(W *pw, const orc_wind_cfg *c, const double *Vin)
   w.x11_0 = calloc(nc, sizeof(double)); w.x11_1 = calloc(nc, sizeof(double));
   w.x22_0 = calloc(nc, sizeof(double)); w.x22_1 = calloc(nc, sizeof(double));
   w.xi_s = calloc(nc, sizeof(double)); w.tr_s = calloc(nc, sizeof(double));
+  w.flag = calloc(nc, 1);
 
   geometry(&w);
   for (a = 0; a < w.na; a++)
@@ -935,6 +1067,7 @@ static void w_free(W *pw)
   for (d = 0; d < 2; d++) for (a = 0; a < 6; a++) free(w.pc[d][a]);
   free(w.fr); free(w.ft); free(w.fp); free(w.roff); free(w.toff); free(w.moff); free(w.dvds);
   free(w.x11_0); free(w.x11_1); free(w.x22_0); free(w.x22_1); free(w.xi_s); free(w.tr_s);
+  free(w.flag);
 }
 
 /* Copy the interior of the state back to [nv][nx2][nx1] arrays. */

@@ -47,6 +47,7 @@ struct Tables {
   const double *lf;            // [NLF][na][nt][nr]
   const double *f1c, *f2c;     // [nr], [nt]: centre bilinear weights
   const double *sta, *cta;     // [na]: sin / cos of the bin angle
+  double x1l0, sin_x2l0;       // x1 and |sin x2| of the lower edge of the first ghost zone
 };
 
 struct StageArgs {
@@ -60,6 +61,9 @@ struct StageArgs {
   Const k;
   Tables T;
   Scalars* s;
+  // SHOCK_FLATTENING MULTID: 1 per shocked zone over the interior and two
+  // ghost layers, [nt+4][nr+4]; null without flattening
+  const unsigned char* shock;
 };
 
 // The wind's userdef sides: r-beg maps like outflow (plus v_r <= 0), theta-
@@ -140,12 +144,76 @@ __device__ __forceinline__ void ldc(const double* tab, int n, int idx, double* c
   for (int q = 0; q < 6; q++) c[q] = tab[q * (n + 2 * NG) + idx];
 }
 
-// HLLC at one interface plus the tracer's upwind flux (Src/adv_flux.c:42-52)
+// HLL at an interface next to a shocked zone: HLLC's speeds and supersonic
+// cases, then the HLL flux (Src/HD/hllc.c:57-104 with FLAG_HLL).
+template <int DIR>
+__device__ __forceinline__ void hll_face(const double* vL, const double* vR, double gamma,
+                                         double gmm1, double* flux, double& press,
+                                         double& cmax) {
+  const int n = VX1 + DIR;
+  double uL[NV], uR[NV], fL[NV], fR[NV];
+  prim_to_cons(vL, uL, gmm1);
+  prim_to_cons(vR, uR, gmm1);
+  double a2L = gamma * vL[PRS] / vL[RHO];
+  double a2R = gamma * vR[PRS] / vR[RHO];
+  double aL = sqrt(a2L), aR = sqrt(a2R);
+  double SL = dmin(vL[n] - aL, vR[n] - aR);
+  double SR = dmax(vL[n] + aL, vR[n] + aR);
+  cmax = dmax(fabs(SL), fabs(SR));
+  phys_flux<DIR>(vL, uL, fL);
+  phys_flux<DIR>(vR, uR, fR);
+  if (SL > 0.0) {
+#pragma unroll
+    for (int q = 0; q < NV; q++) flux[q] = fL[q];
+    press = vL[PRS];
+  } else if (SR < 0.0) {
+#pragma unroll
+    for (int q = 0; q < NV; q++) flux[q] = fR[q];
+    press = vR[PRS];
+  } else {
+    const double scrh = 1.0 / (SR - SL);
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+      flux[q] = SL * SR * (uR[q] - uL[q]) + SR * fL[q] - SL * fR[q];
+      flux[q] *= scrh;
+    }
+    press = (SR * vL[PRS] - SL * vR[PRS]) * scrh;
+  }
+}
+
+// HLLC (or HLL next to a shocked zone) at one interface plus the tracer's
+// upwind flux (Src/adv_flux.c:42-52)
 template <int DIR>
 __device__ __forceinline__ void face(const double* vL, const double* vR, const Const& k,
-                                     double* F, double& p, double& cmax) {
-  hllc<DIR>(vL, vR, k.gamma, k.gmm1, F, p, cmax);
+                                     double* F, double& p, double& cmax, bool hll = false) {
+  if (hll) hll_face<DIR>(vL, vR, k.gamma, k.gmm1, F, p, cmax);
+  else hllc<DIR>(vL, vR, k.gamma, k.gmm1, F, p, cmax);
   F[TRC] = F[RHO] * (F[RHO] > 0.0 ? vL[TRC] : vR[TRC]);
+}
+
+// Shock flags of zone (j, i), j in [-2, nt + 1], i in [-2, nr + 1].
+__device__ __forceinline__ bool shocked(const StageArgs& a, int j, int i) {
+  return a.shock[(long long)(j + 2) * (a.nr + 4) + (i + 2)] != 0;
+}
+// FLAG_MINMOD of zone (j, i), j in [-1, nt], i in [-1, nr]: the zone or a
+// face neighbour is shocked (flag_shock.c:214-225).
+__device__ __forceinline__ bool minmod_zone(const StageArgs& a, int j, int i) {
+  if (!a.shock) return false;
+  return shocked(a, j, i) || shocked(a, j, i - 1) || shocked(a, j, i + 1) ||
+         shocked(a, j - 1, i) || shocked(a, j + 1, i);
+}
+// FLAG_HLL at the interface between two face-neighbour zones.
+__device__ __forceinline__ bool hll_interface(const StageArgs& a, int j0, int i0, int j1,
+                                              int i1) {
+  if (!a.shock) return false;
+  return shocked(a, j0, i0) || shocked(a, j1, i1);
+}
+
+__device__ __forceinline__ void plm_zone(const StageArgs& a, int j, int i, const double* vm1,
+                                         const double* v0, const double* vp1, const double* c,
+                                         double* vp, double* vm) {
+  if (minmod_zone(a, j, i)) plm_mm_general(vm1, v0, vp1, c, vp, vm);
+  else plm_vl_general(vm1, v0, vp1, c, vp, vm);
 }
 
 // One RK2 stage (Src/Time_Stepping/rk_step.c:117-330 with UpdateStage,
@@ -168,7 +236,7 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
     double c[6];
     double vpi[NVW], vmi[NVW];
     ldc(T.pr, a.nr, I, c);
-    plm_vl_general(w1, vc, e1, c, vpi, vmi);
+    plm_zone(a, j, i, w1, vc, e1, c, vpi, vmi);
 
     // ---- VGradCalc + LineForce for both sweeps (line_connect.c:628-738,
     //      :815-903): x1 uses vc = (vp + vm)/2, x2 the cell centre ------
@@ -248,11 +316,11 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       ld6(a, j, i - 2, w2);
       ld6(a, j, i + 2, e2);
       ldc(T.pr, a.nr, I - 1, c);
-      plm_vl_general(w2, w1, vc, c, vpl, vml);
+      plm_zone(a, j, i - 1, w2, w1, vc, c, vpl, vml);
       ldc(T.pr, a.nr, I + 1, c);
-      plm_vl_general(vc, e1, e2, c, vpr, vmr);
-      face<0>(vpl, vmi, k, Fm, pm, cm1);
-      face<0>(vpi, vmr, k, Fp, pp, cp1);
+      plm_zone(a, j, i + 1, vc, e1, e2, c, vpr, vmr);
+      face<0>(vpl, vmi, k, Fm, pm, cm1, hll_interface(a, j, i - 1, j, i));
+      face<0>(vpi, vmr, k, Fp, pp, cp1, hll_interface(a, j, i, j, i + 1));
     }
     {
       const double dmu = T.dmu[J];
@@ -295,13 +363,13 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       ld6(a, j + 2, i, n2);
       double vpc[NVW], vmc[NVW], vpl[NVW], vml[NVW], vpr[NVW], vmr[NVW];
       ldc(T.pt, a.nt, J, c);
-      plm_vl_general(s1, vc, n1, c, vpc, vmc);
+      plm_zone(a, j, i, s1, vc, n1, c, vpc, vmc);
       ldc(T.pt, a.nt, J - 1, c);
-      plm_vl_general(s2, s1, vc, c, vpl, vml);
+      plm_zone(a, j - 1, i, s2, s1, vc, c, vpl, vml);
       ldc(T.pt, a.nt, J + 1, c);
-      plm_vl_general(vc, n1, n2, c, vpr, vmr);
-      face<1>(vpl, vmc, k, Gm, qm, cm2);
-      face<1>(vpc, vmr, k, Gp, qp, cp2);
+      plm_zone(a, j + 1, i, vc, n1, n2, c, vpr, vmr);
+      face<1>(vpl, vmc, k, Gm, qm, cm2, hll_interface(a, j - 1, i, j, i));
+      face<1>(vpc, vmr, k, Gp, qp, cp2, hll_interface(a, j, i, j + 1, i));
     }
     {
       double base = fabs(T.x1[I]) * T.dx1[I];
@@ -396,6 +464,44 @@ __device__ __forceinline__ bool block_reduce_stage(Scalars* s, double cdt, int f
   }
   __syncthreads();   // the shared slots are reused by the next call
   return tid == 0;
+}
+
+// FlagShock with SHOCK_FLATTENING MULTID (Src/flag_shock.c:81-253) for the
+// interior and two ghost layers (the reference runs MULTID with 3 ghost
+// zones and tests all but the outermost), from the stage-1 input as the
+// boundary leaves it (floors and side rules through ld6).  Spherical 2-D:
+// div v from the face areas r^2 dmu and r dr |sin theta| and the volume
+// dVr dmu (Src/set_geometry.c:118-186); shocked if div v < 0 and
+// |dp_1| + |dp_2| > 5 min(p of the zone and its 4 neighbours).
+__global__ __launch_bounds__(BX* BY) void wind_flag_kernel(StageArgs a, unsigned char* shock) {
+  const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
+  if (I >= a.nr + 4 || J >= a.nt + 4) return;
+  const int i = I - NG, j = J - NG;
+  const Tables& T = a.T;
+  double c[NVW], e[NVW], w[NVW], n[NVW], s[NVW];
+  ld6(a, a.Vin, j, i, c);
+  ld6(a, a.Vin, j, i + 1, e);
+  ld6(a, a.Vin, j, i - 1, w);
+  ld6(a, a.Vin, j + 1, i, n);
+  ld6(a, a.Vin, j - 1, i, s);
+  const double dmu = T.dmu[J];
+  const double x1p = T.x1r[I], x1m = I > 0 ? T.x1r[I - 1] : T.x1l0;
+  const double A1p = x1p * x1p * dmu, A1m = x1m * x1m * dmu;
+  const double base = fabs(T.x1[I]) * T.dx1[I];
+  const double A2p = base * T.sp[J], A2m = base * (J > 0 ? T.sp[J - 1] : T.sin_x2l0);
+  const double dvx1 = A1p * (e[VX1] + c[VX1]) - A1m * (w[VX1] + c[VX1]);
+  const double dvx2 = A2p * (n[VX2] + c[VX2]) - A2m * (s[VX2] + c[VX2]);
+  const double divv = (dvx1 + dvx2) / (T.dVr[I] * dmu);
+  unsigned char f = 0;
+  if (divv < 0.0) {
+    double pt_min = c[PRS];
+    const double pt_min1 = dmin(e[PRS], w[PRS]), pt_min2 = dmin(n[PRS], s[PRS]);
+    pt_min = dmin(pt_min, pt_min1);
+    pt_min = dmin(pt_min, pt_min2);
+    const double gradp = fabs(e[PRS] - w[PRS]) + fabs(n[PRS] - s[PRS]);
+    f = gradp > 5.0 * pt_min;
+  }
+  shock[(long long)J * (a.nr + 4) + I] = f;
 }
 
 template <int STAGE>
@@ -516,6 +622,7 @@ struct pmx_wind {
   double* tr_s;
   double* tab;        // one allocation for all geometry tables
   double* lf;
+  unsigned char* shock;  // SHOCK_FLATTENING MULTID zone flags, or null
   Tables T;
   Scalars* s;
   long step;
@@ -586,12 +693,24 @@ StageArgs stage_args(pmx_wind* w, const double* Vin, double* Vout) {
   a.k = w->k;
   a.T = w->T;
   a.s = w->s;
+  a.shock = w->shock;
   return a;
+}
+
+// FlagShock of the step (SHOCK_FLATTENING MULTID) from V^n, before stage 1.
+int flag_shock(pmx_wind* w, hipStream_t st) {
+  if (!w->shock) return 0;
+  dim3 grid((w->nr + 4 + BX - 1) / BX, (w->nt + 4 + BY - 1) / BY), block(BX, BY);
+  hipLaunchKernelGGL(wind_flag_kernel, grid, block, 0, st, stage_args(w, w->V[0], w->V[1]),
+                     w->shock);
+  PMX_HIP_CHECK(hipGetLastError());
+  return 0;
 }
 
 int advance(pmx_wind* w) {
   dim3 grid((w->nr + BX - 1) / BX, (w->nt + BY - 1) / BY), block(BX, BY);
   hipEvent_t e[2];
+  if (flag_shock(w, w->stream)) return -1;
   if (timed_begin(w, e)) return -1;
   hipLaunchKernelGGL(wind_stage_kernel<1>, grid, block, 0, w->stream,
                      stage_args(w, w->V[0], w->V[1]));
@@ -700,7 +819,7 @@ int pmx_wind_destroy(pmx_wind* w) {
   for (auto& p : w->pend_stage) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
   for (auto& p : w->pend_cool) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
   hipFree(w->V[0]); hipFree(w->V[1]); hipFree(w->U); hipFree(w->xi_s); hipFree(w->tr_s);
-  hipFree(w->tab); hipFree(w->lf); hipFree(w->s);
+  hipFree(w->tab); hipFree(w->lf); hipFree(w->s); hipFree(w->shock);
   hipFree(w->sw.list); hipFree(w->sw.fb); hipFree(w->sw.feat); hipFree(w->sw.yhat);
   hipFree(w->sw.f0); hipFree(w->sw.cnt); hipFree(w->cargs); hipFree(w->coff);
   if (w->stream) hipStreamDestroy(w->stream);
@@ -850,6 +969,9 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
   CK(hipMalloc(&w->tab, sizeof(double) * packed.size()));
   CK(hipMalloc(&w->lf, sizeof(double) * lf.size()));
   CK(hipMalloc(&w->s, sizeof(Scalars)));
+  if (d->shock_flattening) CK(hipMalloc(&w->shock, (size_t)(w->nr + 4) * (w->nt + 4)));
+  w->T.x1l0 = x1l[0];
+  w->T.sin_x2l0 = fabs(sin(x2l[0]));
   CK(hipMalloc(&w->sw.cnt, sizeof(unsigned long long) * 8));
   CK(hipMemset(w->sw.cnt, 0, sizeof(unsigned long long) * 8));
   CK(hipMalloc(&w->cargs, sizeof(CoolArgs)));
@@ -1215,6 +1337,8 @@ int batch_timed(pmx_wind_batch* b, hipEvent_t* e, bool begin, bool cool) {
 
 int batch_advance(pmx_wind_batch* b) {
   hipEvent_t e[2];
+  for (pmx_wind* w : b->grids)
+    if (flag_shock(w, b->stream)) return -1;
   if (batch_timed(b, e, true, false)) return -1;
   launch_batch_stage<1>(b);
   PMX_HIP_CHECK(hipGetLastError());

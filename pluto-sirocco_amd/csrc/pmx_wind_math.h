@@ -118,6 +118,21 @@ PMX_HD void plm_vl_general(const double* vm1, const double* v0,
   }
 }
 
+// The same reconstruction with the minmod slope that SHOCK_FLATTENING
+// MULTID imposes on FLAG_MINMOD zones (plm_states.c:181-186, SET_MM_LIMITER
+// and ABS_MIN of plm_coeffs.h:76-77 and macros.h:153).
+PMX_HD void plm_mm_general(const double* vm1, const double* v0, const double* vp1,
+                           const double* c, double* vp, double* vm) {
+#pragma unroll
+  for (int nv = 0; nv < NVW; nv++) {
+    double dvp = (vp1[nv] - v0[nv]) * c[2];
+    double dvm = (v0[nv] - vm1[nv]) * c[3];
+    double dl = (dvp * dvm > 0.0 ? (fabs(dvp) < fabs(dvm) ? dvp : dvm) : 0.0);
+    vp[nv] = v0[nv] + dl * c[4];
+    vm[nv] = v0[nv] - dl * c[5];
+  }
+}
+
 // ---------------------------------------------------------------------
 // Line force: the force multiplier of one angle bin of LineForce
 // (line_connect.c:873-902), 1 + M with M = krad t^alpha capped at 4400,

@@ -239,6 +239,8 @@ class WindConfig:
     # > 0: the production problem's internal boundary (UserDefBoundary side 0:
     # density / pressure / temperature floors), density floor in g/cm^3
     dfloor: float = 0.0
+    # SHOCK_FLATTENING MULTID: shocked zones take minmod slopes and HLL fluxes
+    shock_flattening: bool = False
     # units of the problem (definitions.h)
     unit_density: float = 1e-14
     unit_length: float = 1e9
@@ -259,7 +261,9 @@ class WindConfig:
         if self.dfloor > 0.0:
             if self.cooling != COOL_PLAIN:
                 raise ValueError("the cv_idl boundary build uses the plain cooling")
-            return "wind_cvidl"
+            return "wind_cvidl_flat" if self.shock_flattening else "wind_cvidl"
+        if self.shock_flattening:
+            raise ValueError("shock flattening is built with the cv_idl boundaries only")
         return {COOL_BLONDIN: "wind_blondin", COOL_PLAIN: "wind_plain"}[self.cooling]
 
     def pluto_ini(self) -> str:

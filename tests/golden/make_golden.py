@@ -82,6 +82,10 @@ def cvidl_cases():
     cfg = dataclasses.replace(setups.cvidl_wind(32, 16, dfloor=1.e-15), first_dt=1.e-3)
     yield ("wind_cvidl_mixed_32x16", cfg,
            setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7)), 8)
+    # and with the production problem's SHOCK_FLATTENING MULTID
+    cfg = dataclasses.replace(cfg, shock_flattening=True)
+    yield ("wind_cvidl_flat_mixed_32x16", cfg,
+           setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7)), 8)
 
 
 def wind_inputs(cfg):

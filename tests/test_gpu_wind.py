@@ -115,6 +115,21 @@ def test_wind_production_boundaries_match_oracle(pmx):
     np.testing.assert_allclose(dts, o["dt_seq"], rtol=RTOL, atol=0)
 
 
+def test_wind_shock_flattening_matches_oracle(pmx):
+    """SHOCK_FLATTENING MULTID with the production boundaries on a 256 x 128
+    grid whose mixed state has many shocked zones, 6 steps."""
+    import dataclasses
+    cfg = dataclasses.replace(setups.cvidl_wind(256, 128, dfloor=1.e-15), first_dt=1.e-3,
+                              shock_flattening=True)
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
+    o = refrun.run_wind_oracle(cfg, V0, 6)
+    assert refrun.wind_counters()["shock_zones"] > 1000
+    V, U, dts, tm = gpu_run(pmx, cfg, V0, 6)
+    assert_close(V, o["V"], "V")
+    assert_close(U, o["U"], "U")
+    np.testing.assert_allclose(dts, o["dt_seq"], rtol=RTOL, atol=0)
+
+
 @pytest.mark.parametrize("cooling", [setups.COOL_BLONDIN, setups.COOL_PLAIN])
 @pytest.mark.parametrize("t0", [0.0, 5.0])
 def test_wind_single_cooling_call(pmx, cooling, t0):

@@ -50,6 +50,7 @@ COVERAGE = {
     "wind_blondin_mixed_cfl_24x12": ["bl_widen", "dt_cfl", "c2p_fail"],
     "wind_plain_mixed_24x12": ["rkf12_ok", "ck45_calls", "ck45_reject", "dt_cool"],
     "wind_cvidl_mixed_32x16": ["floor_cells", "ck45_calls", "c2p_fail"],
+    "wind_cvidl_flat_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
 }
 
 
@@ -80,6 +81,19 @@ def test_cvidl_fixture_pins_floors_and_sides():
     pole = dataclasses.replace(cfg, bc=cfg.bc[:2] + (setups.BC_REFLECTIVE,) + cfg.bc[3:])
     r = refrun.run_wind_oracle(pole, g["V0"], int(g["nsteps"]), fluxes=fl, grid=grid)
     assert np.array_equal(r["V"], g["V"])
+
+
+def test_shock_flattening_fixture_pins_the_switches():
+    """MULTID shock flattening changes the fixture's result, so the flags,
+    the minmod slopes and the HLL fluxes are pinned by it."""
+    import dataclasses
+    cfg, g = golden_io.load("wind_cvidl_flat_mixed_32x16")
+    grid = ({"x": g["x1"], "xl": g["x1l"], "xr": g["x1r"], "dx": g["dx1"]},
+            {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
+    fl = (g["flux_r"], g["flux_t"], g["flux_p"])
+    r = refrun.run_wind_oracle(dataclasses.replace(cfg, shock_flattening=False), g["V0"],
+                               int(g["nsteps"]), fluxes=fl, grid=grid)
+    assert not np.array_equal(r["V"], g["V"])
 
 
 def test_wind_fixture_inputs_match_generators():
