@@ -156,6 +156,11 @@ typedef struct {
    * The reference then runs with 3 ghost zones; the test covers the first
    * two ghost layers. */
   int shock_flattening;
+  /* 1: CHAR_LIMITING YES: PLM limits the differences of the characteristic
+   * variables of the primitive HD system (Src/States/plm_states.c:479-676,
+   * Src/HD/eigenv.c:92-189, :575-605), then bounds the primitive slopes by
+   * the primitive differences; the tracer is limited in primitive form. */
+  int char_limiting;
 } pmx_wind_desc;
 
 /* Create a context: uploads geometry, PLM weights and the line-force

@@ -186,7 +186,8 @@ class _OrcWindCfg(ctypes.Structure):
                 ("x2", _dp), ("x2l", _dp), ("x2r", _dp), ("dx2", _dp),
                 ("nangles", ctypes.c_int),
                 ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp),
-                ("dfloor", ctypes.c_double), ("shock_flattening", ctypes.c_int)]
+                ("dfloor", ctypes.c_double), ("shock_flattening", ctypes.c_int),
+                ("char_limiting", ctypes.c_int)]
 
 
 def _wind_struct(cfg, fluxes=None, grid=None):
@@ -227,6 +228,7 @@ def _wind_struct(cfg, fluxes=None, grid=None):
     c.flux_r, c.flux_t, c.flux_p = arr(fr), arr(ft), arr(fp)
     c.dfloor = cfg.dfloor
     c.shock_flattening = int(cfg.shock_flattening)
+    c.char_limiting = int(cfg.char_limiting)
     return c, keep
 
 

@@ -306,6 +306,65 @@ static void flag_shock(W *w)
     }
 }
 
+/* PLM on characteristic variables, CHAR_LIMITING YES (Src/States/
+ * plm_states.c:479-676) with the HD primitive eigenvectors (Src/HD/
+ * eigenv.c:92-189, the adiabatic case) and PrimToChar (eigenv.c:575-605),
+ * for one zone of a pencil in direction d: v the zone, dvp / dvm the
+ * weighted forward / backward differences, flag its zone flags.  The
+ * eigenvector arrays are cleared per sweep (ResetState), so only the
+ * entries below are non-zero.  The tracer is limited in primitive form. */
+static void plm_char(double gamma, int d, const double *v, const double *dvp,
+                     const double *dvm, double cp, double cm, double dp, double dm, int mm,
+                     double *vp, double *vm)
+{
+  const int VXn = VX1 + d, VXt = d ? VX3 : VX2, VXb = d ? VX1 : VX3;   /* set_indexes.c */
+  double R[5][5] = {{0}}, L[5][5] = {{0}};
+  double dwp[5], dwm[5], dw_lim[5], dv_lim[NVW];
+  double cs2 = gamma*v[PRS]/v[RHO];                      /* SoundSpeed2, eos.c:33 */
+  double cs = sqrt(cs2), rhocs = v[RHO]*cs, rho_cs = v[RHO]/cs;
+  int k, nv;
+  R[RHO][0] = 0.5*rho_cs; R[VXn][0] = -0.5; R[PRS][0] = 0.5*rhocs;
+  R[RHO][1] = 0.5*rho_cs; R[VXn][1] = 0.5;  R[PRS][1] = 0.5*rhocs;
+  R[RHO][2] = 1.0; R[VXt][3] = 1.0; R[VXb][4] = 1.0;
+  L[0][VXn] = -1.0; L[0][PRS] = 1.0/rhocs;
+  L[1][VXn] = 1.0;  L[1][PRS] = 1.0/rhocs;
+  L[2][RHO] = 1.0; L[3][VXt] = 1.0; L[4][VXb] = 1.0;
+  L[2][PRS] = -1.0/cs2;
+#define PRIM_TO_CHAR(vv, w) do { \
+    w[0] = L[0][VXn]*vv[VXn] + L[0][PRS]*vv[PRS]; \
+    w[1] = L[1][VXn]*vv[VXn] + L[1][PRS]*vv[PRS]; \
+    w[2] = vv[RHO] + L[2][PRS]*vv[PRS]; \
+    w[3] = vv[VXt]; \
+    w[4] = vv[VXb]; } while (0)
+  PRIM_TO_CHAR(dvm, dwm);
+  PRIM_TO_CHAR(dvp, dwp);
+#undef PRIM_TO_CHAR
+  for (k = 5; k--; ) {
+    if (mm) dw_lim[k] = (dwp[k]*dwm[k] > 0.0 ? (fabs(dwp[k]) < fabs(dwm[k]) ? dwp[k] : dwm[k])
+                                              : 0.0);
+    else dw_lim[k] = (dwp[k]*dwm[k] > 0.0 ? (dwp[k])*(dwm[k])*(cp*(dwm[k]) + cm*(dwp[k]))
+                      /((dwp[k])*(dwp[k]) + (dwm[k])*(dwm[k]) + (cp + cm - 2.0)*(dwp[k])*(dwm[k]))
+                      : 0.0);
+  }
+  for (nv = 5; nv--; ) {
+    double dc = 0.0, d2v;
+    for (k = 0; k < 5; k++) dc += dw_lim[k]*R[nv][k];
+    if (dvp[nv]*dvm[nv] > 0.0) {
+      d2v = (fabs(cp*dvp[nv]) < fabs(cm*dvm[nv]) ? cp*dvp[nv] : cm*dvm[nv]);  /* ABS_MIN */
+      dv_lim[nv] = (d2v*dc > 0.0 ? (fabs(d2v) < fabs(dc) ? d2v : dc) : 0.0); /* MINMOD_LIMITER */
+    } else dv_lim[nv] = 0.0;
+  }
+  /* the tracer: SET_LIMITER (van Leer, general grid) */
+  nv = TRC;
+  dv_lim[nv] = (dvp[nv]*dvm[nv] > 0.0 ? (dvp[nv])*(dvm[nv])*(cp*(dvm[nv]) + cm*(dvp[nv]))
+                /((dvp[nv])*(dvp[nv]) + (dvm[nv])*(dvm[nv]) + (cp + cm - 2.0)*(dvp[nv])*(dvm[nv]))
+                : 0.0);
+  for (nv = NVW; nv--; ) {
+    vp[nv] = v[nv] + dv_lim[nv]*dp;
+    vm[nv] = v[nv] - dv_lim[nv]*dm;
+  }
+}
+
 /* HLLC with the FLAG_HLL switch of SHOCK_FLATTENING MULTID
  * (Src/HD/hllc.c:57-104): the same speeds and supersonic cases, then the
  * HLL flux for a subsonic interface. */
@@ -542,6 +601,15 @@ static void update_stage(W *w, double dt)
       for (ip = nbeg - 1; ip <= nend + 1; ip++) {
         /* plm_states.c:177-190: a FLAG_MINMOD zone takes the minmod slope */
         int mm = (d ? w->flag[CI(w, ip, i)] : w->flag[CI(w, j, ip)]) & FLAG_MINMOD;
+        if (w->c->char_limiting) {
+          double dvp[NVW], dvm[NVW];
+          for (nv = 0; nv < NVW; nv++) {
+            dvp[nv] = dv[ip*NVW + nv]*wp[ip];
+            dvm[nv] = dv[(ip - 1)*NVW + nv]*wm[ip];
+          }
+          plm_char(gamma, d, v[ip], dvp, dvm, cp[ip], cm[ip], dp[ip], dm[ip], mm, vp[ip], vm[ip]);
+          continue;
+        }
         for (nv = 0; nv < NVW; nv++) {
           double dvp = dv[ip*NVW + nv]*wp[ip], dvm = dv[(ip - 1)*NVW + nv]*wm[ip];
           double dl;

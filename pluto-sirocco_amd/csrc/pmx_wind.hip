@@ -209,16 +209,21 @@ __device__ __forceinline__ bool hll_interface(const StageArgs& a, int j0, int i0
   return shocked(a, j0, i0) || shocked(a, j1, i1);
 }
 
+// CHAR: the kernel is built with the characteristic-limiting path (kernels
+// for grids without it leave it out: it costs registers).
+template <int DIR, int CHAR>
 __device__ __forceinline__ void plm_zone(const StageArgs& a, int j, int i, const double* vm1,
                                          const double* v0, const double* vp1, const double* c,
                                          double* vp, double* vm) {
-  if (minmod_zone(a, j, i)) plm_mm_general(vm1, v0, vp1, c, vp, vm);
+  const bool mm = minmod_zone(a, j, i);
+  if (CHAR && a.k.char_lim) plm_char_general<DIR>(vm1, v0, vp1, c, a.k.gamma, mm, vp, vm);
+  else if (mm) plm_mm_general(vm1, v0, vp1, c, vp, vm);
   else plm_vl_general(vm1, v0, vp1, c, vp, vm);
 }
 
 // One RK2 stage (Src/Time_Stepping/rk_step.c:117-330 with UpdateStage,
 // Src/Time_Stepping/update_stage.c:35-410, for GEOMETRY SPHERICAL in 2-D).
-template <int STAGE>
+template <int STAGE, int CHAR>
 __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, double& cdt,
                                            int& flags) {
   {
@@ -236,7 +241,7 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
     double c[6];
     double vpi[NVW], vmi[NVW];
     ldc(T.pr, a.nr, I, c);
-    plm_zone(a, j, i, w1, vc, e1, c, vpi, vmi);
+    plm_zone<0, CHAR>(a, j, i, w1, vc, e1, c, vpi, vmi);
 
     // ---- VGradCalc + LineForce for both sweeps (line_connect.c:628-738,
     //      :815-903): x1 uses vc = (vp + vm)/2, x2 the cell centre ------
@@ -316,9 +321,9 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       ld6(a, j, i - 2, w2);
       ld6(a, j, i + 2, e2);
       ldc(T.pr, a.nr, I - 1, c);
-      plm_zone(a, j, i - 1, w2, w1, vc, c, vpl, vml);
+      plm_zone<0, CHAR>(a, j, i - 1, w2, w1, vc, c, vpl, vml);
       ldc(T.pr, a.nr, I + 1, c);
-      plm_zone(a, j, i + 1, vc, e1, e2, c, vpr, vmr);
+      plm_zone<0, CHAR>(a, j, i + 1, vc, e1, e2, c, vpr, vmr);
       face<0>(vpl, vmi, k, Fm, pm, cm1, hll_interface(a, j, i - 1, j, i));
       face<0>(vpi, vmr, k, Fp, pp, cp1, hll_interface(a, j, i, j, i + 1));
     }
@@ -363,11 +368,11 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       ld6(a, j + 2, i, n2);
       double vpc[NVW], vmc[NVW], vpl[NVW], vml[NVW], vpr[NVW], vmr[NVW];
       ldc(T.pt, a.nt, J, c);
-      plm_zone(a, j, i, s1, vc, n1, c, vpc, vmc);
+      plm_zone<1, CHAR>(a, j, i, s1, vc, n1, c, vpc, vmc);
       ldc(T.pt, a.nt, J - 1, c);
-      plm_zone(a, j - 1, i, s2, s1, vc, c, vpl, vml);
+      plm_zone<1, CHAR>(a, j - 1, i, s2, s1, vc, c, vpl, vml);
       ldc(T.pt, a.nt, J + 1, c);
-      plm_zone(a, j + 1, i, vc, n1, n2, c, vpr, vmr);
+      plm_zone<1, CHAR>(a, j + 1, i, vc, n1, n2, c, vpr, vmr);
       face<1>(vpl, vmc, k, Gm, qm, cm2, hll_interface(a, j - 1, i, j, i));
       face<1>(vpc, vmr, k, Gp, qp, cp2, hll_interface(a, j, i, j + 1, i));
     }
@@ -504,13 +509,13 @@ __global__ __launch_bounds__(BX* BY) void wind_flag_kernel(StageArgs a, unsigned
   shock[(long long)J * (a.nr + 4) + I] = f;
 }
 
-template <int STAGE>
+template <int STAGE, int CHAR>
 __global__ __launch_bounds__(BX* BY) void wind_stage_kernel(StageArgs a) {
   const int i = blockIdx.x * BX + threadIdx.x;
   const int j = blockIdx.y * BY + threadIdx.y;
   double cdt = 0.0;
   int flags = 0;
-  if (i < a.nr && j < a.nt) cell_stage<STAGE>(a, i, j, cdt, flags);
+  if (i < a.nr && j < a.nt) cell_stage<STAGE, CHAR>(a, i, j, cdt, flags);
   if (block_reduce_stage<STAGE, BX * BY>(a.s, cdt, flags)) {
     if (STAGE == 2 && blockIdx.x == 0 && blockIdx.y == 0) {
       // update_stage.c:389-394: invDt = MAX(invDt, C_dt) over the domain,
@@ -712,13 +717,21 @@ int advance(pmx_wind* w) {
   hipEvent_t e[2];
   if (flag_shock(w, w->stream)) return -1;
   if (timed_begin(w, e)) return -1;
-  hipLaunchKernelGGL(wind_stage_kernel<1>, grid, block, 0, w->stream,
-                     stage_args(w, w->V[0], w->V[1]));
+  if (w->k.char_lim)
+    hipLaunchKernelGGL((wind_stage_kernel<1, 1>), grid, block, 0, w->stream,
+                       stage_args(w, w->V[0], w->V[1]));
+  else
+    hipLaunchKernelGGL((wind_stage_kernel<1, 0>), grid, block, 0, w->stream,
+                       stage_args(w, w->V[0], w->V[1]));
   PMX_HIP_CHECK(hipGetLastError());
   if (timed_end(w, e, false)) return -1;
   if (timed_begin(w, e)) return -1;
-  hipLaunchKernelGGL(wind_stage_kernel<2>, grid, block, 0, w->stream,
-                     stage_args(w, w->V[1], w->V[0]));
+  if (w->k.char_lim)
+    hipLaunchKernelGGL((wind_stage_kernel<2, 1>), grid, block, 0, w->stream,
+                       stage_args(w, w->V[1], w->V[0]));
+  else
+    hipLaunchKernelGGL((wind_stage_kernel<2, 0>), grid, block, 0, w->stream,
+                       stage_args(w, w->V[1], w->V[0]));
   PMX_HIP_CHECK(hipGetLastError());
   return timed_end(w, e, false);
 }
@@ -805,6 +818,7 @@ Const make_const(const pmx_wind_desc* d) {
   // cv_idl/init.c:184-188: dfloor = DFLOOR / UNIT_DENSITY, pfloor with T = 500 K
   k.dfloor = d->dfloor / k.UD;
   k.pfloor = k.dfloor * 5.e2 / (k.KELVIN * d->mu);
+  k.char_lim = d->char_limiting != 0;
   return k;
 }
 }  // namespace wind
@@ -1192,7 +1206,7 @@ __device__ __forceinline__ int find_item(const int* off, int G, int q) {
   return lo;
 }
 
-template <int STAGE, int TBX, int TBY>
+template <int STAGE, int CHAR, int TBX, int TBY>
 __global__ __launch_bounds__(TBX* TBY) __attribute__((amdgpu_waves_per_eu(PMX_WIND_WAVES, PMX_WIND_WAVES)))
 void wind_batch_stage_kernel(const StageArgs* args, const int* tile_off, int G, int total) {
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
@@ -1204,7 +1218,7 @@ void wind_batch_stage_kernel(const StageArgs* args, const int* tile_off, int G, 
     const int i = tx * TBX + threadIdx.x, j = ty * TBY + threadIdx.y;
     double cdt = 0.0;
     int flags = 0;
-    if (i < a.nr && j < a.nt) cell_stage<STAGE>(a, i, j, cdt, flags);
+    if (i < a.nr && j < a.nt) cell_stage<STAGE, CHAR>(a, i, j, cdt, flags);
     block_reduce_stage<STAGE, TBX * TBY>(a.s, cdt, flags);
   }
 }
@@ -1254,6 +1268,7 @@ struct pmx_wind_batch {
   long long* cell_off;    // device, G + 1 (multiples of 128)
   long long total_cells;  // padded
   long long cells;        // interior cells of all grids
+  int char_any;           // some grid limits characteristic variables
   int shape;              // index into the tile-shape table
   int blocks;             // persistent workgroups per stage launch
   hipStream_t stream;
@@ -1282,8 +1297,12 @@ void launch_batch_stage(pmx_wind_batch* b) {
   const dim3 grid(b->blocks);
 #define PMX_SHAPE(q, X, Y)                                                                   \
   case q:                                                                                    \
-    hipLaunchKernelGGL((wind_batch_stage_kernel<STAGE, X, Y>), grid, dim3(X, Y), 0, b->stream, \
-                       args, b->tile_off, b->G, b->total_tiles);                             \
+    if (b->char_any)                                                                         \
+      hipLaunchKernelGGL((wind_batch_stage_kernel<STAGE, 1, X, Y>), grid, dim3(X, Y), 0,     \
+                         b->stream, args, b->tile_off, b->G, b->total_tiles);                \
+    else                                                                                     \
+      hipLaunchKernelGGL((wind_batch_stage_kernel<STAGE, 0, X, Y>), grid, dim3(X, Y), 0,     \
+                         b->stream, args, b->tile_off, b->G, b->total_tiles);                \
     break;
   switch (b->shape) {
     PMX_SHAPE(0, 64, 2)
@@ -1431,6 +1450,7 @@ int pmx_wind_batch_create(const pmx_wind_desc* descs, int ngrids, pmx_wind_batch
     cmv[g] = w->cfl_max_var;
     coff[g + 1] = coff[g] + (w->sv + 127) / 128 * 128;
     b->cells += w->sv;
+    b->char_any |= w->k.char_lim;
   }
   b->total_cells = coff[G];
   CK(hipMalloc(&b->args1, sizeof(StageArgs) * G));

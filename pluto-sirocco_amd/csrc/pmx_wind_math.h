@@ -49,6 +49,7 @@ struct Const {
   double t_x;                              // T_x
   double E_cost;                           // UL / UD / UV^3, cv_idl/radiat.c:47
   double dfloor, pfloor;                   // internal-boundary floors (code units); 0: off
+  int char_lim;                            // CHAR_LIMITING YES
 };
 
 // ---------------------------------------------------------------------
@@ -130,6 +131,81 @@ PMX_HD void plm_mm_general(const double* vm1, const double* v0, const double* vp
     double dl = (dvp * dvm > 0.0 ? (fabs(dvp) < fabs(dvm) ? dvp : dvm) : 0.0);
     vp[nv] = v0[nv] + dl * c[4];
     vm[nv] = v0[nv] - dl * c[5];
+  }
+}
+
+// PLM on characteristic variables (CHAR_LIMITING YES, plm_states.c:479-676)
+// for one zone of a DIR-sweep, with the adiabatic HD primitive eigenvectors
+// (eigenv.c:92-189) and PrimToChar (eigenv.c:575-605).  The reference
+// clears the eigenvector arrays before each sweep (ResetState), so the
+// projection R.dw runs over a matrix whose other entries are zero; it is
+// written here as that full 5 x 5 product so that it also propagates
+// non-finite differences as the reference does.  mm: FLAG_MINMOD zone.
+template <int DIR>
+PMX_HD void plm_char_general(const double* vm1, const double* v0, const double* vp1,
+                             const double* c, double gamma, bool mm, double* vp, double* vm) {
+  constexpr int VXn = VX1 + DIR, VXt = DIR ? VX3 : VX2, VXb = DIR ? VX1 : VX3;
+  const double cp = c[0], cm = c[1];
+  double dvp[NVW], dvm[NVW];
+#pragma unroll
+  for (int nv = 0; nv < NVW; nv++) {
+    dvp[nv] = (vp1[nv] - v0[nv]) * c[2];
+    dvm[nv] = (v0[nv] - vm1[nv]) * c[3];
+  }
+  const double cs2 = gamma * v0[PRS] / v0[RHO];            // SoundSpeed2, eos.c:33
+  const double cs = sqrt(cs2), rhocs = v0[RHO] * cs, rho_cs = v0[RHO] / cs;
+  double R[5][5];
+#pragma unroll
+  for (int a = 0; a < 5; a++)
+#pragma unroll
+    for (int b = 0; b < 5; b++) R[a][b] = 0.0;
+  R[RHO][0] = 0.5 * rho_cs; R[VXn][0] = -0.5; R[PRS][0] = 0.5 * rhocs;
+  R[RHO][1] = 0.5 * rho_cs; R[VXn][1] = 0.5;  R[PRS][1] = 0.5 * rhocs;
+  R[RHO][2] = 1.0; R[VXt][3] = 1.0; R[VXb][4] = 1.0;
+  const double L0p = 1.0 / rhocs, L2p = -1.0 / cs2;
+  double dwp[5], dwm[5], dw[5];
+  dwm[0] = -1.0 * dvm[VXn] + L0p * dvm[PRS];
+  dwm[1] = 1.0 * dvm[VXn] + L0p * dvm[PRS];
+  dwm[2] = dvm[RHO] + L2p * dvm[PRS];
+  dwm[3] = dvm[VXt];
+  dwm[4] = dvm[VXb];
+  dwp[0] = -1.0 * dvp[VXn] + L0p * dvp[PRS];
+  dwp[1] = 1.0 * dvp[VXn] + L0p * dvp[PRS];
+  dwp[2] = dvp[RHO] + L2p * dvp[PRS];
+  dwp[3] = dvp[VXt];
+  dwp[4] = dvp[VXb];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const double p = dwp[k], m = dwm[k];
+    if (mm) dw[k] = (p * m > 0.0 ? (fabs(p) < fabs(m) ? p : m) : 0.0);
+    else dw[k] = (p * m > 0.0 ? (p) * (m) * (cp * (m) + cm * (p)) /
+                                    ((p) * (p) + (m) * (m) + (cp + cm - 2.0) * (p) * (m))
+                              : 0.0);
+  }
+  double dl[NVW];
+#pragma unroll
+  for (int nv = 0; nv < 5; nv++) {
+    double dc = 0.0;
+#pragma unroll
+    for (int k = 0; k < 5; k++) dc += dw[k] * R[nv][k];
+    if (dvp[nv] * dvm[nv] > 0.0) {
+      const double a = cp * dvp[nv], b = cm * dvm[nv];
+      const double d2v = fabs(a) < fabs(b) ? a : b;                        // ABS_MIN
+      dl[nv] = (d2v * dc > 0.0 ? (fabs(d2v) < fabs(dc) ? d2v : dc) : 0.0);  // MINMOD_LIMITER
+    } else {
+      dl[nv] = 0.0;
+    }
+  }
+  {
+    const double p = dvp[TRC], m = dvm[TRC];
+    dl[TRC] = (p * m > 0.0 ? (p) * (m) * (cp * (m) + cm * (p)) /
+                                 ((p) * (p) + (m) * (m) + (cp + cm - 2.0) * (p) * (m))
+                           : 0.0);
+  }
+#pragma unroll
+  for (int nv = 0; nv < NVW; nv++) {
+    vp[nv] = v0[nv] + dl[nv] * c[4];
+    vm[nv] = v0[nv] - dl[nv] * c[5];
   }
 }
 

@@ -76,7 +76,8 @@ class WindDesc(ctypes.Structure):
                 ("x2", _dp), ("x2l", _dp), ("x2r", _dp), ("dx2", _dp),
                 ("nangles", ctypes.c_int),
                 ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp),
-                ("dfloor", ctypes.c_double), ("shock_flattening", ctypes.c_int)]
+                ("dfloor", ctypes.c_double), ("shock_flattening", ctypes.c_int),
+                ("char_limiting", ctypes.c_int)]
 
 
 _lib = None
@@ -292,6 +293,7 @@ def wind_desc_scalars(cfg: setups.WindConfig) -> WindDesc:
     d.nangles = cfg.nangles
     d.dfloor = cfg.dfloor
     d.shock_flattening = int(cfg.shock_flattening)
+    d.char_limiting = int(cfg.char_limiting)
     return d
 
 

@@ -241,6 +241,8 @@ class WindConfig:
     dfloor: float = 0.0
     # SHOCK_FLATTENING MULTID: shocked zones take minmod slopes and HLL fluxes
     shock_flattening: bool = False
+    # CHAR_LIMITING YES: PLM limits the characteristic differences
+    char_limiting: bool = False
     # units of the problem (definitions.h)
     unit_density: float = 1e-14
     unit_length: float = 1e9
@@ -261,9 +263,14 @@ class WindConfig:
         if self.dfloor > 0.0:
             if self.cooling != COOL_PLAIN:
                 raise ValueError("the cv_idl boundary build uses the plain cooling")
+            if self.char_limiting:
+                if not self.shock_flattening:
+                    raise ValueError("characteristic limiting is built with shock flattening")
+                return "wind_cvidl_char"
             return "wind_cvidl_flat" if self.shock_flattening else "wind_cvidl"
-        if self.shock_flattening:
-            raise ValueError("shock flattening is built with the cv_idl boundaries only")
+        if self.shock_flattening or self.char_limiting:
+            raise ValueError("shock flattening / characteristic limiting are built with the "
+                             "cv_idl boundaries only")
         return {COOL_BLONDIN: "wind_blondin", COOL_PLAIN: "wind_plain"}[self.cooling]
 
     def pluto_ini(self) -> str:

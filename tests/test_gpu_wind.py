@@ -130,6 +130,21 @@ def test_wind_shock_flattening_matches_oracle(pmx):
     np.testing.assert_allclose(dts, o["dt_seq"], rtol=RTOL, atol=0)
 
 
+def test_wind_char_limiting_matches_oracle(pmx):
+    """The production problem's reconstruction (CHAR_LIMITING YES with
+    SHOCK_FLATTENING MULTID) and boundaries on a 256 x 128 mixed state,
+    6 steps."""
+    import dataclasses
+    cfg = dataclasses.replace(setups.cvidl_wind(256, 128, dfloor=1.e-15), first_dt=1.e-3,
+                              shock_flattening=True, char_limiting=True)
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
+    o = refrun.run_wind_oracle(cfg, V0, 6)
+    V, U, dts, tm = gpu_run(pmx, cfg, V0, 6)
+    assert_close(V, o["V"], "V")
+    assert_close(U, o["U"], "U")
+    np.testing.assert_allclose(dts, o["dt_seq"], rtol=RTOL, atol=0)
+
+
 @pytest.mark.parametrize("cooling", [setups.COOL_BLONDIN, setups.COOL_PLAIN])
 @pytest.mark.parametrize("t0", [0.0, 5.0])
 def test_wind_single_cooling_call(pmx, cooling, t0):

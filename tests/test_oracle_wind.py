@@ -51,6 +51,7 @@ COVERAGE = {
     "wind_plain_mixed_24x12": ["rkf12_ok", "ck45_calls", "ck45_reject", "dt_cool"],
     "wind_cvidl_mixed_32x16": ["floor_cells", "ck45_calls", "c2p_fail"],
     "wind_cvidl_flat_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
+    "wind_cvidl_char_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
 }
 
 
@@ -92,6 +93,20 @@ def test_shock_flattening_fixture_pins_the_switches():
             {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
     fl = (g["flux_r"], g["flux_t"], g["flux_p"])
     r = refrun.run_wind_oracle(dataclasses.replace(cfg, shock_flattening=False), g["V0"],
+                               int(g["nsteps"]), fluxes=fl, grid=grid)
+    assert not np.array_equal(r["V"], g["V"])
+
+
+def test_char_limiting_fixture_pins_the_projection():
+    """Characteristic limiting changes the fixture's result, so the
+    eigenvector projection and the primitive re-limiting are pinned by
+    the bitwise match with the reference build."""
+    import dataclasses
+    cfg, g = golden_io.load("wind_cvidl_char_mixed_32x16")
+    grid = ({"x": g["x1"], "xl": g["x1l"], "xr": g["x1r"], "dx": g["dx1"]},
+            {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
+    fl = (g["flux_r"], g["flux_t"], g["flux_p"])
+    r = refrun.run_wind_oracle(dataclasses.replace(cfg, char_limiting=False), g["V0"],
                                int(g["nsteps"]), fluxes=fl, grid=grid)
     assert not np.array_equal(r["V"], g["V"])
 
