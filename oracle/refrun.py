@@ -53,8 +53,15 @@ def read_dump(path: pathlib.Path, shape) -> dict:
     arr = np.frombuffer(raw, dtype="<f8", count=2 * n, offset=48)
     V = arr[:n].reshape(nvar, nx3, nx2, nx1).copy()
     U = arr[n:].reshape(nvar, nx3, nx2, nx1).copy()
-    assert V.shape == tuple(shape)
-    return {"V": V, "U": U, "step": step, "t": t, "dt": dt, "wall": wall}
+    out = {"step": step, "t": t, "dt": dt, "wall": wall}
+    # variables beyond the configuration's (the entropy of ENTROPY_SWITCH
+    # builds) are returned apart
+    nv = shape[0]
+    out["V"], out["U"] = V[:nv].copy(), U[:nv].copy()
+    if nvar > nv:
+        out["V_extra"], out["U_extra"] = V[nv:].copy(), U[nv:].copy()
+    assert out["V"].shape == tuple(shape)
+    return out
 
 
 def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
@@ -72,7 +79,11 @@ def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
     with tempfile.TemporaryDirectory(prefix="pmxref_") as tmp:
         tmp = pathlib.Path(tmp)
         (tmp / "pluto.ini").write_text(cfg.pluto_ini())
-        np.ascontiguousarray(V0, dtype="<f8").tofile(tmp / "ic.bin")
+        ic = np.ascontiguousarray(V0, dtype="<f8")
+        if getattr(cfg, "entropy_switch", False):
+            # the build carries Vc[ENTR] too; Boundary() sets it from rho, p
+            ic = np.concatenate([ic, np.zeros_like(ic[:1])])
+        ic.tofile(tmp / "ic.bin")
         if hasattr(cfg, "nangles"):  # line-driven wind: SIROCCO flux files
             import setups
             setups.write_flux_files(cfg, tmp, fluxes)
@@ -187,7 +198,7 @@ class _OrcWindCfg(ctypes.Structure):
                 ("nangles", ctypes.c_int),
                 ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp),
                 ("dfloor", ctypes.c_double), ("shock_flattening", ctypes.c_int),
-                ("char_limiting", ctypes.c_int)]
+                ("char_limiting", ctypes.c_int), ("entropy_switch", ctypes.c_int)]
 
 
 def _wind_struct(cfg, fluxes=None, grid=None):
@@ -229,6 +240,7 @@ def _wind_struct(cfg, fluxes=None, grid=None):
     c.dfloor = cfg.dfloor
     c.shock_flattening = int(cfg.shock_flattening)
     c.char_limiting = int(cfg.char_limiting)
+    c.entropy_switch = int(cfg.entropy_switch)
     return c, keep
 
 

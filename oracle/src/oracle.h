@@ -80,6 +80,8 @@ typedef struct {
   double dfloor;              /* > 0: cv_idl internal-boundary floors (g/cm^3) */
   int    shock_flattening;    /* 1: SHOCK_FLATTENING MULTID (flag_shock.c)     */
   int    char_limiting;       /* 1: CHAR_LIMITING YES (plm_states.c:479-746)   */
+  int    entropy_switch;      /* 1: ENTROPY_SWITCH ALWAYS (pressure from the
+                                 advected entropy, HD/mappers.c:186-204)      */
 } orc_wind_cfg;
 
 /* Advance V ([6][nx2][nx1], in/out) by nsteps steps of the reference's

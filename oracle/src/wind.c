@@ -88,6 +88,9 @@ typedef struct {
   double invDt_hyp, dt_cool, t;
   int stage;
   unsigned char *flag;                /* zone flags [j][i] (FLAG_MINMOD, FLAG_HLL) */
+  /* ENTROPY_SWITCH: Vc[ENTR] = p / rho^gamma and Uc[ENTR] = rho Vc[ENTR]
+   * (pluto.h:737-746), kept beside V and U, [j][i] */
+  double *S, *Ue, *U0e;
 } W;
 
 enum { FLAG_MINMOD = 1, FLAG_HLL = 4 };   /* Src/pluto.h:212-214 */
@@ -125,6 +128,31 @@ static int c2p(double gamma, double *u, double *v)
   v[PRS] = gmm1*(u[ENG] - kin);
   if (v[PRS] < 0.0) { v[PRS] = 1.e-12; u[ENG] = v[PRS]/gmm1 + kin; fail |= 4; }
   v[TRC] = u[TRC]*tau;                         /* mappers.c:247-249 */
+  return fail;
+}
+
+/* ConsToPrim with FLAG_ENTROPY (ENTROPY_SWITCH ALWAYS flags every zone,
+ * flag_shock.c:140-143): the pressure from the entropy, the energy
+ * redefined from it (Src/HD/mappers.c:129-204, :247-249). */
+static int c2p_entropy(double gamma, double *u, double *ue, double *v, double *s)
+{
+  double gmm1 = gamma - 1.0, m2, tau, kin, rhog1;
+  int fail = 0;
+  m2 = u[VX1]*u[VX1] + u[VX2]*u[VX2] + u[VX3]*u[VX3];
+  if (u[RHO] < 0.0) { u[RHO] = 1.e-12; fail = 1; }
+  v[RHO] = u[RHO];
+  tau = 1.0/u[RHO];
+  v[VX1] = u[VX1]*tau;
+  v[VX2] = u[VX2]*tau;
+  v[VX3] = u[VX3]*tau;
+  kin = 0.5*m2/u[RHO];
+  if (u[ENG] < 0.0) { u[ENG] = 1.e-12/gmm1 + kin; fail |= 2; }
+  rhog1 = pow(v[RHO], gmm1);
+  v[PRS] = (*ue)*rhog1;
+  if (v[PRS] < 0.0) { v[PRS] = 1.e-12; fail |= 4; }
+  u[ENG] = v[PRS]/gmm1 + kin;
+  v[TRC] = u[TRC]*tau;
+  *s = (*ue)*tau;
   return fail;
 }
 
@@ -224,6 +252,9 @@ static void internal_floors(W *w)
       if (floor_cell(w, v)) {
         for (nv = 0; nv < NVW; nv++) w->V[VI(w, nv, j, i)] = v[nv];
         p2c(w->c->gamma, v, w->U + NVW*CI(w, j, i));
+        /* PrimToCons3D of the changed zone also forms Uc[ENTR] = rho
+         * Vc[ENTR], from the entropy the last cons->prim left */
+        if (w->c->entropy_switch) w->Ue[CI(w, j, i)] = v[RHO]*w->S[CI(w, j, i)];
         cnt[CNT_FLOOR_D]++;
       }
     }
@@ -410,6 +441,8 @@ static void riemann_face(double gamma, int n, int hll, const double *vL, const d
 /* ------------------------------------------------------------------ */
 /* boundaries (Src/boundary.c), sides in order x1beg, x1end, x2beg, x2end */
 /* ------------------------------------------------------------------ */
+static void compute_entropy(W *w);
+
 static void boundary(W *w)
 {
   int side, nv;
@@ -439,6 +472,17 @@ static void boundary(W *w)
       }
     }
   }
+  if (w->c->entropy_switch) compute_entropy(w);   /* boundary.c:491-493 */
+}
+
+/* ComputeEntropy (Src/entropy_switch.c:14-37, Entropy() of
+ * Src/EOS/Ideal/eos.c:77-96): s = p / rho^gamma on every zone. */
+static void compute_entropy(W *w)
+{
+  int i, j;
+  for (j = 0; j < w->t2; j++)
+    for (i = 0; i < w->t1; i++)
+      w->S[CI(w, j, i)] = w->V[VI(w, PRS, j, i)]/pow(w->V[VI(w, RHO, j, i)], w->c->gamma);
 }
 
 /* ------------------------------------------------------------------ */
@@ -580,6 +624,11 @@ static void update_stage(W *w, double dt)
   double (*rhs)[NVW] = malloc(sizeof(double)*NVW*maxn);
   double *press = malloc(sizeof(double)*maxn), *cmax = malloc(sizeof(double)*maxn);
   double *dv = malloc(sizeof(double)*NVW*maxn);
+  /* the entropy scalar: zone values, face states, flux, F.A, rhs */
+  double *se = malloc(sizeof(double)*maxn), *sep = malloc(sizeof(double)*maxn);
+  double *sem = malloc(sizeof(double)*maxn), *fe = malloc(sizeof(double)*maxn);
+  double *fAe = malloc(sizeof(double)*maxn), *rhse = malloc(sizeof(double)*maxn);
+  const int ent = w->c->entropy_switch;
 
   if (w->stage == 1) memset(w->Cdt, 0, sizeof(double)*w->t1*w->t2);
   vgrad(w);
@@ -594,6 +643,8 @@ static void update_stage(W *w, double dt)
       for (ip = 0; ip < ntot; ip++)
         for (nv = 0; nv < NVW; nv++)
           v[ip][nv] = d ? w->V[VI(w, nv, ip, i)] : w->V[VI(w, nv, j, ip)];
+      if (ent)
+        for (ip = 0; ip < ntot; ip++) se[ip] = d ? w->S[CI(w, ip, i)] : w->S[CI(w, j, ip)];
 
       /* States, plm_states.c:141-258 (general grid, van Leer) */
       for (ip = nbeg - 2; ip <= nend + 1; ip++)
@@ -620,6 +671,20 @@ static void update_stage(W *w, double dt)
           vm[ip][nv] = v[ip][nv] - dl*dm[ip];
         }
       }
+      /* the entropy scalar: limited in primitive form like the tracer
+       * (van Leer; minmod in FLAG_MINMOD zones unless CHAR_LIMITING,
+       * whose scalars always take the van Leer limiter) */
+      if (ent)
+        for (ip = nbeg - 1; ip <= nend + 1; ip++) {
+          int mm = !w->c->char_limiting &&
+                   ((d ? w->flag[CI(w, ip, i)] : w->flag[CI(w, j, ip)]) & FLAG_MINMOD);
+          double dvp = (se[ip + 1] - se[ip])*wp[ip], dvm = (se[ip] - se[ip - 1])*wm[ip], dl;
+          if (mm) dl = (dvp*dvm > 0.0 ? (fabs(dvp) < fabs(dvm) ? dvp : dvm) : 0.0);
+          else dl = (dvp*dvm > 0.0 ? (dvp)*(dvm)*(cp[ip]*(dvm) + cm[ip]*(dvp))
+                     /((dvp)*(dvp) + (dvm)*(dvm) + (cp[ip] + cm[ip] - 2.0)*(dvp)*(dvm)) : 0.0);
+          sep[ip] = se[ip] + dl*dp[ip];
+          sem[ip] = se[ip] - dl*dm[ip];
+        }
       /* Riemann + tracer flux (adv_flux.c) */
       for (ip = nbeg - 1; ip <= nend; ip++) {
         double uL[NVW], uR[NVW];
@@ -630,11 +695,16 @@ static void update_stage(W *w, double dt)
         riemann_face(gamma, d, (f0 | f1) & FLAG_HLL, vp[ip], uL, vm[ip + 1], uR, flux[ip],
                      &press[ip], &cmax[ip]);
         flux[ip][TRC] = flux[ip][RHO]*(flux[ip][RHO] > 0.0 ? vp[ip][TRC] : vm[ip + 1][TRC]);
+        if (ent) {                                /* adv_flux.c:131-134 */
+          if (flux[ip][RHO] >= 0.0) fe[ip] = sep[ip]*flux[ip][RHO];
+          else fe[ip] = sem[ip + 1]*flux[ip][RHO];
+        }
       }
       /* TotalFlux (rhs.c:466-560) */
       for (ip = nbeg - 1; ip <= nend; ip++) {
         double A = d ? w->A2[CI(w, ip, i)] : w->A1[CI(w, j, ip)];
         for (nv = 0; nv < NVW; nv++) fA[ip][nv] = flux[ip][nv]*A;
+        if (ent) fAe[ip] = fe[ip]*A;
         if (d == 0) fA[ip][VX3] *= fabs(w->x1r[ip]);
         else fA[ip][VX3] *= fabs(w->sp[ip]);
       }
@@ -645,6 +715,7 @@ static void update_stage(W *w, double dt)
         double dtdl = d ? dt/w->dx2[ip]*(1.0/w->rt[i]) : dt/w->dx1[ip];
         double vc[NVW], g[3], grad[3], Sm;
         for (nv = 0; nv < NVW; nv++) rhs[ip][nv] = -dtdV*(fA[ip][nv] - fA[ip - 1][nv]);
+        if (ent) rhse[ip] = -dtdV*(fAe[ip] - fAe[ip - 1]);
         rhs[ip][VX1 + d] -= dtdl*(press[ip] - press[ip - 1]);
         if (d == 0) rhs[ip][VX3] /= fabs(w->x1[ip]);
         else rhs[ip][VX3] /= fabs(w->s[ip]);
@@ -684,6 +755,7 @@ static void update_stage(W *w, double dt)
       for (ip = nbeg; ip <= nend; ip++) {
         double *u = w->U + NVW*(d ? CI(w, ip, i) : CI(w, j, ip));
         for (nv = 0; nv < NVW; nv++) u[nv] += rhs[ip][nv];
+        if (ent) w->Ue[d ? CI(w, ip, i) : CI(w, j, ip)] += rhse[ip];
       }
       if (w->stage == 1) {                      /* update_stage.c:303-316 */
         for (ip = nbeg; ip <= nend; ip++) {
@@ -700,7 +772,7 @@ static void update_stage(W *w, double dt)
     w->invDt_hyp /= (double)2;
   }
   free(v); free(vp); free(vm); free(flux); free(fA); free(rhs); free(press); free(cmax);
-  free(dv);
+  free(dv); free(se); free(sep); free(sem); free(fe); free(fAe); free(rhse);
 }
 
 static void advance_step(W *w, double dt)
@@ -716,13 +788,18 @@ static void advance_step(W *w, double dt)
       double v[NVW];
       for (nv = 0; nv < NVW; nv++) v[nv] = w->V[VI(w, nv, j, i)];
       p2c(w->c->gamma, v, w->U + NVW*CI(w, j, i));
+      if (w->c->entropy_switch) w->Ue[CI(w, j, i)] = v[RHO]*w->S[CI(w, j, i)];
     }
   memcpy(w->U0, w->U, sizeof(double)*NVW*nc);
+  if (w->c->entropy_switch) memcpy(w->U0e, w->Ue, sizeof(double)*nc);
   update_stage(w, dt);
   for (j = NG; j < NG + w->n2; j++)
     for (i = NG; i < NG + w->n1; i++) {
       double v[NVW];
-      if (c2p(w->c->gamma, w->U + NVW*CI(w, j, i), v)) cnt[CNT_C2P_FAIL]++;
+      size_t c = CI(w, j, i);
+      int f = w->c->entropy_switch ? c2p_entropy(w->c->gamma, w->U + NVW*c, &w->Ue[c], v, &w->S[c])
+                                   : c2p(w->c->gamma, w->U + NVW*c, v);
+      if (f) cnt[CNT_C2P_FAIL]++;
       for (nv = 0; nv < NVW; nv++) w->V[VI(w, nv, j, i)] = v[nv];
     }
   w->stage = 2;
@@ -730,9 +807,17 @@ static void advance_step(W *w, double dt)
   update_stage(w, dt);
   for (j = NG; j < NG + w->n2; j++)
     for (i = NG; i < NG + w->n1; i++) {
-      double v[NVW], *u = w->U + NVW*CI(w, j, i), *u0 = w->U0 + NVW*CI(w, j, i);
+      size_t c = CI(w, j, i);
+      double v[NVW], *u = w->U + NVW*c, *u0 = w->U0 + NVW*c;
+      int f;
       for (nv = 0; nv < NVW; nv++) u[nv] = 0.5*u0[nv] + 0.5*u[nv];
-      if (c2p(w->c->gamma, u, v)) cnt[CNT_C2P_FAIL]++;
+      if (w->c->entropy_switch) {
+        w->Ue[c] = 0.5*w->U0e[c] + 0.5*w->Ue[c];
+        f = c2p_entropy(w->c->gamma, u, &w->Ue[c], v, &w->S[c]);
+      } else {
+        f = c2p(w->c->gamma, u, v);
+      }
+      if (f) cnt[CNT_C2P_FAIL]++;
       for (nv = 0; nv < NVW; nv++) w->V[VI(w, nv, j, i)] = v[nv];
     }
 }
@@ -1100,6 +1185,8 @@ static void w_init(W *pw, const orc_wind_cfg *c, const double *Vin)
   w.x22_0 = calloc(nc, sizeof(double)); w.x22_1 = calloc(nc, sizeof(double));
   w.xi_s = calloc(nc, sizeof(double)); w.tr_s = calloc(nc, sizeof(double));
   w.flag = calloc(nc, 1);
+  w.S = calloc(nc, sizeof(double)); w.Ue = calloc(nc, sizeof(double));
+  w.U0e = calloc(nc, sizeof(double));
 
   geometry(&w);
   for (a = 0; a < w.na; a++)
@@ -1135,7 +1222,7 @@ static void w_free(W *pw)
   for (d = 0; d < 2; d++) for (a = 0; a < 6; a++) free(w.pc[d][a]);
   free(w.fr); free(w.ft); free(w.fp); free(w.roff); free(w.toff); free(w.moff); free(w.dvds);
   free(w.x11_0); free(w.x11_1); free(w.x22_0); free(w.x22_1); free(w.xi_s); free(w.tr_s);
-  free(w.flag);
+  free(w.flag); free(w.S); free(w.Ue); free(w.U0e);
 }
 
 /* Copy the interior of the state back to [nv][nx2][nx1] arrays. */

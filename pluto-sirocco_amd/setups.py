@@ -243,6 +243,8 @@ class WindConfig:
     shock_flattening: bool = False
     # CHAR_LIMITING YES: PLM limits the characteristic differences
     char_limiting: bool = False
+    # ENTROPY_SWITCH ALWAYS: the pressure comes from an advected entropy
+    entropy_switch: bool = False
     # units of the problem (definitions.h)
     unit_density: float = 1e-14
     unit_length: float = 1e9
@@ -263,12 +265,17 @@ class WindConfig:
         if self.dfloor > 0.0:
             if self.cooling != COOL_PLAIN:
                 raise ValueError("the cv_idl boundary build uses the plain cooling")
+            if self.entropy_switch:
+                if not (self.shock_flattening and self.char_limiting):
+                    raise ValueError("the entropy switch is built with the production "
+                                     "reconstruction")
+                return "wind_cvidl_prod"
             if self.char_limiting:
                 if not self.shock_flattening:
                     raise ValueError("characteristic limiting is built with shock flattening")
                 return "wind_cvidl_char"
             return "wind_cvidl_flat" if self.shock_flattening else "wind_cvidl"
-        if self.shock_flattening or self.char_limiting:
+        if self.shock_flattening or self.char_limiting or self.entropy_switch:
             raise ValueError("shock flattening / characteristic limiting are built with the "
                              "cv_idl boundaries only")
         return {COOL_BLONDIN: "wind_blondin", COOL_PLAIN: "wind_plain"}[self.cooling]

@@ -52,6 +52,7 @@ COVERAGE = {
     "wind_cvidl_mixed_32x16": ["floor_cells", "ck45_calls", "c2p_fail"],
     "wind_cvidl_flat_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
     "wind_cvidl_char_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
+    "wind_cvidl_prod_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
 }
 
 
@@ -107,6 +108,20 @@ def test_char_limiting_fixture_pins_the_projection():
             {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
     fl = (g["flux_r"], g["flux_t"], g["flux_p"])
     r = refrun.run_wind_oracle(dataclasses.replace(cfg, char_limiting=False), g["V0"],
+                               int(g["nsteps"]), fluxes=fl, grid=grid)
+    assert not np.array_equal(r["V"], g["V"])
+
+
+def test_entropy_switch_fixture_pins_the_entropy_path():
+    """With ENTROPY_SWITCH ALWAYS the pressure comes from the advected
+    entropy; recovering it from the energy instead changes the fixture's
+    result."""
+    import dataclasses
+    cfg, g = golden_io.load("wind_cvidl_prod_mixed_32x16")
+    grid = ({"x": g["x1"], "xl": g["x1l"], "xr": g["x1r"], "dx": g["dx1"]},
+            {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
+    fl = (g["flux_r"], g["flux_t"], g["flux_p"])
+    r = refrun.run_wind_oracle(dataclasses.replace(cfg, entropy_switch=False), g["V0"],
                                int(g["nsteps"]), fluxes=fl, grid=grid)
     assert not np.array_equal(r["V"], g["V"])
 
