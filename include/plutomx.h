@@ -161,6 +161,11 @@ typedef struct {
    * Src/HD/eigenv.c:92-189, :575-605), then bounds the primitive slopes by
    * the primitive differences; the tracer is limited in primitive form. */
   int char_limiting;
+  /* 1: ENTROPY_SWITCH ALWAYS: an entropy s = p / rho^gamma, set from the
+   * bounded state before each stage (Src/entropy_switch.c:14-37), is
+   * reconstructed and advected with the mass flux (Src/adv_flux.c:131-134);
+   * cons->prim takes the pressure from it (Src/HD/mappers.c:186-204). */
+  int entropy_switch;
 } pmx_wind_desc;
 
 /* Create a context: uploads geometry, PLM weights and the line-force

@@ -64,6 +64,7 @@ struct StageArgs {
   // SHOCK_FLATTENING MULTID: 1 per shocked zone over the interior and two
   // ghost layers, [nt+4][nr+4]; null without flattening
   const unsigned char* shock;
+  double* Ue;          // ENTROPY_SWITCH: Uc[ENTR] = rho s per zone; null without it
 };
 
 // The wind's userdef sides: r-beg maps like outflow (plus v_r <= 0), theta-
@@ -223,7 +224,12 @@ __device__ __forceinline__ void plm_zone(const StageArgs& a, int j, int i, const
 
 // One RK2 stage (Src/Time_Stepping/rk_step.c:117-330 with UpdateStage,
 // Src/Time_Stepping/update_stage.c:35-410, for GEOMETRY SPHERICAL in 2-D).
-template <int STAGE, int CHAR>
+// FEAT: bit 0 builds the characteristic-limiting path, bit 1 the entropy
+// switch (kernels for grids without them leave them out: they cost
+// registers); grids of a kernel built with a feature still check it.
+enum { FEAT_CHAR = 1, FEAT_ENT = 2 };
+
+template <int STAGE, int FEAT>
 __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, double& cdt,
                                            int& flags) {
   {
@@ -238,10 +244,18 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
     const int floored = ld6(a, a.Vin, j, i, vc);
     ld6(a, j, i - 1, w1);
     ld6(a, j, i + 1, e1);
+    const bool ent = (FEAT & FEAT_ENT) && k.ent;
+    const bool smm = !k.char_lim;      // scalars follow FLAG_MINMOD without characteristics
+    double sc = 0.0, sw1 = 0.0, se1 = 0.0;
+    if (ent) {
+      sc = entropy_of(vc, k.gamma);
+      sw1 = entropy_of(w1, k.gamma);
+      se1 = entropy_of(e1, k.gamma);
+    }
     double c[6];
     double vpi[NVW], vmi[NVW];
     ldc(T.pr, a.nr, I, c);
-    plm_zone<0, CHAR>(a, j, i, w1, vc, e1, c, vpi, vmi);
+    plm_zone<0, FEAT & FEAT_CHAR>(a, j, i, w1, vc, e1, c, vpi, vmi);
 
     // ---- VGradCalc + LineForce for both sweeps (line_connect.c:628-738,
     //      :815-903): x1 uses vc = (vp + vm)/2, x2 the cell centre ------
@@ -316,17 +330,28 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
 
     // ---- x1 faces i-1/2 and i+1/2 --------------------------------------
     double rhs1[NVW], Fm[NVW], Fp[NVW], pm, pp, cm1, cp1;
+    double fem = 0.0, fep = 0.0;       // entropy fluxes (adv_flux.c:131-134)
     {
       double w2[NVW], e2[NVW], vpl[NVW], vml[NVW], vpr[NVW], vmr[NVW];
       ld6(a, j, i - 2, w2);
       ld6(a, j, i + 2, e2);
-      ldc(T.pr, a.nr, I - 1, c);
-      plm_zone<0, CHAR>(a, j, i - 1, w2, w1, vc, c, vpl, vml);
-      ldc(T.pr, a.nr, I + 1, c);
-      plm_zone<0, CHAR>(a, j, i + 1, vc, e1, e2, c, vpr, vmr);
+      double cl[6], cr[6];
+      ldc(T.pr, a.nr, I - 1, cl);
+      plm_zone<0, FEAT & FEAT_CHAR>(a, j, i - 1, w2, w1, vc, cl, vpl, vml);
+      ldc(T.pr, a.nr, I + 1, cr);
+      plm_zone<0, FEAT & FEAT_CHAR>(a, j, i + 1, vc, e1, e2, cr, vpr, vmr);
       face<0>(vpl, vmi, k, Fm, pm, cm1, hll_interface(a, j, i - 1, j, i));
       face<0>(vpi, vmr, k, Fp, pp, cp1, hll_interface(a, j, i, j, i + 1));
+      if (ent) {
+        double spl, sml, spi, smi, spr, smr;
+        plm_scalar(entropy_of(w2, k.gamma), sw1, sc, cl, smm && minmod_zone(a, j, i - 1), spl, sml);
+        plm_scalar(sw1, sc, se1, c, smm && minmod_zone(a, j, i), spi, smi);
+        plm_scalar(sc, se1, entropy_of(e2, k.gamma), cr, smm && minmod_zone(a, j, i + 1), spr, smr);
+        fem = Fm[RHO] >= 0.0 ? spl * Fm[RHO] : smi * Fm[RHO];
+        fep = Fp[RHO] >= 0.0 ? spi * Fp[RHO] : smr * Fp[RHO];
+      }
     }
+    double rhs1e = 0.0;
     {
       const double dmu = T.dmu[J];
       double xrm = T.x1r[I - 1], xrp = T.x1r[I];
@@ -340,6 +365,7 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       double dtdl = dt / T.dx1[I];
 #pragma unroll
       for (int nv = 0; nv < NVW; nv++) rhs1[nv] = -dtdV * (fAp[nv] - fAm[nv]);
+      if (ent) rhs1e = -dtdV * (fep * Ap - fem * Am);
       rhs1[VX1] -= dtdl * (pp - pm);
       rhs1[VX3] /= fabs(T.x1[I]);
       // rhs_source.c:227-297
@@ -360,6 +386,7 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
 
     // ---- x2 faces j-1/2 and j+1/2 --------------------------------------
     double rhs2[NVW], Gm[NVW], Gp[NVW], qm, qp, cm2, cp2;
+    double gem = 0.0, gep = 0.0;
     {
       double s1[NVW], n1[NVW], s2[NVW], n2[NVW];
       ld6(a, j - 1, i, s1);
@@ -367,15 +394,26 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       ld6(a, j - 2, i, s2);
       ld6(a, j + 2, i, n2);
       double vpc[NVW], vmc[NVW], vpl[NVW], vml[NVW], vpr[NVW], vmr[NVW];
+      double cl[6], cr[6];
       ldc(T.pt, a.nt, J, c);
-      plm_zone<1, CHAR>(a, j, i, s1, vc, n1, c, vpc, vmc);
-      ldc(T.pt, a.nt, J - 1, c);
-      plm_zone<1, CHAR>(a, j - 1, i, s2, s1, vc, c, vpl, vml);
-      ldc(T.pt, a.nt, J + 1, c);
-      plm_zone<1, CHAR>(a, j + 1, i, vc, n1, n2, c, vpr, vmr);
+      plm_zone<1, FEAT & FEAT_CHAR>(a, j, i, s1, vc, n1, c, vpc, vmc);
+      ldc(T.pt, a.nt, J - 1, cl);
+      plm_zone<1, FEAT & FEAT_CHAR>(a, j - 1, i, s2, s1, vc, cl, vpl, vml);
+      ldc(T.pt, a.nt, J + 1, cr);
+      plm_zone<1, FEAT & FEAT_CHAR>(a, j + 1, i, vc, n1, n2, cr, vpr, vmr);
       face<1>(vpl, vmc, k, Gm, qm, cm2, hll_interface(a, j - 1, i, j, i));
       face<1>(vpc, vmr, k, Gp, qp, cp2, hll_interface(a, j, i, j + 1, i));
+      if (ent) {
+        const double ss1 = entropy_of(s1, k.gamma), sn1 = entropy_of(n1, k.gamma);
+        double spl, sml, spc, smc, spr, smr;
+        plm_scalar(entropy_of(s2, k.gamma), ss1, sc, cl, smm && minmod_zone(a, j - 1, i), spl, sml);
+        plm_scalar(ss1, sc, sn1, c, smm && minmod_zone(a, j, i), spc, smc);
+        plm_scalar(sc, sn1, entropy_of(n2, k.gamma), cr, smm && minmod_zone(a, j + 1, i), spr, smr);
+        gem = Gm[RHO] >= 0.0 ? spl * Gm[RHO] : smc * Gm[RHO];
+        gep = Gp[RHO] >= 0.0 ? spc * Gp[RHO] : smr * Gp[RHO];
+      }
     }
+    double rhs2e = 0.0;
     {
       double base = fabs(T.x1[I]) * T.dx1[I];
       double Am = base * T.sp[J - 1], Ap = base * T.sp[J];
@@ -388,6 +426,7 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       double dtdl = dt / T.dx2[J] * (1.0 / T.rt[I]);
 #pragma unroll
       for (int nv = 0; nv < NVW; nv++) rhs2[nv] = -dtdV * (fAp[nv] - fAm[nv]);
+      if (ent) rhs2e = -dtdV * (gep * Ap - gem * Am);
       rhs2[VX2] -= dtdl * (qp - qm);
       rhs2[VX3] /= fabs(T.s[J]);
       // rhs_source.c:313-396
@@ -432,7 +471,29 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       for (int nv = 0; nv < NVW; nv++) u[nv] = 0.5 * u0[nv] + 0.5 * u[nv];
     }
     double vo[NVW];
-    flags = c2p6(u, vo, k.gmm1);
+    if (ent) {
+      // Uc[ENTR]: stage 1 PrimToCons of the bounded state; stage 2 the
+      // stage-1 value, or for a zone the floors changed rho times the
+      // entropy the stage-1 cons->prim left (Ue / rho of the stage-1 U)
+      double ue;
+      if (STAGE == 1) ue = vc[RHO] * sc;
+      else if (floored) ue = vc[RHO] * (a.Ue[cell] * (1.0 / a.U[RHO * a.sv + cell]));
+      else ue = a.Ue[cell];
+      ue += rhs1e;
+      ue += rhs2e;
+      if (STAGE == 2) {
+        double vn[NVW];
+#pragma unroll
+        for (int nv = 0; nv < NVW; nv++) vn[nv] = a.Vn[nv * a.sv + cell];
+        if (k.dfloor > 0.0) floor_cell(k, vn);
+        const double u0e = vn[RHO] * entropy_of(vn, k.gamma);
+        ue = 0.5 * u0e + 0.5 * ue;
+      }
+      flags = c2p6_entropy(u, ue, vo, k.gmm1);
+      a.Ue[cell] = ue;
+    } else {
+      flags = c2p6(u, vo, k.gmm1);
+    }
 #pragma unroll
     for (int nv = 0; nv < NVW; nv++) {
       a.U[nv * a.sv + cell] = u[nv];
@@ -509,13 +570,13 @@ __global__ __launch_bounds__(BX* BY) void wind_flag_kernel(StageArgs a, unsigned
   shock[(long long)J * (a.nr + 4) + I] = f;
 }
 
-template <int STAGE, int CHAR>
+template <int STAGE, int FEAT>
 __global__ __launch_bounds__(BX* BY) void wind_stage_kernel(StageArgs a) {
   const int i = blockIdx.x * BX + threadIdx.x;
   const int j = blockIdx.y * BY + threadIdx.y;
   double cdt = 0.0;
   int flags = 0;
-  if (i < a.nr && j < a.nt) cell_stage<STAGE, CHAR>(a, i, j, cdt, flags);
+  if (i < a.nr && j < a.nt) cell_stage<STAGE, FEAT>(a, i, j, cdt, flags);
   if (block_reduce_stage<STAGE, BX * BY>(a.s, cdt, flags)) {
     if (STAGE == 2 && blockIdx.x == 0 && blockIdx.y == 0) {
       // update_stage.c:389-394: invDt = MAX(invDt, C_dt) over the domain,
@@ -628,6 +689,7 @@ struct pmx_wind {
   double* tab;        // one allocation for all geometry tables
   double* lf;
   unsigned char* shock;  // SHOCK_FLATTENING MULTID zone flags, or null
+  double* Ue;            // ENTROPY_SWITCH: Uc[ENTR], or null
   Tables T;
   Scalars* s;
   long step;
@@ -699,6 +761,7 @@ StageArgs stage_args(pmx_wind* w, const double* Vin, double* Vout) {
   a.T = w->T;
   a.s = w->s;
   a.shock = w->shock;
+  a.Ue = w->Ue;
   return a;
 }
 
@@ -712,26 +775,30 @@ int flag_shock(pmx_wind* w, hipStream_t st) {
   return 0;
 }
 
+int features(const pmx_wind* w) {
+  return (w->k.char_lim ? FEAT_CHAR : 0) | (w->k.ent ? FEAT_ENT : 0);
+}
+
+template <int STAGE>
+void launch_stage(pmx_wind* w, dim3 grid, dim3 block, const StageArgs& a) {
+  switch (features(w)) {
+    case 0: hipLaunchKernelGGL((wind_stage_kernel<STAGE, 0>), grid, block, 0, w->stream, a); break;
+    case 1: hipLaunchKernelGGL((wind_stage_kernel<STAGE, 1>), grid, block, 0, w->stream, a); break;
+    case 2: hipLaunchKernelGGL((wind_stage_kernel<STAGE, 2>), grid, block, 0, w->stream, a); break;
+    default: hipLaunchKernelGGL((wind_stage_kernel<STAGE, 3>), grid, block, 0, w->stream, a);
+  }
+}
+
 int advance(pmx_wind* w) {
   dim3 grid((w->nr + BX - 1) / BX, (w->nt + BY - 1) / BY), block(BX, BY);
   hipEvent_t e[2];
   if (flag_shock(w, w->stream)) return -1;
   if (timed_begin(w, e)) return -1;
-  if (w->k.char_lim)
-    hipLaunchKernelGGL((wind_stage_kernel<1, 1>), grid, block, 0, w->stream,
-                       stage_args(w, w->V[0], w->V[1]));
-  else
-    hipLaunchKernelGGL((wind_stage_kernel<1, 0>), grid, block, 0, w->stream,
-                       stage_args(w, w->V[0], w->V[1]));
+  launch_stage<1>(w, grid, block, stage_args(w, w->V[0], w->V[1]));
   PMX_HIP_CHECK(hipGetLastError());
   if (timed_end(w, e, false)) return -1;
   if (timed_begin(w, e)) return -1;
-  if (w->k.char_lim)
-    hipLaunchKernelGGL((wind_stage_kernel<2, 1>), grid, block, 0, w->stream,
-                       stage_args(w, w->V[1], w->V[0]));
-  else
-    hipLaunchKernelGGL((wind_stage_kernel<2, 0>), grid, block, 0, w->stream,
-                       stage_args(w, w->V[1], w->V[0]));
+  launch_stage<2>(w, grid, block, stage_args(w, w->V[1], w->V[0]));
   PMX_HIP_CHECK(hipGetLastError());
   return timed_end(w, e, false);
 }
@@ -819,6 +886,7 @@ Const make_const(const pmx_wind_desc* d) {
   k.dfloor = d->dfloor / k.UD;
   k.pfloor = k.dfloor * 5.e2 / (k.KELVIN * d->mu);
   k.char_lim = d->char_limiting != 0;
+  k.ent = d->entropy_switch != 0;
   return k;
 }
 }  // namespace wind
@@ -833,7 +901,7 @@ int pmx_wind_destroy(pmx_wind* w) {
   for (auto& p : w->pend_stage) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
   for (auto& p : w->pend_cool) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
   hipFree(w->V[0]); hipFree(w->V[1]); hipFree(w->U); hipFree(w->xi_s); hipFree(w->tr_s);
-  hipFree(w->tab); hipFree(w->lf); hipFree(w->s); hipFree(w->shock);
+  hipFree(w->tab); hipFree(w->lf); hipFree(w->s); hipFree(w->shock); hipFree(w->Ue);
   hipFree(w->sw.list); hipFree(w->sw.fb); hipFree(w->sw.feat); hipFree(w->sw.yhat);
   hipFree(w->sw.f0); hipFree(w->sw.cnt); hipFree(w->cargs); hipFree(w->coff);
   if (w->stream) hipStreamDestroy(w->stream);
@@ -984,6 +1052,7 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
   CK(hipMalloc(&w->lf, sizeof(double) * lf.size()));
   CK(hipMalloc(&w->s, sizeof(Scalars)));
   if (d->shock_flattening) CK(hipMalloc(&w->shock, (size_t)(w->nr + 4) * (w->nt + 4)));
+  if (d->entropy_switch) CK(hipMalloc(&w->Ue, sizeof(double) * w->sv));
   w->T.x1l0 = x1l[0];
   w->T.sin_x2l0 = fabs(sin(x2l[0]));
   CK(hipMalloc(&w->sw.cnt, sizeof(unsigned long long) * 8));
@@ -1206,7 +1275,7 @@ __device__ __forceinline__ int find_item(const int* off, int G, int q) {
   return lo;
 }
 
-template <int STAGE, int CHAR, int TBX, int TBY>
+template <int STAGE, int FEAT, int TBX, int TBY>
 __global__ __launch_bounds__(TBX* TBY) __attribute__((amdgpu_waves_per_eu(PMX_WIND_WAVES, PMX_WIND_WAVES)))
 void wind_batch_stage_kernel(const StageArgs* args, const int* tile_off, int G, int total) {
   for (int item = blockIdx.x; item < total; item += gridDim.x) {
@@ -1218,7 +1287,7 @@ void wind_batch_stage_kernel(const StageArgs* args, const int* tile_off, int G, 
     const int i = tx * TBX + threadIdx.x, j = ty * TBY + threadIdx.y;
     double cdt = 0.0;
     int flags = 0;
-    if (i < a.nr && j < a.nt) cell_stage<STAGE, CHAR>(a, i, j, cdt, flags);
+    if (i < a.nr && j < a.nt) cell_stage<STAGE, FEAT>(a, i, j, cdt, flags);
     block_reduce_stage<STAGE, TBX * TBY>(a.s, cdt, flags);
   }
 }
@@ -1268,7 +1337,7 @@ struct pmx_wind_batch {
   long long* cell_off;    // device, G + 1 (multiples of 128)
   long long total_cells;  // padded
   long long cells;        // interior cells of all grids
-  int char_any;           // some grid limits characteristic variables
+  int feat_any;           // some grid uses characteristic limiting or the entropy switch
   int shape;              // index into the tile-shape table
   int blocks;             // persistent workgroups per stage launch
   hipStream_t stream;
@@ -1297,8 +1366,8 @@ void launch_batch_stage(pmx_wind_batch* b) {
   const dim3 grid(b->blocks);
 #define PMX_SHAPE(q, X, Y)                                                                   \
   case q:                                                                                    \
-    if (b->char_any)                                                                         \
-      hipLaunchKernelGGL((wind_batch_stage_kernel<STAGE, 1, X, Y>), grid, dim3(X, Y), 0,     \
+    if (b->feat_any)                                                                         \
+      hipLaunchKernelGGL((wind_batch_stage_kernel<STAGE, 3, X, Y>), grid, dim3(X, Y), 0,     \
                          b->stream, args, b->tile_off, b->G, b->total_tiles);                \
     else                                                                                     \
       hipLaunchKernelGGL((wind_batch_stage_kernel<STAGE, 0, X, Y>), grid, dim3(X, Y), 0,     \
@@ -1450,7 +1519,7 @@ int pmx_wind_batch_create(const pmx_wind_desc* descs, int ngrids, pmx_wind_batch
     cmv[g] = w->cfl_max_var;
     coff[g + 1] = coff[g] + (w->sv + 127) / 128 * 128;
     b->cells += w->sv;
-    b->char_any |= w->k.char_lim;
+    b->feat_any |= features(w);
   }
   b->total_cells = coff[G];
   CK(hipMalloc(&b->args1, sizeof(StageArgs) * G));

@@ -50,6 +50,7 @@ struct Const {
   double E_cost;                           // UL / UD / UV^3, cv_idl/radiat.c:47
   double dfloor, pfloor;                   // internal-boundary floors (code units); 0: off
   int char_lim;                            // CHAR_LIMITING YES
+  int ent;                                 // ENTROPY_SWITCH ALWAYS
 };
 
 // ---------------------------------------------------------------------
@@ -99,6 +100,51 @@ PMX_HD int c2p6(double* u, double* v, double gmm1) {
   int f = cons_to_prim(u, v, gmm1);
   v[TRC] = u[TRC] * (1.0 / u[RHO]);
   return f;
+}
+
+// ConsToPrim with FLAG_ENTROPY, which ENTROPY_SWITCH ALWAYS sets on every
+// zone (flag_shock.c:140-143, HD/mappers.c:129-204, :247-249): the pressure
+// from the conserved entropy ue = rho s, s = p / rho^gamma, and the energy
+// redefined from it.
+PMX_HD int c2p6_entropy(double* u, double ue, double* v, double gmm1) {
+  const double small_rho = 1.e-12, small_p = 1.e-12;  // Src/globals.h:115-116
+  int fail = 0;
+  double m2 = u[VX1] * u[VX1] + u[VX2] * u[VX2] + u[VX3] * u[VX3];
+  if (u[RHO] < 0.0) { u[RHO] = small_rho; fail = 1; }
+  v[RHO] = u[RHO];
+  double tau = 1.0 / u[RHO];
+  v[VX1] = u[VX1] * tau;
+  v[VX2] = u[VX2] * tau;
+  v[VX3] = u[VX3] * tau;
+  double kin = 0.5 * m2 / u[RHO];
+  if (u[ENG] < 0.0) { u[ENG] = small_p / gmm1 + kin; fail |= 2; }
+  double rhog1 = pow(v[RHO], gmm1);
+  v[PRS] = ue * rhog1;
+  if (v[PRS] < 0.0) { v[PRS] = small_p; fail |= 4; }
+  u[ENG] = v[PRS] / gmm1 + kin;
+  v[TRC] = u[TRC] * tau;
+  return fail;
+}
+
+// The entropy scalar Vc[ENTR] = p / rho^gamma that ComputeEntropy sets on
+// every zone once the boundary is in place (entropy_switch.c:14-37,
+// EOS/Ideal/eos.c:77-96).
+PMX_HD double entropy_of(const double* v, double gamma) {
+  return v[PRS] / pow(v[RHO], gamma);
+}
+
+// PLM of one scalar (the entropy) as the tracer: van Leer, or minmod for a
+// FLAG_MINMOD zone (plm_states.c:177-258; characteristic limiting leaves
+// scalars on the van Leer limiter, :651-660).
+PMX_HD void plm_scalar(double sm1, double s0, double sp1, const double* c, bool mm, double& sp,
+                       double& sm) {
+  double dvp = (sp1 - s0) * c[2], dvm = (s0 - sm1) * c[3], dl;
+  if (mm) dl = (dvp * dvm > 0.0 ? (fabs(dvp) < fabs(dvm) ? dvp : dvm) : 0.0);
+  else dl = (dvp * dvm > 0.0 ? (dvp) * (dvm) * (c[0] * (dvm) + c[1] * (dvp)) /
+                                  ((dvp) * (dvp) + (dvm) * (dvm) + (c[0] + c[1] - 2.0) * (dvp) * (dvm))
+                            : 0.0);
+  sp = s0 + dl * c[4];
+  sm = s0 - dl * c[5];
 }
 
 // General-grid van Leer PLM of one cell (Src/States/plm_states.c:141-258

@@ -100,13 +100,14 @@ def test_every_tile_shape(pmx, shape):
 def test_batch_with_production_boundaries_matches_single_grids(pmx):
     """Grids with the production problem's floors and userdef sides (one
     also with shock flattening, one with flattening and characteristic
-    limiting) in a batch with two plain grids: every grid as on its own,
-    bit for bit."""
+    limiting, one with its full reconstruction incl. the entropy switch) in
+    a batch with two plain grids: every grid as on its own, bit for bit."""
     cv = dataclasses.replace(setups.cvidl_wind(40, 24, dfloor=1.e-15), first_dt=1.e-3)
     cvf = dataclasses.replace(setups.cvidl_wind(32, 16, dfloor=1.e-15), first_dt=1.e-3,
                               shock_flattening=True)
     cvc = dataclasses.replace(cvf, nx=(24, 24), char_limiting=True)
-    cfgs = small_batch()[:2] + [cv, cvf, cvc]
+    cvp = dataclasses.replace(cvc, nx=(32, 24), entropy_switch=True)
+    cfgs = small_batch()[:2] + [cv, cvf, cvc, cvp]
     V0s = [setups.wind_ic_mixed(c, seed=50 + g, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
            for g, c in enumerate(cfgs)]
     batch = run_batch(pmx, cfgs, V0s, 6, graph=True)

@@ -145,6 +145,34 @@ def test_wind_char_limiting_matches_oracle(pmx):
     np.testing.assert_allclose(dts, o["dt_seq"], rtol=RTOL, atol=0)
 
 
+def test_wind_production_reconstruction_matches_oracle(pmx):
+    """The production problem's full reconstruction (CHAR_LIMITING YES,
+    SHOCK_FLATTENING MULTID, ENTROPY_SWITCH ALWAYS) and boundaries on a
+    256 x 128 mixed state, 6 steps."""
+    import dataclasses
+    cfg = dataclasses.replace(setups.cvidl_wind(256, 128, dfloor=1.e-15), first_dt=1.e-3,
+                              shock_flattening=True, char_limiting=True, entropy_switch=True)
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
+    o = refrun.run_wind_oracle(cfg, V0, 6)
+    V, U, dts, tm = gpu_run(pmx, cfg, V0, 6)
+    assert_close(V, o["V"], "V")
+    assert_close(U, o["U"], "U")
+    np.testing.assert_allclose(dts, o["dt_seq"], rtol=RTOL, atol=0)
+
+
+def test_wind_entropy_switch_without_char_limiting_matches_oracle(pmx):
+    """The entropy switch with van Leer / minmod scalars (no characteristic
+    limiting) and shock flattening, 128 x 64, 6 steps."""
+    import dataclasses
+    cfg = dataclasses.replace(setups.cvidl_wind(128, 64, dfloor=1.e-15), first_dt=1.e-3,
+                              shock_flattening=True, entropy_switch=True)
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
+    o = refrun.run_wind_oracle(cfg, V0, 6)
+    V, U, dts, tm = gpu_run(pmx, cfg, V0, 6)
+    assert_close(V, o["V"], "V")
+    assert_close(U, o["U"], "U")
+
+
 @pytest.mark.parametrize("cooling", [setups.COOL_BLONDIN, setups.COOL_PLAIN])
 @pytest.mark.parametrize("t0", [0.0, 5.0])
 def test_wind_single_cooling_call(pmx, cooling, t0):
