@@ -26,6 +26,7 @@ extern "C" {
 /* Riemann solver (pluto.ini [Solver]) */
 #define PMX_SOLVER_HLLC 0
 #define PMX_SOLVER_ROE  1
+#define PMX_SOLVER_HLL  2   /* the wind only: Src/HD/hll.c, the production problem's [Solver] */
 /* boundary types (pluto.ini [Boundary]) */
 #define PMX_BC_PERIODIC   0
 #define PMX_BC_OUTFLOW    1
@@ -166,6 +167,7 @@ typedef struct {
    * reconstructed and advected with the mass flux (Src/adv_flux.c:131-134);
    * cons->prim takes the pressure from it (Src/HD/mappers.c:186-204). */
   int entropy_switch;
+  int solver;                   /* PMX_SOLVER_HLLC (0) or PMX_SOLVER_HLL */
 } pmx_wind_desc;
 
 /* Create a context: uploads geometry, PLM weights and the line-force

@@ -198,7 +198,8 @@ class _OrcWindCfg(ctypes.Structure):
                 ("nangles", ctypes.c_int),
                 ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp),
                 ("dfloor", ctypes.c_double), ("shock_flattening", ctypes.c_int),
-                ("char_limiting", ctypes.c_int), ("entropy_switch", ctypes.c_int)]
+                ("char_limiting", ctypes.c_int), ("entropy_switch", ctypes.c_int),
+                ("solver", ctypes.c_int)]
 
 
 def _wind_struct(cfg, fluxes=None, grid=None):
@@ -241,6 +242,7 @@ def _wind_struct(cfg, fluxes=None, grid=None):
     c.shock_flattening = int(cfg.shock_flattening)
     c.char_limiting = int(cfg.char_limiting)
     c.entropy_switch = int(cfg.entropy_switch)
+    c.solver = int(cfg.solver)
     return c, keep
 
 

@@ -82,6 +82,7 @@ typedef struct {
   int    char_limiting;       /* 1: CHAR_LIMITING YES (plm_states.c:479-746)   */
   int    entropy_switch;      /* 1: ENTROPY_SWITCH ALWAYS (pressure from the
                                  advected entropy, HD/mappers.c:186-204)      */
+  int    solver;              /* 0: HLLC, 2: HLL (Src/HD/hll.c)               */
 } orc_wind_cfg;
 
 /* Advance V ([6][nx2][nx1], in/out) by nsteps steps of the reference's

@@ -692,8 +692,9 @@ static void update_stage(W *w, double dt)
         int f1 = d ? w->flag[CI(w, ip + 1, i)] : w->flag[CI(w, j, ip + 1)];
         p2c(gamma, vp[ip], uL);
         p2c(gamma, vm[ip + 1], uR);
-        riemann_face(gamma, d, (f0 | f1) & FLAG_HLL, vp[ip], uL, vm[ip + 1], uR, flux[ip],
-                     &press[ip], &cmax[ip]);
+        /* HLL_Solver (Src/HD/hll.c:29-103) is HLLC's FLAG_HLL branch */
+        riemann_face(gamma, d, w->c->solver == 2 || ((f0 | f1) & FLAG_HLL), vp[ip], uL,
+                     vm[ip + 1], uR, flux[ip], &press[ip], &cmax[ip]);
         flux[ip][TRC] = flux[ip][RHO]*(flux[ip][RHO] > 0.0 ? vp[ip][TRC] : vm[ip + 1][TRC]);
         if (ent) {                                /* adv_flux.c:131-134 */
           if (flux[ip][RHO] >= 0.0) fe[ip] = sep[ip]*flux[ip][RHO];

@@ -187,7 +187,8 @@ __device__ __forceinline__ void hll_face(const double* vL, const double* vR, dou
 template <int DIR>
 __device__ __forceinline__ void face(const double* vL, const double* vR, const Const& k,
                                      double* F, double& p, double& cmax, bool hll = false) {
-  if (hll) hll_face<DIR>(vL, vR, k.gamma, k.gmm1, F, p, cmax);
+  // HLL_Solver (Src/HD/hll.c:29-103) is HLLC's FLAG_HLL branch
+  if (hll || k.hll) hll_face<DIR>(vL, vR, k.gamma, k.gmm1, F, p, cmax);
   else hllc<DIR>(vL, vR, k.gamma, k.gmm1, F, p, cmax);
   F[TRC] = F[RHO] * (F[RHO] > 0.0 ? vL[TRC] : vR[TRC]);
 }
@@ -887,6 +888,7 @@ Const make_const(const pmx_wind_desc* d) {
   k.pfloor = k.dfloor * 5.e2 / (k.KELVIN * d->mu);
   k.char_lim = d->char_limiting != 0;
   k.ent = d->entropy_switch != 0;
+  k.hll = d->solver == PMX_SOLVER_HLL;
   return k;
 }
 }  // namespace wind
@@ -924,6 +926,9 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
     return -1;
   }
   if (!(d->dfloor >= 0.0)) { set_error("dfloor must be >= 0"); return -1; }
+  if (d->solver != PMX_SOLVER_HLLC && d->solver != PMX_SOLVER_HLL) {
+    set_error("the wind's solver is PMX_SOLVER_HLLC or PMX_SOLVER_HLL"); return -1;
+  }
   if (!d->x1 || !d->x1l || !d->x1r || !d->dx1 || !d->x2 || !d->x2l || !d->x2r || !d->dx2 ||
       !d->flux_r || !d->flux_t || !d->flux_p) {
     set_error("missing grid or flux arrays"); return -1;

@@ -51,6 +51,7 @@ struct Const {
   double dfloor, pfloor;                   // internal-boundary floors (code units); 0: off
   int char_lim;                            // CHAR_LIMITING YES
   int ent;                                 // ENTROPY_SWITCH ALWAYS
+  int hll;                                 // [Solver] hll instead of hllc
 };
 
 // ---------------------------------------------------------------------

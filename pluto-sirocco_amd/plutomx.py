@@ -77,7 +77,8 @@ class WindDesc(ctypes.Structure):
                 ("nangles", ctypes.c_int),
                 ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp),
                 ("dfloor", ctypes.c_double), ("shock_flattening", ctypes.c_int),
-                ("char_limiting", ctypes.c_int), ("entropy_switch", ctypes.c_int)]
+                ("char_limiting", ctypes.c_int), ("entropy_switch", ctypes.c_int),
+                ("solver", ctypes.c_int)]
 
 
 _lib = None
@@ -295,6 +296,7 @@ def wind_desc_scalars(cfg: setups.WindConfig) -> WindDesc:
     d.shock_flattening = int(cfg.shock_flattening)
     d.char_limiting = int(cfg.char_limiting)
     d.entropy_switch = int(cfg.entropy_switch)
+    d.solver = int(cfg.solver)
     return d
 
 

@@ -22,7 +22,7 @@ RHO, VX1, VX2, VX3, PRS = range(5)
 
 PLM, PPM = 0, 1
 LIM_DEFAULT, LIM_VANLEER, LIM_MC, LIM_MINMOD, LIM_FLAT = range(5)
-HLLC, ROE = 0, 1
+HLLC, ROE, HLL = 0, 1, 2   # HLL: the wind only (the production problem's [Solver])
 BC_PERIODIC, BC_OUTFLOW, BC_REFLECTIVE = range(3)
 # the production wind problem's userdef sides (cv_idl/init.c): X1-beg outflow
 # with v_r <= 0, X2-beg reflective for velocities and outflow for rho and p
@@ -30,7 +30,7 @@ BC_WIND_INNER, BC_WIND_POLE = 3, 4
 
 _BC_NAMES = {BC_PERIODIC: "periodic", BC_OUTFLOW: "outflow",
              BC_REFLECTIVE: "reflective", BC_WIND_INNER: "userdef", BC_WIND_POLE: "userdef"}
-_SOLVER_NAMES = {HLLC: "hllc", ROE: "roe"}
+_SOLVER_NAMES = {HLLC: "hllc", ROE: "roe", HLL: "hll"}
 
 
 @dataclasses.dataclass

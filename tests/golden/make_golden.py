@@ -94,6 +94,10 @@ def cvidl_cases():
     cfg = dataclasses.replace(cfg, entropy_switch=True)
     yield ("wind_cvidl_prod_mixed_32x16", cfg,
            setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7)), 8)
+    # and its [Solver] hll
+    cfg = dataclasses.replace(cfg, solver=setups.HLL)
+    yield ("wind_cvidl_prod_hll_mixed_32x16", cfg,
+           setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7)), 8)
 
 
 def wind_inputs(cfg):

@@ -160,6 +160,20 @@ def test_wind_production_reconstruction_matches_oracle(pmx):
     np.testing.assert_allclose(dts, o["dt_seq"], rtol=RTOL, atol=0)
 
 
+def test_wind_production_setup_with_hll_matches_oracle(pmx):
+    """The production problem's set-up with its HLL solver, 128 x 64,
+    6 steps."""
+    import dataclasses
+    cfg = dataclasses.replace(setups.cvidl_wind(128, 64, dfloor=1.e-15), first_dt=1.e-3,
+                              shock_flattening=True, char_limiting=True, entropy_switch=True,
+                              solver=setups.HLL)
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
+    o = refrun.run_wind_oracle(cfg, V0, 6)
+    V, U, dts, tm = gpu_run(pmx, cfg, V0, 6)
+    assert_close(V, o["V"], "V")
+    assert_close(U, o["U"], "U")
+
+
 def test_wind_entropy_switch_without_char_limiting_matches_oracle(pmx):
     """The entropy switch with van Leer / minmod scalars (no characteristic
     limiting) and shock flattening, 128 x 64, 6 steps."""

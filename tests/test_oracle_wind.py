@@ -53,6 +53,7 @@ COVERAGE = {
     "wind_cvidl_flat_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
     "wind_cvidl_char_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
     "wind_cvidl_prod_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
+    "wind_cvidl_prod_hll_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
 }
 
 
@@ -122,6 +123,19 @@ def test_entropy_switch_fixture_pins_the_entropy_path():
             {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
     fl = (g["flux_r"], g["flux_t"], g["flux_p"])
     r = refrun.run_wind_oracle(dataclasses.replace(cfg, entropy_switch=False), g["V0"],
+                               int(g["nsteps"]), fluxes=fl, grid=grid)
+    assert not np.array_equal(r["V"], g["V"])
+
+
+def test_hll_fixture_pins_the_solver():
+    """The production problem's HLL solver: HLLC instead changes the
+    fixture's result."""
+    import dataclasses
+    cfg, g = golden_io.load("wind_cvidl_prod_hll_mixed_32x16")
+    grid = ({"x": g["x1"], "xl": g["x1l"], "xr": g["x1r"], "dx": g["dx1"]},
+            {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
+    fl = (g["flux_r"], g["flux_t"], g["flux_p"])
+    r = refrun.run_wind_oracle(dataclasses.replace(cfg, solver=setups.HLLC), g["V0"],
                                int(g["nsteps"]), fluxes=fl, grid=grid)
     assert not np.array_equal(r["V"], g["V"])
 
