@@ -19,6 +19,7 @@ Also reported:
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import pathlib
@@ -105,6 +106,32 @@ def wind_legs(steps: int = 200) -> dict:
     plain, Tp = run_wind()
     out["c3_plain"] = dict(plain, workload=f"wind 512x256, PLM+HLLC+RK2, line force, plain "
                                             f"cooling, {steps} steps")
+    # the same wind with the production problem's set-up (cv_idl boundaries
+    # and floors, MULTID shock flattening, characteristic limiting, entropy
+    # switch, HLL), 20 steps: from this start state the set-up reaches a zone
+    # whose Cash-Karp integration exceeds its sub-step limit near step 90
+    # (the oracle stops there too), after which dt collapses
+    prod_cfg = dataclasses.replace(setups.cvidl_wind(512, 256, dfloor=1.e-20),
+                                   shock_flattening=True, char_limiting=True,
+                                   entropy_switch=True, solver=setups.HLL)
+    with plutomx.Wind(prod_cfg) as w:
+        w.set_prim(V0)
+        w.set_time(0.0, prod_cfg.first_dt, 0)
+        w.run(2)
+        w.sync()
+        w.timing(True, reset=True)
+        psteps = min(steps, 20)
+        t0 = time.perf_counter()
+        w.run(psteps)
+        w.sync()
+        el = time.perf_counter() - t0
+        tm = w.timing(False)
+    out["c3_production_setup"] = {
+        "Mcell_updates_per_s": round(prod_cfg.ncells * psteps / el / 1e6, 2),
+        "ms_per_step": round(1e3 * el / psteps, 4),
+        "stage_ms_per_launch": round(tm["stage_ms"] / max(tm["stage_launches"], 1), 4),
+        "workload": "C3 with cv_idl boundaries and floors, SHOCK_FLATTENING MULTID, "
+                    f"CHAR_LIMITING, ENTROPY_SWITCH ALWAYS, HLL, {psteps} steps"}
     m = plutomx.Mlp(seed=11)
     t0 = time.perf_counter()
     m.sample(cfg, 2_000_000, seed=5)

@@ -55,7 +55,15 @@ typedef struct {
   double cfl;            /* runtime->cfl */
   double cfl_max_var;    /* runtime->cfl_max_var */
   const double *dx1, *dx2, *dx3;   /* host arrays, nx1 / nx2 / nx3 entries */
+  double first_dt;       /* runtime->first_dt: > 0 enables NextTimeStep's "dt is too
+                            small" stop (dt < 1e-9 first_dt, Src/main.c:672-683),
+                            reported as PMX_FLAG_DT_TOO_SMALL */
 } pmx_hydro_desc;
+
+/* Set in the flags that *_get_time return, and kept, once NextTimeStep has
+ * produced a dt below 1e-9 first_dt: the point where the reference prints
+ * "dt is too small" and quits.  The state after it is not meaningful. */
+#define PMX_FLAG_DT_TOO_SMALL 4096
 
 const char *pmx_last_error(void);
 
@@ -168,6 +176,7 @@ typedef struct {
    * cons->prim takes the pressure from it (Src/HD/mappers.c:186-204). */
   int entropy_switch;
   int solver;                   /* PMX_SOLVER_HLLC (0) or PMX_SOLVER_HLL */
+  double first_dt;              /* as in pmx_hydro_desc (PMX_FLAG_DT_TOO_SMALL) */
 } pmx_wind_desc;
 
 /* Create a context: uploads geometry, PLM weights and the line-force

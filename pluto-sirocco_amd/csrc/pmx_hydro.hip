@@ -33,7 +33,7 @@ struct DevScalars {
   unsigned long long cdt_bits;   // max C_dt of stage 1
   double inv_dt_hyp;             // Dts->invDt_hyp of the last step
   int err;                       // cons->prim flags of the last step
-  int pad;
+  int halt;                      // PMX_FLAG_DT_TOO_SMALL once set (sticky)
 };
 
 // dt/dx for every cell and direction, and reset of the step reductions.
@@ -51,7 +51,8 @@ __global__ void prep_kernel(DevScalars* s, const double* dx, double* dtdx, int n
 
 // Src/update_stage.c:389-394 then Src/main.c:596-697 (NextTimeStep, no
 // parabolic terms, no cooling) and the time update of main.c:296.
-__global__ void finish_kernel(DevScalars* s, int ndim, double cfl, double cfl_max_var) {
+__global__ void finish_kernel(DevScalars* s, int ndim, double cfl, double cfl_max_var,
+                              double first_dt) {
   double inv = 0.0;
   double c = __longlong_as_double((long long)s->cdt_bits);
   inv = dmax(inv, c);
@@ -63,6 +64,7 @@ __global__ void finish_kernel(DevScalars* s, int ndim, double cfl, double cfl_ma
   dt_hyp *= cfl;
   double dtnext = dt_hyp;
   dtnext = dmin(dtnext, cfl_max_var * dt);
+  if (dtnext < first_dt * 1.e-9) s->halt = PMX_FLAG_DT_TOO_SMALL;   // main.c:672-683
   if (cfl_max_var != 1.0) s->dt = dtnext;
 }
 
@@ -220,7 +222,7 @@ int advance(pmx_hydro* h, const double* dt_override) {
     if (stage(h, h->V[1], h->V[0], h->V[0], MODE_WEIGHTED, 0.5, 0.5, 1)) return -1;
   }
   hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, h->stream, h->s, h->d.dims,
-                     h->d.cfl, h->d.cfl_max_var);
+                     h->d.cfl, h->d.cfl_max_var, h->d.first_dt);
   PMX_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -441,7 +443,7 @@ int pmx_hydro_get_time(pmx_hydro* h, double* t, double* dt, double* inv_dt_hyp, 
   if (t) *t = s0.t;
   if (dt) *dt = s0.dt;
   if (inv_dt_hyp) *inv_dt_hyp = s0.inv_dt_hyp;
-  if (flags) *flags = s0.err;
+  if (flags) *flags = s0.err | s0.halt;
   return 0;
 }
 

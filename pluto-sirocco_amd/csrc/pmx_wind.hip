@@ -642,7 +642,8 @@ __global__ __launch_bounds__(128) void wind_cool_kernel(CoolArgs a, unsigned lon
 // the invDt / dt_cool reset that the next step performs first
 // (main.c:406-415).  With cooling both happen every second step
 // (main.c:329-333), without every step: `next` says whether this is one.
-__global__ void wind_end_kernel(Scalars* s, int next, double cfl, double cfl_max_var) {
+__global__ void wind_end_kernel(Scalars* s, int next, double cfl, double cfl_max_var,
+                                double first_dt) {
   double dt = s->dt;
   s->t += dt;
   double dt_cool = __longlong_as_double((long long)s->dtcool_bits);
@@ -653,6 +654,7 @@ __global__ void wind_end_kernel(Scalars* s, int next, double cfl, double cfl_max
     double dtn = dt_hyp;
     dtn = dmin(dtn, dt_cool);
     dtn = dmin(dtn, cfl_max_var * dt);
+    if (dtn < first_dt * 1.e-9) s->halt = PMX_FLAG_DT_TOO_SMALL;   // main.c:672-683
     if (cfl_max_var != 1.0) s->dt = dtn;
     s->inv_dt_hyp = 0.0;
     s->dtcool_bits = (unsigned long long)__double_as_longlong(1.e38);
@@ -889,6 +891,7 @@ Const make_const(const pmx_wind_desc* d) {
   k.char_lim = d->char_limiting != 0;
   k.ent = d->entropy_switch != 0;
   k.hll = d->solver == PMX_SOLVER_HLL;
+  k.first_dt = d->first_dt;
   return k;
 }
 }  // namespace wind
@@ -1173,7 +1176,7 @@ int pmx_wind_get_time(pmx_wind* w, double* t, double* dt, long* step, double* in
     memcpy(&c, &s0.dtcool_bits, sizeof(double));
     *dt_cool = c;
   }
-  if (flags) *flags = s0.err;
+  if (flags) *flags = s0.err | s0.halt;
   return 0;
 }
 
@@ -1198,7 +1201,7 @@ int pmx_wind_run(pmx_wind* w, int nsteps) {
     }
     const int next = (w->cooling == PMX_COOL_NONE) || (st % 2 == 1);
     hipLaunchKernelGGL(wind_end_kernel, dim3(1), dim3(1), 0, w->stream, w->s, next, w->cfl,
-                       w->cfl_max_var);
+                       w->cfl_max_var, w->k.first_dt);
     PMX_HIP_CHECK(hipGetLastError());
     w->step++;
   }
@@ -1319,6 +1322,7 @@ __global__ void wind_batch_end_kernel(const StageArgs* args, int G, int next, co
     double dtn = dt_hyp;
     dtn = dmin(dtn, dt_cool);
     dtn = dmin(dtn, cfl_max_var[g] * dt);
+    if (dtn < args[g].k.first_dt * 1.e-9) s->halt = PMX_FLAG_DT_TOO_SMALL;
     if (cfl_max_var[g] != 1.0) s->dt = dtn;
     s->inv_dt_hyp = 0.0;
     s->dtcool_bits = (unsigned long long)__double_as_longlong(1.e38);

@@ -14,7 +14,7 @@ struct Scalars {
   unsigned long long cdt_bits;     // max C_dt of the current step's stage 1
   unsigned long long dtcool_bits;  // running min of the cooling dt candidates
   int err;
-  int pad;
+  int halt;                        // PMX_FLAG_DT_TOO_SMALL once set (sticky)
 };
 
 // Arguments of the cooling kernels.

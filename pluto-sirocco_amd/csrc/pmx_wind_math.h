@@ -52,6 +52,7 @@ struct Const {
   int char_lim;                            // CHAR_LIMITING YES
   int ent;                                 // ENTROPY_SWITCH ALWAYS
   int hll;                                 // [Solver] hll instead of hllc
+  double first_dt;                         // runtime->first_dt (dt-too-small stop)
 };
 
 // ---------------------------------------------------------------------

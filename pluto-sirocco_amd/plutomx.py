@@ -51,6 +51,9 @@ class PmxError(RuntimeError):
     pass
 
 
+PMX_FLAG_DT_TOO_SMALL = 4096   # NextTimeStep stopped: dt < 1e-9 first_dt
+
+
 class HydroDesc(ctypes.Structure):
     _fields_ = [("dims", ctypes.c_int), ("nx1", ctypes.c_int), ("nx2", ctypes.c_int),
                 ("nx3", ctypes.c_int), ("recon", ctypes.c_int), ("solver", ctypes.c_int),
@@ -58,7 +61,8 @@ class HydroDesc(ctypes.Structure):
                 ("cfl", ctypes.c_double), ("cfl_max_var", ctypes.c_double),
                 ("dx1", ctypes.POINTER(ctypes.c_double)),
                 ("dx2", ctypes.POINTER(ctypes.c_double)),
-                ("dx3", ctypes.POINTER(ctypes.c_double))]
+                ("dx3", ctypes.POINTER(ctypes.c_double)),
+                ("first_dt", ctypes.c_double)]
 
 
 class WindDesc(ctypes.Structure):
@@ -78,7 +82,7 @@ class WindDesc(ctypes.Structure):
                 ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp),
                 ("dfloor", ctypes.c_double), ("shock_flattening", ctypes.c_int),
                 ("char_limiting", ctypes.c_int), ("entropy_switch", ctypes.c_int),
-                ("solver", ctypes.c_int)]
+                ("solver", ctypes.c_int), ("first_dt", ctypes.c_double)]
 
 
 _lib = None
@@ -203,6 +207,7 @@ class Hydro:
         for s in range(6):
             d.bc[s] = cfg.bc[s]
         d.gamma, d.cfl, d.cfl_max_var = cfg.gamma, cfg.cfl, cfg.cfl_max_var
+        d.first_dt = cfg.first_dt
         self._dx = [cell_widths(cfg, q) for q in range(3)]
         d.dx1, d.dx2, d.dx3 = (_dptr(x) for x in self._dx)
         h = ctypes.c_void_p()
@@ -297,6 +302,7 @@ def wind_desc_scalars(cfg: setups.WindConfig) -> WindDesc:
     d.char_limiting = int(cfg.char_limiting)
     d.entropy_switch = int(cfg.entropy_switch)
     d.solver = int(cfg.solver)
+    d.first_dt = cfg.first_dt
     return d
 
 

@@ -202,3 +202,23 @@ def test_full_size_periodic_conservation(sedov256):
     for nv in (1, 2, 3):
         tot = np.abs(U0[0]).sum()
         assert abs(r["U"][nv].sum() - U0[nv].sum()) <= 1e-12 * tot
+
+
+def test_dt_too_small_stop_is_reported():
+    """NextTimeStep's stop (dt < 1e-9 first_dt, Src/main.c:672-683) sets
+    PMX_FLAG_DT_TOO_SMALL, which stays set; a normal run leaves it clear."""
+    import dataclasses
+    base = setups.c1_kh(32)
+    V0 = setups.kelvin_helmholtz(base, seed=3)
+    ok = run_gpu(base, V0, 2)
+    assert not ok["flags"] & plutomx.PMX_FLAG_DT_TOO_SMALL
+    cfg = dataclasses.replace(base, first_dt=1.e9)     # any real dt is "too small"
+    with plutomx.Hydro(cfg) as h:
+        h.set_prim(V0)
+        h.set_time(0.0, base.first_dt)
+        h.run(2)
+        h.sync()
+        assert h.get_time()[3] & plutomx.PMX_FLAG_DT_TOO_SMALL
+        h.run(1)
+        h.sync()
+        assert h.get_time()[3] & plutomx.PMX_FLAG_DT_TOO_SMALL

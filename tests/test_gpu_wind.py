@@ -187,6 +187,21 @@ def test_wind_entropy_switch_without_char_limiting_matches_oracle(pmx):
     assert_close(U, o["U"], "U")
 
 
+def test_wind_dt_too_small_stop_is_reported(pmx):
+    """NextTimeStep's stop (dt < 1e-9 first_dt) sets PMX_FLAG_DT_TOO_SMALL
+    for the wind too; a normal run leaves it clear."""
+    import dataclasses
+    base = setups.WindConfig(nx=(32, 16), cooling=setups.COOL_PLAIN, first_dt=1.e-3)
+    V0 = setups.wind_ic(base)
+    for cfg, expect in ((base, False), (dataclasses.replace(base, first_dt=1.e9), True)):
+        with pmx.Wind(cfg) as w:
+            w.set_prim(V0)
+            w.set_time(0.0, base.first_dt, 0)
+            w.run(4)
+            w.sync()
+            assert bool(w.get_time()["flags"] & pmx.PMX_FLAG_DT_TOO_SMALL) == expect
+
+
 @pytest.mark.parametrize("cooling", [setups.COOL_BLONDIN, setups.COOL_PLAIN])
 @pytest.mark.parametrize("t0", [0.0, 5.0])
 def test_wind_single_cooling_call(pmx, cooling, t0):
