@@ -177,6 +177,16 @@ typedef struct {
   int entropy_switch;
   int solver;                   /* PMX_SOLVER_HLLC (0) or PMX_SOLVER_HLL */
   double first_dt;              /* as in pmx_hydro_desc (PMX_FLAG_DT_TOO_SMALL) */
+  /* krad == alpharad == 999: the tabulated force multiplier of
+   * M_UV_data.dat (line_connect.c:183-257, :858-887).  Per angle bin,
+   * M = linterp(log10 t) over the cell's fit, 10^ of the linear
+   * interpolation in log10 M between the two fit points around log10 t
+   * (the end value outside the fit range), capped at 4400.  fm_t [fm_points]
+   * and fm_m [fm_points][nx2][nx1] are t and M as the file holds them
+   * (pmx_sirocco_read_force_multiplier); their log10 is taken once at
+   * create.  Ignored for other krad / alpharad. */
+  int fm_points;                /* 1 .. 4096 */
+  const double *fm_t, *fm_m;
 } pmx_wind_desc;
 
 /* Create a context: uploads geometry, PLM weights and the line-force
@@ -196,6 +206,18 @@ int pmx_wind_get_cons(pmx_wind *w, double *U);
  * does at start-up (line_connect.c:291-306): xi = L_x / n_H / r^2,
  * t_r = T_x. */
 int pmx_wind_set_heatcool(pmx_wind *w, const double *xi, const double *t_r);
+
+/* The heating / cooling prefactors per cell, pre[6][nx2][nx1] in the
+ * column order of prefactors.dat: comp_h, comp_c, xray_h, brem_c, line_c,
+ * xi_ion (pmx_sirocco_read_prefactors).  CoolingSource passes the first
+ * five to Radiat (cv_idl/cooling_source.c:101-105, cv_idl/radiat.c:118-123)
+ * and BLONDIN cooling multiplies its rates by them
+ * (Src/Cooling/BLONDIN/cooling.c:74-79, :190-195); xi_ion is read but the
+ * reference uses it nowhere.  pre == NULL: all 1.0, what
+ * read_sirocco_heatcool sets at start-up (line_connect.c:366-380).  Cells
+ * with a prefactor other than 1 are cooled by the plain integrator also
+ * when a surrogate is attached (the network is trained on unit prefactors). */
+int pmx_wind_set_prefactors(pmx_wind *w, const double *pre);
 
 /* g_time, g_dt and the step counter g_stepNumber of the next step. */
 int pmx_wind_set_time(pmx_wind *w, double t, double dt, long step);
@@ -228,6 +250,56 @@ int pmx_wind_sync(pmx_wind *w);
  * cooling kernels while enabled: summed milliseconds and launch counts. */
 int pmx_wind_timing(pmx_wind *w, int enable, int reset, double *stage_ms, int *stage_launches,
                     double *cool_ms, int *cool_launches);
+
+/* ------------------------------------------------------------------ */
+/* SIROCCO file interface (Src/LineDriven/line_connect.c:43-497): host-  */
+/* side readers of the files a SIROCCO cycle writes.  Records are matched */
+/* to cells of the grid of `grid` (nx1, nx2, x1, x2 with ghosts,          */
+/* unit_length) by the reference's relative tolerances; every matching   */
+/* cell takes the record.  Outputs are interior arrays [..][nx2][nx1].    */
+/* A missing file is not an error (the reference prints a note and goes  */
+/* on): *found is 0 and the outputs hold what the reference leaves.  A    */
+/* malformed record is an error (the reference exits).  `dir` is the      */
+/* directory of the files (NULL: the working directory).  *matched counts */
+/* the records that matched a cell.  No device work.                     */
+/* ------------------------------------------------------------------ */
+/* Angle bins of directional_flux_r.dat: the third word of its second
+ * line (line_connect.c:84-94).  *nangles = 0 when the file is missing. */
+int pmx_sirocco_flux_angles(const char *dir, int *nangles);
+/* read_sirocco_fluxes(), flux part (line_connect.c:68-165):
+ * directional_flux_{r,theta,phi}.dat, records "i j k r[cm] theta f_0 ..
+ * f_{nangles-1}", r matched within 1e-6 of x1 * unit_length and theta
+ * within 1e-6 of x2 (relative).  Cells no record names keep the caller's
+ * values (the reference's arrays are uninitialised there).  *found: bit a
+ * set when the file of axis a (r, theta, phi) was read; matched[3]. */
+int pmx_sirocco_read_fluxes(const char *dir, const pmx_wind_desc *grid, int nangles,
+                            double *flux_r, double *flux_t, double *flux_p, int *found,
+                            long *matched);
+/* Fit points of M_UV_data.dat: the second word of its first line
+ * (line_connect.c:193-198); 0 when the file is missing. */
+int pmx_sirocco_fm_points(const char *dir, int *points);
+/* The force multiplier fits (line_connect.c:183-257): t[points] from the
+ * second line (after one word), records "i j r[cm] theta M_0 .. M_{points-1}"
+ * matched as the fluxes.  Values as in the file (pmx_wind_create takes the
+ * log10).  Unmatched cells keep the caller's values. */
+int pmx_sirocco_read_force_multiplier(const char *dir, const pmx_wind_desc *grid, int points,
+                                      double *t, double *m, int *found, long *matched);
+/* read_sirocco_heatcool(flag = 1), py_heatcool.dat part
+ * (line_connect.c:308-363): 18-word records "i j r[cm] theta vol t_e t_r
+ * xi ne ..."; r and theta matched within 1e-5 (relative to the record's
+ * values); xi floored at 1 and t_r at 1e3 K.  Without the file, every cell
+ * gets the start-up values from rho (code units, [nx2][nx1]): xi = L_x /
+ * n_H / r^2, t_r = T_x.  Unmatched cells keep the caller's values. */
+int pmx_sirocco_read_heatcool(const char *dir, const pmx_wind_desc *grid, const double *rho,
+                              double *xi, double *t_r, int *found, long *matched);
+/* read_sirocco_heatcool(flag = 1), prefactors.dat part
+ * (line_connect.c:365-497): 11-word records "i r[cm] j theta rho comp_h
+ * comp_c xray_h brem_c line_c xi_ion" matched as py_heatcool; pre[6][nx2][nx1]
+ * in that column order, 1.0 for every cell first (and everywhere without
+ * the file).  *negative counts cell prefactors below 0 (the reference
+ * prints each). */
+int pmx_sirocco_read_prefactors(const char *dir, const pmx_wind_desc *grid, double *pre,
+                                int *found, long *matched, long *negative);
 
 /* ------------------------------------------------------------------ */
 /* Batched sweep (no counterpart in the reference, which advances one    */

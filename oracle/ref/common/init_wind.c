@@ -16,6 +16,13 @@
  *   and appends (step, time, dt) to PMX_DUMP".dt".
  *   With PMX_TIME_ONLY set, nothing is written and only the wall time
  *   between the first and the last Analysis() call is printed to stdout.
+ * - A SIROCCO coupling cycle restarts PLUTO with the files SIROCCO wrote.
+ *   Two variables give a fresh start the state such a restart has: PMX_T0
+ *   sets g_time, as RestartFromFile does (Src/restart.c:259), and with
+ *   PMX_SIROCCO_RESTART set read_sirocco_heatcool() is called with
+ *   restart_flag = 1 (Src/main.c:197), so the reference's own reader loads
+ *   py_heatcool.dat and prefactors.dat.  The call reaches it through the
+ *   linker's --wrap=read_sirocco_heatcool; the reader is compiled as it lies.
  */
 #define _POSIX_C_SOURCE 199309L
 #include <time.h>
@@ -74,6 +81,14 @@ void InitDomain (Data *d, Grid *grid)
   }
   free(buf);
   fclose(fp);
+  if (getenv("PMX_T0") != NULL) g_time = atof(getenv("PMX_T0"));
+}
+
+void __real_read_sirocco_heatcool(Data *d, Grid *grid, int flag);
+
+void __wrap_read_sirocco_heatcool(Data *d, Grid *grid, int flag)
+{
+  __real_read_sirocco_heatcool(d, grid, getenv("PMX_SIROCCO_RESTART") != NULL ? 1 : flag);
 }
 
 void Analysis (const Data *d, Grid *grid)

@@ -65,12 +65,19 @@ def read_dump(path: pathlib.Path, shape) -> dict:
 
 
 def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
-                  timeout: float = 3600.0, fluxes=None) -> dict:
+                  timeout: float = 3600.0, fluxes=None, sirocco=None,
+                  t0: float | None = None, files_dir=None) -> dict:
     """Run the reference for ``nsteps`` (>= 2) steps from ``V0``.
 
     PLUTO's ``-maxsteps N`` integrates N+1 steps (Src/main.c:233-240), so
     N = nsteps - 1 is passed.  Returns V, U (Uc), t, next dt, the dt
-    sequence, and (time_only) the wall time of steps 2..nsteps."""
+    sequence, and (time_only) the wall time of steps 2..nsteps.
+
+    Wind only: ``sirocco`` (setups.SiroccoInputs) writes the files of a
+    SIROCCO cycle; with heat-cool or prefactor data the run reads them as a
+    restart does (restart_flag = 1).  ``t0`` starts the clock there, as a
+    restart from a file at that time would.  ``files_dir``: take the
+    SIROCCO files (*.dat) from there instead, as a restart."""
     if nsteps < 2:
         raise ValueError("the reference driver needs nsteps >= 2")
     binary = reference_binary(cfg.reference_problem())
@@ -84,13 +91,25 @@ def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
             # the build carries Vc[ENTR] too; Boundary() sets it from rho, p
             ic = np.concatenate([ic, np.zeros_like(ic[:1])])
         ic.tofile(tmp / "ic.bin")
-        if hasattr(cfg, "nangles"):  # line-driven wind: SIROCCO flux files
+        env = dict(os.environ)
+        for k in ("PMX_TIME_ONLY", "PMX_T0", "PMX_SIROCCO_RESTART"):
+            env.pop(k, None)
+        if files_dir is not None:
+            import shutil
+            for f in pathlib.Path(files_dir).glob("*.dat"):
+                shutil.copy(f, tmp / f.name)
+            env["PMX_SIROCCO_RESTART"] = "1"
+        elif hasattr(cfg, "nangles"):  # line-driven wind: SIROCCO flux files
             import setups
             setups.write_flux_files(cfg, tmp, fluxes)
-        env = dict(os.environ)
+            if sirocco is not None:
+                setups.write_sirocco_files(cfg, tmp, sirocco, V0)
+                if sirocco.xi is not None or sirocco.pre is not None:
+                    env["PMX_SIROCCO_RESTART"] = "1"
+        if t0 is not None:
+            env["PMX_T0"] = repr(float(t0))
         env["PMX_IC"] = str(tmp / "ic.bin")
         env["PMX_DUMP"] = str(tmp / "dump.bin")
-        env.pop("PMX_TIME_ONLY", None)
         if time_only:
             env["PMX_TIME_ONLY"] = "1"
         proc = subprocess.run([str(binary), "-i", "pluto.ini", "-maxsteps",
@@ -199,11 +218,14 @@ class _OrcWindCfg(ctypes.Structure):
                 ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp),
                 ("dfloor", ctypes.c_double), ("shock_flattening", ctypes.c_int),
                 ("char_limiting", ctypes.c_int), ("entropy_switch", ctypes.c_int),
-                ("solver", ctypes.c_int)]
+                ("solver", ctypes.c_int),
+                ("xi", _dp), ("t_r", _dp), ("pre", _dp),
+                ("fm_points", ctypes.c_int), ("fm_t", _dp), ("fm_m", _dp)]
 
 
-def _wind_struct(cfg, fluxes=None, grid=None):
-    """The orc_wind_cfg of a WindConfig; returns (struct, arrays to keep)."""
+def _wind_struct(cfg, fluxes=None, grid=None, sirocco=None):
+    """The orc_wind_cfg of a WindConfig (and of the SIROCCO restart inputs
+    ``sirocco``, a setups.SiroccoInputs); returns (struct, arrays to keep)."""
     import setups
     L = lib()
     if not hasattr(L, "_wind_bound"):
@@ -243,16 +265,26 @@ def _wind_struct(cfg, fluxes=None, grid=None):
     c.char_limiting = int(cfg.char_limiting)
     c.entropy_switch = int(cfg.entropy_switch)
     c.solver = int(cfg.solver)
+    if sirocco is not None:
+        if sirocco.xi is not None:
+            xi, t_r = sirocco.heatcool_floored()
+            c.xi, c.t_r = arr(xi), arr(t_r)
+        if sirocco.pre is not None:
+            c.pre = arr(sirocco.pre[:5])
+        if sirocco.fm_t is not None:
+            c.fm_points = len(sirocco.fm_t)
+            c.fm_t, c.fm_m = arr(sirocco.fm_t), arr(sirocco.fm_m)
     return c, keep
 
 
 def run_wind_oracle(cfg, V0: np.ndarray, nsteps: int, dt0: float | None = None,
-                    t0: float = 0.0, fluxes=None, grid=None) -> dict:
+                    t0: float = 0.0, fluxes=None, grid=None, sirocco=None) -> dict:
     """Run the wind restatement (orc_wind_run) for nsteps steps from step
     number 0.  ``fluxes`` = (flux_r, flux_t, flux_p) [nangles][1][nt][nr]
     (default: setups.wind_fluxes); ``grid`` = (radial, polar) dicts with
-    x, xl, xr, dx including ghosts (default: setups.wind_grid)."""
-    c, keep = _wind_struct(cfg, fluxes, grid)
+    x, xl, xr, dx including ghosts (default: setups.wind_grid);
+    ``sirocco``: setups.SiroccoInputs of a restarted cycle."""
+    c, keep = _wind_struct(cfg, fluxes, grid, sirocco)
     V = np.ascontiguousarray(V0, dtype=np.float64).copy()
     U = np.empty_like(V)
     dt = np.array([cfg.first_dt if dt0 is None else dt0])

@@ -83,6 +83,18 @@ typedef struct {
   int    entropy_switch;      /* 1: ENTROPY_SWITCH ALWAYS (pressure from the
                                  advected entropy, HD/mappers.c:186-204)      */
   int    solver;              /* 0: HLLC, 2: HLL (Src/HD/hll.c)               */
+  /* What read_sirocco_heatcool(flag = 1) leaves on a restart
+   * (line_connect.c:308-497); NULL = its start-up values:                   */
+  const double *xi, *t_r;     /* [nx2][nx1] sirocco_xi, sirocco_t_r (read by
+                                 the cooling once g_time > 3)                */
+  const double *pre;          /* [5][nx2][nx1] comp_h, comp_c, xray_h, brem_c,
+                                 line_c prefactors (NULL: all 1.0)           */
+  /* krad = alpharad = 999: the tabulated force multiplier
+   * (line_connect.c:183-257, :858-887)                                      */
+  int    fm_points;
+  const double *fm_t;         /* [fm_points] t of the fits, as in the file   */
+  const double *fm_m;         /* [fm_points][nx2][nx1] M per cell, as in the
+                                 file (the reader keeps log10 of both)       */
 } orc_wind_cfg;
 
 /* Advance V ([6][nx2][nx1], in/out) by nsteps steps of the reference's

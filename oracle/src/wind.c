@@ -10,10 +10,13 @@
  *                     LineForce)
  *   vgrad_setup/vgrad Src/LineDriven/line_connect.c:504-742 (VGradCalc),
  *                     :746-777 (bilinear)
- *   line_force()      Src/LineDriven/line_connect.c:815-903 (k, alpha mode)
+ *   line_force()      Src/LineDriven/line_connect.c:815-903 (k, alpha mode;
+ *                     tabulated M(t) with linterp(), :781-811, for
+ *                     krad = alpharad = 999)
  *   blondin()         Src/Cooling/BLONDIN/cooling.c:50-319
  *   cooling_plain()   Test_Problems/LineDrivenWind/cv_idl/cooling_source.c
- *                     (CoolingSource) with cv_idl/radiat.c (Radiat) and
+ *                     (CoolingSource, per-cell prefactors :101-105) with
+ *                     cv_idl/radiat.c (Radiat, :118-123) and
  *                     Src/Cooling/cooling_ode_solver.c (RKF12, CK45)
  *   run()             Src/main.c:215-337, :383-485 (Strang order, cooling
  *                     dt every second step), :521-697
@@ -91,6 +94,9 @@ typedef struct {
   /* ENTROPY_SWITCH: Vc[ENTR] = p / rho^gamma and Uc[ENTR] = rho Vc[ENTR]
    * (pluto.h:737-746), kept beside V and U, [j][i] */
   double *S, *Ue, *U0e;
+  /* tabulated force multiplier: log10 t [fm_points], log10 M [p][j][i]
+     (interior layout), as read_sirocco_fluxes keeps them */
+  double *fm_lt, *fm_lm;
 } W;
 
 enum { FLAG_MINMOD = 1, FLAG_HLL = 4 };   /* Src/pluto.h:212-214 */
@@ -98,6 +104,15 @@ enum { FLAG_MINMOD = 1, FLAG_HLL = 4 };   /* Src/pluto.h:212-214 */
 #define VI(w, nv, j, i) (((size_t)(nv)*(w)->t2 + (j))*(w)->t1 + (i))
 #define CI(w, j, i) ((size_t)(j)*(w)->t1 + (i))
 #define AI(w, a, j, i) (((size_t)(a)*(w)->t2 + (j))*(w)->t1 + (i))
+
+/* the five prefactors of cell (j, i) in the order comp_h, comp_c, xray_h,
+   brem_c, line_c (cv_idl/cooling_source.c:101-105) */
+static void prefactors(const W *w, int j, int i, double *pre)
+{
+  int p;
+  for (p = 0; p < 5; p++)
+    pre[p] = w->c->pre ? w->c->pre[((size_t)p*w->n2 + (j - NG))*w->n1 + (i - NG)] : 1.0;
+}
 
 /* ------------------------------------------------------------------ */
 static void p2c(double gamma, const double *v, double *u)
@@ -580,7 +595,31 @@ static void vgrad(W *w)
 #undef P2
 }
 
-/* LineForce(v, grad, ...) for cell (j, i): k, alpha mode */
+/* linterp(), line_connect.c:781-811: the first fit point not below x by
+   a linear scan, linear interpolation in log10 M, 10^ of the result; the
+   end values outside the fit range; 0 for a NaN result. */
+static double linterp(double x, const double *xa, const double *ya, size_t ystride, int n)
+{
+  int idx = 0;
+  double result;
+  while (idx < n && xa[idx] < x) idx++;
+  if (idx == 0) {
+    result = pow(10.0, ya[0]);
+  } else if (idx >= n) {
+    result = pow(10.0, ya[(size_t)(n - 1)*ystride]);
+  } else {
+    double x_low = xa[idx - 1], x_high = xa[idx];
+    double y_low = ya[(size_t)(idx - 1)*ystride], y_high = ya[(size_t)idx*ystride];
+    double slope = (y_high - y_low)/(x_high - x_low);
+    result = pow(10.0, y_low + slope*(x - x_low));
+  }
+  if (isnan(result)) result = 0.0;
+  return result;
+}
+
+/* LineForce(v, grad, ...) for cell (j, i), line_connect.c:815-903: the
+   k, alpha mode, or the tabulated M(t) of the cell for krad = alpharad =
+   999 */
 static void line_force(W *w, const double *v, double *grad, int j, int i)
 {
   const orc_wind_cfg *c = w->c;
@@ -588,7 +627,8 @@ static void line_force(W *w, const double *v, double *grad, int j, int i)
   double rho = v[RHO]*w->UD;
   double krad = c->krad, alpharad = c->alpharad, M_max = 4400.;
   double T, v_th;
-  int a;
+  int a, tab = (krad == 999 && alpharad == 999);
+  const double *mcol = tab ? w->fm_lm + (size_t)(j - NG)*w->n1 + (i - NG) : NULL;
   T = v[PRS]/v[RHO]*w->KELVIN*c->mu;
   v_th = sqrt((2.0*C_kB*T)/C_mp);
   grad[0] = grad[1] = grad[2] = 0.0;
@@ -597,7 +637,12 @@ static void line_force(W *w, const double *v, double *grad, int j, int i)
     double flux_r = w->fr[q], flux_t = w->ft[q], M;
     if (w->dvds[q] > 0.0) {
       double t_UV = sigma_e*rho*v_th/w->dvds[q];
-      M = krad*pow(t_UV, alpharad);
+      if (tab) {
+        double log10_t_UV = log10(t_UV);
+        M = linterp(log10_t_UV, w->fm_lt, mcol, (size_t)w->n1*w->n2, c->fm_points);
+      } else {
+        M = krad*pow(t_UV, alpharad);
+      }
       if (M > M_max) M = M_max;
     } else {
       M = 0.0;
@@ -925,7 +970,10 @@ static void blondin(W *w, double dt)
   for (j = NG; j < NG + w->n2; j++)
   for (i = NG; i < NG + w->n1; i++) {
     double r, rho, p, T, T_f, t_l, t_u, test, E_f;
-    b.comp_c_pre = b.comp_h_pre = b.line_c_pre = b.brem_c_pre = b.xray_h_pre = 1.0;
+    double pre[5];
+    prefactors(w, j, i, pre);                   /* cooling.c:74-79 */
+    b.comp_h_pre = pre[0]; b.comp_c_pre = pre[1]; b.xray_h_pre = pre[2];
+    b.brem_c_pre = pre[3]; b.line_c_pre = pre[4];
     r = w->x1[i]*w->UL;
     rho = w->V[VI(w, RHO, j, i)]*w->UD;
     p = w->V[VI(w, PRS, j, i)];
@@ -974,7 +1022,8 @@ static void blondin(W *w, double dt)
 /* ------------------------------------------------------------------ */
 /* sub-cycled plain integrator: CoolingSource + Radiat + RKF12 / CK45   */
 /* ------------------------------------------------------------------ */
-typedef struct { W *w; double x1; double xi_s, tr_s; } Rad;
+typedef struct { W *w; double x1; double xi_s, tr_s; double pre[5]; } Rad;
+
 
 static void radiat(const Rad *R, double *v, double *rhs)
 {
@@ -995,11 +1044,12 @@ static void radiat(const Rad *R, double *v, double *rhs)
   sqxi = sqrt(xi);
   sqsqxi = pow(xi, 0.25);
   ne = 1.21*nH;
-  comp_heat = 1.0*(8.9e-36*xi*tx);
-  comp_cool = 1.0*(8.9e-36*xi*(4.0*T));
-  xray_heat = 1.0*(1.5e-21*(sqsqxi/sqT));
-  line_cool = 1.0*((1e-16*exp(-1.3e5/T)/sqxi/T) + fmin(fmin(1e-24, 5e-27*sqT), 1.5e-17/T));
-  brem_cool = 1.0*(3.3e-27*sqT);
+  /* cv_idl/radiat.c:118-123 */
+  comp_heat = R->pre[0]*(8.9e-36*xi*tx);
+  comp_cool = R->pre[1]*(8.9e-36*xi*(4.0*T));
+  xray_heat = R->pre[2]*(1.5e-21*(sqsqxi/sqT));
+  line_cool = R->pre[4]*((1e-16*exp(-1.3e5/T)/sqxi/T) + fmin(fmin(1e-24, 5e-27*sqT), 1.5e-17/T));
+  brem_cool = R->pre[3]*(3.3e-27*sqT);
   lambda = nH*(ne*comp_heat + nH*xray_heat - ne*comp_cool - ne*line_cool - ne*brem_cool);
   if (v[TRC] > 0.01) rhs[RHOE] = lambda*E_cost;
   else rhs[RHOE] = 0.0;
@@ -1100,6 +1150,7 @@ static void cooling_plain(W *w, double dt)
     Rad R;
     int stiff;
     R.w = w; R.x1 = w->x1[i]; R.xi_s = w->xi_s[CI(w, j, i)]; R.tr_s = w->tr_s[CI(w, j, i)];
+    prefactors(w, j, i, R.pre);
     for (nv = NVC; nv--;) v0[nv] = v1[nv] = k1[nv] = 0.0;
     for (nv = 0; nv < NVW; nv++) v0[nv] = v1[nv] = w->V[VI(w, nv, j, i)];
     prs = v0[PRS];
@@ -1210,6 +1261,19 @@ static void w_init(W *pw, const orc_wind_cfg *c, const double *Vin)
       w.xi_s[CI(&w, j, i)] = lx/nH/(r*r);
       w.tr_s[CI(&w, j, i)] = c->t_x;
     }
+  if (c->krad == 999 && c->alpharad == 999) {   /* line_connect.c:210-231 */
+    size_t np = (size_t)c->fm_points, nint = (size_t)w.n1*w.n2, q;
+    w.fm_lt = calloc(np ? np : 1, sizeof(double));
+    w.fm_lm = calloc(np*nint + 1, sizeof(double));
+    for (q = 0; q < np; q++) w.fm_lt[q] = log10(c->fm_t[q]);
+    for (q = 0; q < np*nint; q++) w.fm_lm[q] = log10(c->fm_m[q]);
+  }
+  if (c->xi)                                    /* restart: the file's values */
+    for (j = 0; j < w.n2; j++)
+      for (i = 0; i < w.n1; i++) {
+        w.xi_s[CI(&w, j + NG, i + NG)] = c->xi[(size_t)j*w.n1 + i];
+        w.tr_s[CI(&w, j + NG, i + NG)] = c->t_r[(size_t)j*w.n1 + i];
+      }
 
   *pw = w;
 }
@@ -1223,7 +1287,7 @@ static void w_free(W *pw)
   for (d = 0; d < 2; d++) for (a = 0; a < 6; a++) free(w.pc[d][a]);
   free(w.fr); free(w.ft); free(w.fp); free(w.roff); free(w.toff); free(w.moff); free(w.dvds);
   free(w.x11_0); free(w.x11_1); free(w.x22_0); free(w.x22_1); free(w.xi_s); free(w.tr_s);
-  free(w.flag); free(w.S); free(w.Ue); free(w.U0e);
+  free(w.flag); free(w.S); free(w.Ue); free(w.U0e); free(w.fm_lt); free(w.fm_lm);
 }
 
 /* Copy the interior of the state back to [nv][nx2][nx1] arrays. */
@@ -1353,6 +1417,7 @@ double orc_wind_cool_time(const orc_wind_cfg *c, double rho, double p0, double x
   rv[0] = rho;
   w.V = rv;
   R.w = &w; R.x1 = 1.0; R.xi_s = xi; R.tr_s = t_r;
+  R.pre[0] = R.pre[1] = R.pre[2] = R.pre[3] = R.pre[4] = 1.0;
   a = log(p0/(c->gamma - 1.0));
   b = log(p1/(c->gamma - 1.0));
   if (a == b) return 0.0;

@@ -62,6 +62,7 @@ __global__ void mlp_sample_kernel(Const k, long long n, unsigned long long seed,
   double p0 = rho * pow(10.0, lT) / (k.KELVIN * k.mu);
   RadCell R;
   R.rho = rho; R.trc = 1.0; R.x1 = 1.0; R.xi_s = pow(10.0, lx); R.tr_s = k.t_x; R.t = 5.0;
+  wind::load_pre(nullptr, 0, 0, R.pre);
   R.nev = 0;
   double T0 = p0 / rho * k.KELVIN * k.mu;
   double E0 = p0 / (k.gamma - 1.0);
@@ -410,6 +411,7 @@ __device__ __forceinline__ RadCell cell_state(const CoolArgs& a, long long cell)
   R.x1 = a.x1[(int)(cell % a.nr) + 2];
   R.xi_s = a.xi_s[cell];
   R.tr_s = a.tr_s[cell];
+  wind::load_pre(a.pre, a.sv, cell, R.pre);
   R.t = a.s->t;
   R.nev = 0;
   return R;
@@ -495,7 +497,9 @@ __global__ __launch_bounds__(128) void surr_classify_kernel(CoolBatch b, SurrWor
         const double T0 = p / R.rho * k.KELVIN * k.mu;
         const double x[4] = {log10(R.rho * k.UD), log10(T0), log10(xi),
                              log10(dt / (E0 / fabs(k1)))};
-        bool in = tx == k.t_x && E0 > 0.0;
+        // the network is trained on unit prefactors and T_r = T_x
+        bool in = tx == k.t_x && E0 > 0.0 && R.pre[0] == 1.0 && R.pre[1] == 1.0 &&
+                  R.pre[2] == 1.0 && R.pre[3] == 1.0 && R.pre[4] == 1.0;
         for (int c = 0; c < 4; c++) {
           in = in && x[c] >= HULL_LO[c] && x[c] <= HULL_HI[c];
           f[c] = (float)x[c];
@@ -966,6 +970,7 @@ int pmx_mlp_cool_cells(pmx_mlp* m, const pmx_wind_desc* d, long long n, const do
   if (e == hipSuccess) {
     wind::CoolArgs a;
     a.V = dV; a.nr = 1; a.nt = (int)n; a.sv = n; a.k = k; a.x1 = dx1; a.xi_s = dxi; a.tr_s = dtr;
+    a.pre = nullptr;
     a.s = ds; a.dtc = ddt;
     const long long off[2] = {0, n};
     e = hipMemcpy(dargs, &a, sizeof(a), hipMemcpyHostToDevice);

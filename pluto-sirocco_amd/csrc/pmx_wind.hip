@@ -28,6 +28,7 @@ namespace wind {
 constexpr int NG = 2;          // ghost cells of the reference grid (PLM)
 constexpr int BX = 64, BY = 2; // stage-kernel workgroup
 constexpr int NLF = 7;         // line-force tables per (angle, cell)
+constexpr int FM_POINTS_MAX = 4096;   // fit points of a tabulated force multiplier
 
 // Waves per SIMD that the batch stage kernel's register allocation targets.
 // The cell update needs ~230 VGPRs: the one-grid kernel reaches 2 waves per
@@ -48,6 +49,9 @@ struct Tables {
   const double *f1c, *f2c;     // [nr], [nt]: centre bilinear weights
   const double *sta, *cta;     // [na]: sin / cos of the bin angle
   double x1l0, sin_x2l0;       // x1 and |sin x2| of the lower edge of the first ghost zone
+  // tabulated force multiplier (k.fm_points > 0): log10 t [fm_points] and
+  // log10 M [fm_points][nt][nr] of the fits, as read_sirocco_fluxes keeps them
+  const double *fm_lt, *fm_lm;
 };
 
 struct StageArgs {
@@ -226,9 +230,10 @@ __device__ __forceinline__ void plm_zone(const StageArgs& a, int j, int i, const
 // One RK2 stage (Src/Time_Stepping/rk_step.c:117-330 with UpdateStage,
 // Src/Time_Stepping/update_stage.c:35-410, for GEOMETRY SPHERICAL in 2-D).
 // FEAT: bit 0 builds the characteristic-limiting path, bit 1 the entropy
-// switch (kernels for grids without them leave them out: they cost
-// registers); grids of a kernel built with a feature still check it.
-enum { FEAT_CHAR = 1, FEAT_ENT = 2 };
+// switch, bit 2 the tabulated force multiplier (kernels for grids without
+// them leave them out: they cost registers); grids of a kernel built with
+// a feature still check it.
+enum { FEAT_CHAR = 1, FEAT_ENT = 2, FEAT_TAB = 4 };
 
 template <int STAGE, int FEAT>
 __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, double& cdt,
@@ -296,7 +301,9 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       double T2 = vc[PRS] / vc[RHO] * k.KELVIN * k.mu;
       double vth1 = sqrt((2.0 * C_kB * T1) / C_mp);
       double vth2 = sqrt((2.0 * C_kB * T2) / C_mp);
-      const double A1 = force_factor(k, rho1, vth1), A2 = force_factor(k, rho2, vth2);
+      const bool tab = (FEAT & FEAT_TAB) && k.fm_points > 0;
+      double A1 = 0.0, A2 = 0.0;
+      if (!tab) { A1 = force_factor(k, rho1, vth1); A2 = force_factor(k, rho2, vth2); }
       const long long sa = a.sv * a.na;
       const double* lf = T.lf + cell;
       for (int ang = 0; ang < a.na; ang++) {
@@ -323,7 +330,12 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
         }
         double fr = q[LF_FR * sa], ft = q[LF_FT * sa];
         double fm1, fm2;
-        force_mult_pair(k, A1, A2, dvds, fm1, fm2);
+        if (tab) {   // both calls read the fit of this cell (line_connect.c:858-862)
+          fm1 = force_mult_tab(k, rho1, vth1, dvds, T.fm_lt, T.fm_lm + cell, a.sv);
+          fm2 = force_mult_tab(k, rho2, vth2, dvds, T.fm_lt, T.fm_lm + cell, a.sv);
+        } else {
+          force_mult_pair(k, A1, A2, dvds, fm1, fm2);
+        }
         g1r += ((fm1 * k.sigma_e * fr / C_c)) / k.UA;
         g2t += ((fm2 * k.sigma_e * ft / C_c)) / k.UA;
       }
@@ -572,7 +584,7 @@ __global__ __launch_bounds__(BX* BY) void wind_flag_kernel(StageArgs a, unsigned
 }
 
 template <int STAGE, int FEAT>
-__global__ __launch_bounds__(BX* BY) void wind_stage_kernel(StageArgs a) {
+__device__ __forceinline__ void stage_grid(const StageArgs& a) {
   const int i = blockIdx.x * BX + threadIdx.x;
   const int j = blockIdx.y * BY + threadIdx.y;
   double cdt = 0.0;
@@ -591,6 +603,21 @@ __global__ __launch_bounds__(BX* BY) void wind_stage_kernel(StageArgs a) {
   }
 }
 
+// The plain stage fits 2 waves per SIMD by itself (forcing the attribute
+// on it costs 13%); with characteristic limiting, the entropy switch or the
+// tabulated force multiplier the allocation grows past 256 registers
+// (VGPRs + AGPRs) and falls to 1 wave unless capped.
+template <int STAGE, int FEAT>
+__global__ __launch_bounds__(BX* BY) void wind_stage_kernel(StageArgs a) {
+  stage_grid<STAGE, FEAT>(a);
+}
+
+template <int STAGE, int FEAT>
+__global__ __launch_bounds__(BX* BY) __attribute__((amdgpu_waves_per_eu(PMX_WIND_WAVES, PMX_WIND_WAVES)))
+void wind_stage_kernel_w(StageArgs a) {
+  stage_grid<STAGE, FEAT>(a);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(128) void wind_cool_kernel(CoolArgs a, unsigned long long* calls) {
   const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -603,7 +630,9 @@ __global__ __launch_bounds__(128) void wind_cool_kernel(CoolArgs a, unsigned lon
     double rho = a.V[RHO * a.sv + cell], p = a.V[PRS * a.sv + cell];
     double pn;
     if (MODE == PMX_COOL_BLONDIN) {
-      pn = blondin_cell(a.k, rho, p, a.x1[i + NG], a.xi_s[cell], a.tr_s[cell], t, dt);
+      double pre[5];
+      load_pre(a.pre, a.sv, cell, pre);
+      pn = blondin_cell(a.k, rho, p, a.x1[i + NG], a.xi_s[cell], a.tr_s[cell], t, dt, pre);
     } else {
       RadCell R;
       R.rho = rho;
@@ -611,6 +640,7 @@ __global__ __launch_bounds__(128) void wind_cool_kernel(CoolArgs a, unsigned lon
       R.x1 = a.x1[i + NG];
       R.xi_s = a.xi_s[cell];
       R.tr_s = a.tr_s[cell];
+      load_pre(a.pre, a.sv, cell, R.pre);
       R.t = t;
       R.nev = 0;
       pn = plain_cell(a.k, R, p, dt, cand, bad);
@@ -689,6 +719,8 @@ struct pmx_wind {
   double* U;
   double* xi_s;
   double* tr_s;
+  double* pre;        // [5][sv] cooling prefactors (1.0 until set)
+  double* fm;         // tabulated force multiplier: log10 t, then log10 M; or null
   double* tab;        // one allocation for all geometry tables
   double* lf;
   unsigned char* shock;  // SHOCK_FLATTENING MULTID zone flags, or null
@@ -779,16 +811,19 @@ int flag_shock(pmx_wind* w, hipStream_t st) {
 }
 
 int features(const pmx_wind* w) {
-  return (w->k.char_lim ? FEAT_CHAR : 0) | (w->k.ent ? FEAT_ENT : 0);
+  return (w->k.char_lim ? FEAT_CHAR : 0) | (w->k.ent ? FEAT_ENT : 0) |
+         (w->k.fm_points > 0 ? FEAT_TAB : 0);
 }
 
 template <int STAGE>
 void launch_stage(pmx_wind* w, dim3 grid, dim3 block, const StageArgs& a) {
   switch (features(w)) {
     case 0: hipLaunchKernelGGL((wind_stage_kernel<STAGE, 0>), grid, block, 0, w->stream, a); break;
-    case 1: hipLaunchKernelGGL((wind_stage_kernel<STAGE, 1>), grid, block, 0, w->stream, a); break;
-    case 2: hipLaunchKernelGGL((wind_stage_kernel<STAGE, 2>), grid, block, 0, w->stream, a); break;
-    default: hipLaunchKernelGGL((wind_stage_kernel<STAGE, 3>), grid, block, 0, w->stream, a);
+    case 1: hipLaunchKernelGGL((wind_stage_kernel_w<STAGE, 1>), grid, block, 0, w->stream, a); break;
+    case 2: hipLaunchKernelGGL((wind_stage_kernel_w<STAGE, 2>), grid, block, 0, w->stream, a); break;
+    case 3: hipLaunchKernelGGL((wind_stage_kernel_w<STAGE, 3>), grid, block, 0, w->stream, a); break;
+    // the tabulated force multiplier: one kernel with every feature
+    default: hipLaunchKernelGGL((wind_stage_kernel_w<STAGE, 7>), grid, block, 0, w->stream, a);
   }
 }
 
@@ -815,6 +850,7 @@ CoolArgs cool_args(pmx_wind* w) {
   a.x1 = w->T.x1;
   a.xi_s = w->xi_s;
   a.tr_s = w->tr_s;
+  a.pre = w->pre;
   a.s = w->s;
   a.dtc = nullptr;
   return a;
@@ -892,6 +928,7 @@ Const make_const(const pmx_wind_desc* d) {
   k.ent = d->entropy_switch != 0;
   k.hll = d->solver == PMX_SOLVER_HLL;
   k.first_dt = d->first_dt;
+  k.fm_points = (d->krad == 999 && d->alpharad == 999) ? d->fm_points : 0;
   return k;
 }
 }  // namespace wind
@@ -906,7 +943,7 @@ int pmx_wind_destroy(pmx_wind* w) {
   for (auto& p : w->pend_stage) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
   for (auto& p : w->pend_cool) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
   hipFree(w->V[0]); hipFree(w->V[1]); hipFree(w->U); hipFree(w->xi_s); hipFree(w->tr_s);
-  hipFree(w->tab); hipFree(w->lf); hipFree(w->s); hipFree(w->shock); hipFree(w->Ue);
+  hipFree(w->pre); hipFree(w->fm); hipFree(w->tab); hipFree(w->lf); hipFree(w->s); hipFree(w->shock); hipFree(w->Ue);
   hipFree(w->sw.list); hipFree(w->sw.fb); hipFree(w->sw.feat); hipFree(w->sw.yhat);
   hipFree(w->sw.f0); hipFree(w->sw.cnt); hipFree(w->cargs); hipFree(w->coff);
   if (w->stream) hipStreamDestroy(w->stream);
@@ -935,6 +972,12 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
   if (!d->x1 || !d->x1l || !d->x1r || !d->dx1 || !d->x2 || !d->x2l || !d->x2r || !d->dx2 ||
       !d->flux_r || !d->flux_t || !d->flux_p) {
     set_error("missing grid or flux arrays"); return -1;
+  }
+  const bool tabulated = d->krad == 999 && d->alpharad == 999;
+  if (tabulated && (d->fm_points < 1 || d->fm_points > FM_POINTS_MAX || !d->fm_t || !d->fm_m)) {
+    set_error("krad = alpharad = 999 needs the force multiplier fits (fm_points 1..4096, "
+              "fm_t, fm_m)");
+    return -1;
   }
   pmx_wind* w = new pmx_wind();
   w->nr = d->nx1; w->nt = d->nx2; w->na = d->nangles; w->cooling = d->cooling;
@@ -1061,6 +1104,23 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
   CK(hipMalloc(&w->s, sizeof(Scalars)));
   if (d->shock_flattening) CK(hipMalloc(&w->shock, (size_t)(w->nr + 4) * (w->nt + 4)));
   if (d->entropy_switch) CK(hipMalloc(&w->Ue, sizeof(double) * w->sv));
+  CK(hipMalloc(&w->pre, sizeof(double) * 5 * w->sv));
+  {
+    std::vector<double> ones((size_t)5 * w->sv, 1.0);
+    CK(hipMemcpy(w->pre, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice));
+  }
+  if (tabulated) {   // the reader keeps log10 of t and M (line_connect.c:215, :228)
+    const size_t np = (size_t)d->fm_points, nm = np * (size_t)w->sv;
+    std::vector<double> lg(np + nm);
+    for (size_t q = 0; q < np; q++) lg[q] = log10(d->fm_t[q]);
+    for (size_t q = 0; q < nm; q++) lg[np + q] = log10(d->fm_m[q]);
+    CK(hipMalloc(&w->fm, sizeof(double) * lg.size()));
+    CK(hipMemcpy(w->fm, lg.data(), sizeof(double) * lg.size(), hipMemcpyHostToDevice));
+    w->T.fm_lt = w->fm;
+    w->T.fm_lm = w->fm + np;
+  } else {
+    w->T.fm_lt = w->T.fm_lm = nullptr;
+  }
   w->T.x1l0 = x1l[0];
   w->T.sin_x2l0 = fabs(sin(x2l[0]));
   CK(hipMalloc(&w->sw.cnt, sizeof(unsigned long long) * 8));
@@ -1132,6 +1192,21 @@ int pmx_wind_set_heatcool(pmx_wind* w, const double* xi, const double* t_r) {
     PMX_HIP_CHECK(hipGetLastError());
   }
   PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  return 0;
+}
+
+int pmx_wind_set_prefactors(pmx_wind* w, const double* pre) {
+  if (!w) { set_error("null argument"); return -1; }
+  if (pre) {
+    PMX_HIP_CHECK(hipMemcpyAsync(w->pre, pre, sizeof(double) * 5 * w->sv,
+                                 hipMemcpyHostToDevice, w->stream));
+    PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  } else {
+    std::vector<double> ones((size_t)5 * w->sv, 1.0);
+    PMX_HIP_CHECK(hipMemcpyAsync(w->pre, ones.data(), sizeof(double) * ones.size(),
+                                 hipMemcpyHostToDevice, w->stream));
+    PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  }
   return 0;
 }
 
@@ -1346,7 +1421,8 @@ struct pmx_wind_batch {
   long long* cell_off;    // device, G + 1 (multiples of 128)
   long long total_cells;  // padded
   long long cells;        // interior cells of all grids
-  int feat_any;           // some grid uses characteristic limiting or the entropy switch
+  int feat_any;           // some grid uses characteristic limiting, the entropy switch or
+                          // the tabulated force multiplier
   int shape;              // index into the tile-shape table
   int blocks;             // persistent workgroups per stage launch
   hipStream_t stream;
@@ -1376,7 +1452,7 @@ void launch_batch_stage(pmx_wind_batch* b) {
 #define PMX_SHAPE(q, X, Y)                                                                   \
   case q:                                                                                    \
     if (b->feat_any)                                                                         \
-      hipLaunchKernelGGL((wind_batch_stage_kernel<STAGE, 3, X, Y>), grid, dim3(X, Y), 0,     \
+      hipLaunchKernelGGL((wind_batch_stage_kernel<STAGE, 7, X, Y>), grid, dim3(X, Y), 0,     \
                          b->stream, args, b->tile_off, b->G, b->total_tiles);                \
     else                                                                                     \
       hipLaunchKernelGGL((wind_batch_stage_kernel<STAGE, 0, X, Y>), grid, dim3(X, Y), 0,     \

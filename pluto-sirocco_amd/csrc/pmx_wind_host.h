@@ -26,6 +26,7 @@ struct CoolArgs {
   const double* x1;      // ghost-inclusive radial centres
   const double* xi_s;
   const double* tr_s;
+  const double* pre;     // [5][sv] cooling prefactors, or null: all 1.0
   Scalars* s;
   const double* dtc;     // per-cell dt (independent cells), or null: s->dt
 };

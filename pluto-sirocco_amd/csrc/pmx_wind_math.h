@@ -53,6 +53,7 @@ struct Const {
   int ent;                                 // ENTROPY_SWITCH ALWAYS
   int hll;                                 // [Solver] hll instead of hllc
   double first_dt;                         // runtime->first_dt (dt-too-small stop)
+  int fm_points;                           // > 0: tabulated force multiplier (krad = alpharad = 999)
 };
 
 // ---------------------------------------------------------------------
@@ -274,6 +275,44 @@ PMX_HD double force_factor(const Const& k, double rho_cgs, double v_th) {
   return pow(k.sigma_e * rho_cgs * v_th, k.alpharad);
 }
 
+// linterp() of line_connect.c:781-811 over one cell's force-multiplier fit:
+// x = log10 t, xa = log10 t of the fit points, ya = log10 M of the cell at
+// the fit points (stride ys).  The first point not below x is found by a
+// linear scan (a NaN x stops at the first point, as there); 10^ of the
+// end value outside the fit range, of the linear interpolation inside;
+// a NaN result is 0.
+PMX_HD double linterp_fm(double x, const double* xa, const double* ya, long long ys, int n) {
+  int idx = 0;
+  while (idx < n && xa[idx] < x) idx++;
+  double result;
+  if (idx == 0) {
+    result = pow(10.0, ya[0]);
+  } else if (idx >= n) {
+    result = pow(10.0, ya[(long long)(n - 1) * ys]);
+  } else {
+    const double x_low = xa[idx - 1], x_high = xa[idx];
+    const double y_low = ya[(long long)(idx - 1) * ys], y_high = ya[(long long)idx * ys];
+    const double slope = (y_high - y_low) / (x_high - x_low);
+    result = pow(10.0, y_low + slope * (x - x_low));
+  }
+  if (isnan(result)) result = 0.0;
+  return result;
+}
+
+// Tabulated force multiplier of one LineForce call and bin
+// (line_connect.c:876-896): t = sigma_e rho v_th / dvds in the reference's
+// operation order, M = linterp(log10 t) capped at 4400, 0 where dvds <= 0.
+PMX_HD double force_mult_tab(const Const& k, double rho_cgs, double v_th, double dvds,
+                             const double* lt, const double* lm, long long ys) {
+  double M = 0.0;
+  if (dvds > 0.0) {
+    const double t_UV = k.sigma_e * rho_cgs * v_th / dvds;
+    M = linterp_fm(log10(t_UV), lt, lm, ys, k.fm_points);
+    if (M > 4400.) M = 4400.;
+  }
+  return 1.0 + M;
+}
+
 PMX_HD void force_mult_pair(const Const& k, double A1, double A2, double dvds,
                             double& f1, double& f2) {
   const double M_max = 4400.;
@@ -294,7 +333,15 @@ PMX_HD void force_mult_pair(const Const& k, double A1, double A2, double dvds,
 // ---------------------------------------------------------------------
 struct Blondin {
   double xi, sqxi, sqsqxi, ne, nH, n, tx, E, hc_init, dt_share;
+  double pre[5];   // comp_h, comp_c, xray_h, brem_c, line_c prefactors (cooling.c:74-79)
 };
+
+// The five prefactors of a cell from a [5][sv] array, or 1.0 for all
+// without one (what read_sirocco_heatcool sets at start-up).
+PMX_HD void load_pre(const double* pre, long long sv, long long cell, double* out) {
+#pragma unroll
+  for (int q = 0; q < 5; q++) out[q] = pre ? pre[q * sv + cell] : 1.0;
+}
 
 PMX_HD double ne_rat(double T) {   // cooling.c:210-222
   double ne_ratio;
@@ -308,17 +355,16 @@ PMX_HD double ne_rat(double T) {   // cooling.c:210-222
   return ne_ratio;
 }
 
-// heatcool(), cooling.c:173-202 with every prefactor equal to 1.0 (the
-// values read_sirocco_heatcool leaves at start-up)
+// heatcool(), cooling.c:173-202, with the cell's prefactors
 PMX_HD double heatcool(Blondin& b, double T) {
   double sqT = sqrt(T);
   b.ne = b.nH * ne_rat(T);
-  double comp_heat = 1.0 * (8.9e-36 * b.xi * b.tx);
-  double comp_cool = 1.0 * (8.9e-36 * b.xi * (4.0 * T));
-  double xray_heat = 1.0 * (1.5e-21 * (b.sqsqxi / sqT));
-  double line_cool = 1.0 * ((1e-16 * exp(-1.3e5 / T) / b.sqxi / T) +
-                            fmin(fmin(1e-24, 5e-27 * sqT), 1.5e-17 / T));
-  double brem_cool = 1.0 * (3.3e-27 * sqT);
+  double comp_heat = b.pre[0] * (8.9e-36 * b.xi * b.tx);
+  double comp_cool = b.pre[1] * (8.9e-36 * b.xi * (4.0 * T));
+  double xray_heat = b.pre[2] * (1.5e-21 * (b.sqsqxi / sqT));
+  double line_cool = b.pre[4] * ((1e-16 * exp(-1.3e5 / T) / b.sqxi / T) +
+                                 fmin(fmin(1e-24, 5e-27 * sqT), 1.5e-17 / T));
+  double brem_cool = b.pre[3] * (3.3e-27 * sqT);
   return b.nH * (b.ne * comp_heat + b.nH * xray_heat - b.ne * comp_cool - b.ne * line_cool -
                  b.ne * brem_cool);
 }
@@ -379,9 +425,11 @@ PMX_HD_NOINLINE double zbrent(Blondin& bl, double x1, double x2, double tol) {
 // pressure (code units), or p unchanged below 1e4 K.
 PMX_HD double blondin_cell(const Const& k, double rho_code, double p,
                                                double r_code, double xi_s, double tr_s,
-                                               double t, double dt) {
+                                               double t, double dt, const double* pre) {
   const double minT = 1.e4;
   Blondin b;
+#pragma unroll
+  for (int q = 0; q < 5; q++) b.pre[q] = pre[q];
   b.dt_share = dt * k.UT;
   double r = r_code * k.UL;
   double rho = rho_code * k.UD;
@@ -433,6 +481,7 @@ struct RadCell {
   double trc;
   double x1;       // code units
   double xi_s, tr_s;
+  double pre[5];   // comp_h, comp_c, xray_h, brem_c, line_c (cooling_source.c:101-105)
   double t;        // g_time
   mutable int nev; // Radiat() evaluations so far (cost accounting)
 };
@@ -451,12 +500,13 @@ PMX_HD double radiat(const Const& k, const RadCell& R, double& E) {
   else { xi = R.xi_s; tx = R.tr_s; }
   double sqT = sqrt(T), sqxi = sqrt(xi), sqsqxi = pow(xi, 0.25);
   double ne = 1.21 * nH;
-  double comp_heat = 1.0 * (8.9e-36 * xi * tx);
-  double comp_cool = 1.0 * (8.9e-36 * xi * (4.0 * T));
-  double xray_heat = 1.0 * (1.5e-21 * (sqsqxi / sqT));
-  double line_cool = 1.0 * ((1e-16 * exp(-1.3e5 / T) / sqxi / T) +
-                            fmin(fmin(1e-24, 5e-27 * sqT), 1.5e-17 / T));
-  double brem_cool = 1.0 * (3.3e-27 * sqT);
+  // cv_idl/radiat.c:118-123
+  double comp_heat = R.pre[0] * (8.9e-36 * xi * tx);
+  double comp_cool = R.pre[1] * (8.9e-36 * xi * (4.0 * T));
+  double xray_heat = R.pre[2] * (1.5e-21 * (sqsqxi / sqT));
+  double line_cool = R.pre[4] * ((1e-16 * exp(-1.3e5 / T) / sqxi / T) +
+                                 fmin(fmin(1e-24, 5e-27 * sqT), 1.5e-17 / T));
+  double brem_cool = R.pre[3] * (3.3e-27 * sqT);
   double lambda = nH * (ne * comp_heat + nH * xray_heat - ne * comp_cool - ne * line_cool -
                         ne * brem_cool);
   double rhs = (R.trc > 0.01) ? lambda * k.E_cost : 0.0;

@@ -44,7 +44,10 @@ EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro
            "pmx_wind_batch_create", "pmx_wind_batch_destroy", "pmx_wind_batch_grid",
            "pmx_wind_batch_set_step", "pmx_wind_batch_set_surrogate", "pmx_wind_batch_autotune",
            "pmx_wind_batch_set_shape", "pmx_wind_batch_run", "pmx_wind_batch_sync",
-           "pmx_wind_batch_timing", "pmx_wind_batch_cells"]
+           "pmx_wind_batch_timing", "pmx_wind_batch_cells",
+           "pmx_wind_set_prefactors", "pmx_sirocco_flux_angles", "pmx_sirocco_read_fluxes",
+           "pmx_sirocco_fm_points", "pmx_sirocco_read_force_multiplier",
+           "pmx_sirocco_read_heatcool", "pmx_sirocco_read_prefactors"]
 
 
 class PmxError(RuntimeError):
@@ -82,7 +85,8 @@ class WindDesc(ctypes.Structure):
                 ("flux_r", _dp), ("flux_t", _dp), ("flux_p", _dp),
                 ("dfloor", ctypes.c_double), ("shock_flattening", ctypes.c_int),
                 ("char_limiting", ctypes.c_int), ("entropy_switch", ctypes.c_int),
-                ("solver", ctypes.c_int), ("first_dt", ctypes.c_double)]
+                ("solver", ctypes.c_int), ("first_dt", ctypes.c_double),
+                ("fm_points", ctypes.c_int), ("fm_t", _dp), ("fm_m", _dp)]
 
 
 _lib = None
@@ -160,6 +164,15 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     lib.pmx_mlp_cool_cells.argtypes = [vp, ctypes.POINTER(WindDesc), ll, dp, dp, dp, dp, dp,
                                        ctypes.c_int, llp, dp]
     lib.pmx_wind_surrogate_stats.argtypes = [vp, llp, llp, llp, llp, ctypes.c_int]
+    lib.pmx_wind_set_prefactors.argtypes = [vp, dp]
+    cp = ctypes.c_char_p
+    wdp = ctypes.POINTER(WindDesc)
+    lib.pmx_sirocco_flux_angles.argtypes = [cp, ip]
+    lib.pmx_sirocco_read_fluxes.argtypes = [cp, wdp, ctypes.c_int, dp, dp, dp, ip, lp]
+    lib.pmx_sirocco_fm_points.argtypes = [cp, ip]
+    lib.pmx_sirocco_read_force_multiplier.argtypes = [cp, wdp, ctypes.c_int, dp, dp, ip, lp]
+    lib.pmx_sirocco_read_heatcool.argtypes = [cp, wdp, dp, dp, dp, ip, lp]
+    lib.pmx_sirocco_read_prefactors.argtypes = [cp, wdp, dp, ip, lp, lp]
     for name in EXPORTS:
         if name != "pmx_last_error":
             getattr(lib, name).restype = ctypes.c_int
@@ -400,9 +413,12 @@ class Mlp:
         return Y
 
 
-def wind_desc(cfg: setups.WindConfig, fluxes=None, grid=None, keep=None) -> WindDesc:
+def wind_desc(cfg: setups.WindConfig, fluxes=None, grid=None, keep=None,
+              sirocco=None) -> WindDesc:
     """Full descriptor of a wind grid; the arrays it points to are appended
-    to `keep` (they must outlive the create call)."""
+    to `keep` (they must outlive the create call).  ``sirocco``
+    (setups.SiroccoInputs) supplies the force multiplier fits of a grid
+    with krad = alpharad = 999."""
     gr, gt = grid if grid is not None else setups.wind_grid(cfg)
     fr, ft, fp = fluxes if fluxes is not None else setups.wind_fluxes(cfg)
     keep = [] if keep is None else keep
@@ -416,7 +432,79 @@ def wind_desc(cfg: setups.WindConfig, fluxes=None, grid=None, keep=None) -> Wind
     d.x1, d.x1l, d.x1r, d.dx1 = arr(gr["x"]), arr(gr["xl"]), arr(gr["xr"]), arr(gr["dx"])
     d.x2, d.x2l, d.x2r, d.dx2 = arr(gt["x"]), arr(gt["xl"]), arr(gt["xr"]), arr(gt["dx"])
     d.flux_r, d.flux_t, d.flux_p = arr(fr), arr(ft), arr(fp)
+    if sirocco is not None and sirocco.fm_t is not None:
+        d.fm_points = len(sirocco.fm_t)
+        d.fm_t, d.fm_m = arr(sirocco.fm_t), arr(sirocco.fm_m)
     return d
+
+
+def _grid_desc(cfg: setups.WindConfig, keep: list) -> WindDesc:
+    """Descriptor with the scalars and the cell centres of a wind grid (what
+    the SIROCCO file readers use)."""
+    gr, gt = setups.wind_grid(cfg)
+    d = wind_desc_scalars(cfg)
+    for a in (np.ascontiguousarray(gr["x"]), np.ascontiguousarray(gt["x"])):
+        keep.append(a)
+    d.x1, d.x2 = _dptr(keep[-2]), _dptr(keep[-1])
+    return d
+
+
+def read_sirocco_files(cfg: setups.WindConfig, directory, V: np.ndarray | None = None) -> dict:
+    """Read the files of a SIROCCO cycle in ``directory`` with the
+    library's readers (read_sirocco_fluxes and read_sirocco_heatcool with
+    restart_flag = 1).  Returns "fluxes" (flux_r, flux_t, flux_p)
+    [nangles][1][nt][nr] or None, "inputs" (setups.SiroccoInputs: xi, t_r
+    after the reader's floors, pre [6][nt][nr] (all 1 without the file),
+    fm_t, fm_m), and per file "found" / "matched" records.  V (the state,
+    for the start-up xi / T_r when py_heatcool.dat is missing) defaults to
+    no heat-cool arrays when py_heatcool.dat is missing."""
+    lib = load_library()
+    keep: list = []
+    g = _grid_desc(cfg, keep)
+    dirb = str(directory).encode()
+    nt, nr = cfg.nx[1], cfg.nx[0]
+    out = {"found": {}, "matched": {}}
+    na = ctypes.c_int()
+    _check(lib.pmx_sirocco_flux_angles(dirb, ctypes.byref(na)), "flux_angles")
+    out["fluxes"] = None
+    if na.value:
+        fl = [np.zeros((na.value, 1, nt, nr)) for _ in range(3)]
+        fnd, m = ctypes.c_int(), (ctypes.c_long * 3)()
+        _check(lib.pmx_sirocco_read_fluxes(dirb, ctypes.byref(g), na.value, *map(_dptr, fl),
+                                           ctypes.byref(fnd), m), "read_fluxes")
+        out["fluxes"] = tuple(fl)
+        out["found"]["fluxes"], out["matched"]["fluxes"] = fnd.value, list(m)
+    inp = setups.SiroccoInputs()
+    npts = ctypes.c_int()
+    _check(lib.pmx_sirocco_fm_points(dirb, ctypes.byref(npts)), "fm_points")
+    if npts.value:
+        t = np.zeros(npts.value)
+        m_ = np.zeros((npts.value, nt, nr))
+        fnd, cnt = ctypes.c_int(), ctypes.c_long()
+        _check(lib.pmx_sirocco_read_force_multiplier(dirb, ctypes.byref(g), npts.value, _dptr(t),
+                                                     _dptr(m_), ctypes.byref(fnd),
+                                                     ctypes.byref(cnt)), "read_force_multiplier")
+        inp.fm_t, inp.fm_m = t, m_
+        out["found"]["fm"], out["matched"]["fm"] = fnd.value, cnt.value
+    if V is not None or (pathlib.Path(directory) / "py_heatcool.dat").exists():
+        xi, t_r = np.zeros((nt, nr)), np.zeros((nt, nr))
+        rho = None if V is None else np.ascontiguousarray(V[setups.RHO, 0], dtype=np.float64)
+        fnd, cnt = ctypes.c_int(), ctypes.c_long()
+        _check(lib.pmx_sirocco_read_heatcool(dirb, ctypes.byref(g),
+                                             None if rho is None else _dptr(rho), _dptr(xi),
+                                             _dptr(t_r), ctypes.byref(fnd), ctypes.byref(cnt)),
+               "read_heatcool")
+        inp.xi, inp.t_r = xi, t_r
+        out["found"]["heatcool"], out["matched"]["heatcool"] = fnd.value, cnt.value
+    pre = np.zeros((6, nt, nr))
+    fnd, cnt, neg = ctypes.c_int(), ctypes.c_long(), ctypes.c_long()
+    _check(lib.pmx_sirocco_read_prefactors(dirb, ctypes.byref(g), _dptr(pre), ctypes.byref(fnd),
+                                           ctypes.byref(cnt), ctypes.byref(neg)), "read_prefactors")
+    inp.pre = pre
+    out["found"]["prefactors"], out["matched"]["prefactors"] = fnd.value, cnt.value
+    out["negative_prefactors"] = neg.value
+    out["inputs"] = inp
+    return out
 
 
 class Wind:
@@ -433,12 +521,12 @@ class Wind:
         w.cfg, w._h, w._lib, w._owned = cfg, handle, load_library(), False
         return w
 
-    def __init__(self, cfg: setups.WindConfig, fluxes=None, grid=None):
+    def __init__(self, cfg: setups.WindConfig, fluxes=None, grid=None, sirocco=None):
         self.cfg = cfg
         self._owned = True
         lib = load_library()
         self._keep = []
-        d = wind_desc(cfg, fluxes, grid, keep=self._keep)
+        d = wind_desc(cfg, fluxes, grid, keep=self._keep, sirocco=sirocco)
         h = ctypes.c_void_p()
         _check(lib.pmx_wind_create(ctypes.byref(d), ctypes.byref(h)), "pmx_wind_create")
         self._keep = []
@@ -475,6 +563,24 @@ class Wind:
         xi = np.ascontiguousarray(xi, dtype=np.float64)
         t_r = np.ascontiguousarray(t_r, dtype=np.float64)
         _check(self._lib.pmx_wind_set_heatcool(self._h, _dptr(xi), _dptr(t_r)), "set_heatcool")
+
+    def set_prefactors(self, pre: np.ndarray | None) -> None:
+        """Cooling prefactors [6][nt][nr] in prefactors.dat's column order
+        (comp_h, comp_c, xray_h, brem_c, line_c, xi_ion), or None for 1.0."""
+        if pre is None:
+            _check(self._lib.pmx_wind_set_prefactors(self._h, None), "set_prefactors")
+            return
+        pre = np.ascontiguousarray(pre, dtype=np.float64)
+        if pre.shape[0] < 5 or pre[0].size != self.cfg.nx[0] * self.cfg.nx[1]:
+            raise PmxError("prefactors must be [>=5][nt][nr]")
+        _check(self._lib.pmx_wind_set_prefactors(self._h, _dptr(pre)), "set_prefactors")
+
+    def set_sirocco(self, inp: "setups.SiroccoInputs") -> None:
+        """The heat-cool arrays (after the reader's floors) and prefactors of
+        a restarted SIROCCO cycle."""
+        if inp.xi is not None:
+            self.set_heatcool(*inp.heatcool_floored())
+        self.set_prefactors(inp.pre)
 
     def get_prim(self) -> np.ndarray:
         out = np.empty(self.cfg.shape)
@@ -542,13 +648,16 @@ class WindBatch:
     """Independent wind grids advanced together (batched persistent stage
     kernel, batched cooling, optional HIP graph replay)."""
 
-    def __init__(self, cfgs):
+    def __init__(self, cfgs, sirocco=None):
+        """``sirocco``: per grid, the setups.SiroccoInputs whose force
+        multiplier fits a grid with krad = alpharad = 999 needs (or None)."""
         lib = load_library()
         self.cfgs = list(cfgs)
+        sirocco = list(sirocco) if sirocco is not None else [None] * len(self.cfgs)
         keep = []
         descs = (WindDesc * len(self.cfgs))()
         for g, cfg in enumerate(self.cfgs):
-            descs[g] = wind_desc(cfg, keep=keep)
+            descs[g] = wind_desc(cfg, keep=keep, sirocco=sirocco[g])
         h = ctypes.c_void_p()
         _check(lib.pmx_wind_batch_create(descs, len(self.cfgs), ctypes.byref(h)), "batch_create")
         self._h, self._lib = h, lib

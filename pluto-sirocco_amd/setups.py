@@ -456,6 +456,111 @@ def write_flux_files(cfg: WindConfig, directory, fluxes=None) -> None:
                     f.write(f"{i} {j} 0 {xs[i] * cfg.unit_length:.17e} {ts[j]:.17e} {vals}\n")
 
 
+@dataclasses.dataclass
+class SiroccoInputs:
+    """What a SIROCCO cycle hands PLUTO besides the fluxes, as its files
+    hold them (Src/LineDriven/line_connect.c:183-497):
+
+    * ``xi``, ``t_r`` [nt][nr]: py_heatcool.dat's ionisation parameter and
+      radiation temperature (the reader floors them at 1 and 1e3 K);
+    * ``pre`` [6][nt][nr]: prefactors.dat's comp_h, comp_c, xray_h, brem_c,
+      line_c and xi_ion prefactors;
+    * ``fm_t`` [points], ``fm_m`` [points][nt][nr]: M_UV_data.dat's force
+      multiplier fits M(t) per cell (used when krad = alpharad = 999).
+
+    A field left None: that file is not written."""
+    xi: np.ndarray | None = None
+    t_r: np.ndarray | None = None
+    pre: np.ndarray | None = None
+    fm_t: np.ndarray | None = None
+    fm_m: np.ndarray | None = None
+
+    def heatcool_floored(self):
+        """xi and t_r as read_sirocco_heatcool leaves them (line_connect.c:354-355)."""
+        xi, t_r = self.xi.copy(), self.t_r.copy()
+        xi[xi < 1.0] = 1.0
+        t_r[t_r < 1.e3] = 1.e3
+        return xi, t_r
+
+
+def sirocco_inputs(cfg: WindConfig, V: np.ndarray, seed: int = 5, heatcool: bool = True,
+                   prefactors: bool = True, fm_points: int = 0,
+                   fm_range: tuple = (1.e-7, 1.e3)) -> SiroccoInputs:
+    """Synthetic SIROCCO cycle outputs for the wind state V: xi spread a
+    decade either side of the start-up value L_x / (n_H r^2) and T_r within
+    a factor 2 of T_x (3% of cells below each reader floor); prefactors in
+    [0.5, 1.5); fits M(t) = k t^alpha (1 + 0.3 (u - 1/2)) with per-cell
+    k in [0.1, 0.4) and alpha in [-0.7, -0.5) at ``fm_points`` values of t
+    spaced logarithmically over ``fm_range``."""
+    nt, nr = cfg.nx[1], cfg.nx[0]
+    idx = np.arange(nt * nr, dtype=np.uint64)
+    u = [uniform01(seed + s, idx).reshape(nt, nr) for s in range(12)]
+    out = SiroccoInputs()
+    if heatcool:
+        gr, _ = wind_grid(cfg)
+        g = cfg.nghost
+        r = gr["x"][g:-g][None, :] * cfg.unit_length
+        nH = V[RHO, 0] * cfg.unit_density / (1.43 * CONST["mp"])
+        xi0 = cfg.l_star * cfg.f_x / nH / (r * r)
+        out.xi = xi0 * 10.0 ** (2.0 * u[0] - 1.0)
+        out.xi[u[1] < 0.03] = 0.5
+        out.t_r = cfg.t_x * 2.0 ** (2.0 * u[2] - 1.0)
+        out.t_r[u[3] < 0.03] = 500.0
+    if prefactors:
+        out.pre = np.stack([0.5 + u[4 + q] for q in range(6)])
+    if fm_points:
+        out.fm_t = np.logspace(np.log10(fm_range[0]), np.log10(fm_range[1]), fm_points)
+        k = 0.1 + 0.3 * u[10]
+        alpha = -0.7 + 0.2 * u[11]
+        wig = np.stack([uniform01(seed + 100 + q, idx).reshape(nt, nr) for q in range(fm_points)])
+        out.fm_m = k[None] * out.fm_t[:, None, None] ** alpha[None] * (1.0 + 0.3 * (wig - 0.5))
+    return out
+
+
+def write_sirocco_files(cfg: WindConfig, directory, inp: SiroccoInputs,
+                        V: np.ndarray | None = None) -> None:
+    """Write py_heatcool.dat, prefactors.dat and M_UV_data.dat for the
+    fields of ``inp`` that are set, in the layouts the reference parses
+    (line_connect.c:183-257, :330-343, :430-437); every cell of the grid
+    gets one record, values with 17 significant digits.  V (for the
+    density columns) defaults to a unit density."""
+    import pathlib
+    directory = pathlib.Path(directory)
+    gr, gt = wind_grid(cfg)
+    g = cfg.nghost
+    xs, ts = gr["x"][g:-g], gt["x"][g:-g]
+    nt, nr = cfg.nx[1], cfg.nx[0]
+    dens = (V[RHO, 0] if V is not None else np.ones((nt, nr))) * cfg.unit_density
+    e = lambda v: f"{v:.17e}"
+    if inp.xi is not None:
+        with open(directory / "py_heatcool.dat", "w") as f:
+            f.write("i j rcen thetacen vol t_e t_r xi ne heat_xray heat_comp heat_lines "
+                    "heat_ff cool_comp cool_lines cool_ff rho n_h\n")
+            for j in range(nt):
+                for i in range(nr):
+                    rest = " ".join(["1.0e+30", "1.0e+05", e(inp.t_r[j, i]), e(inp.xi[j, i]),
+                                     "1.0e+00"] + ["0.0e+00"] * 7 +
+                                    [e(dens[j, i]), e(dens[j, i] / (1.43 * CONST["mp"]))])
+                    f.write(f"{i} {j} {e(xs[i] * cfg.unit_length)} {e(ts[j])} {rest}\n")
+    if inp.pre is not None:
+        with open(directory / "prefactors.dat", "w") as f:
+            f.write("i rcen j thetacen rho comp_h comp_c xray_h brem_c line_c xi_ion\n")
+            for j in range(nt):
+                for i in range(nr):
+                    vals = " ".join(e(inp.pre[q, j, i]) for q in range(6))
+                    f.write(f"{i} {e(xs[i] * cfg.unit_length)} {j} {e(ts[j])} "
+                            f"{e(dens[j, i])} {vals}\n")
+    if inp.fm_t is not None:
+        n = len(inp.fm_t)
+        with open(directory / "M_UV_data.dat", "w") as f:
+            f.write(f"MPOINTS {n}\n")
+            f.write("t " + " ".join(e(t) for t in inp.fm_t) + "\n")
+            for j in range(nt):
+                for i in range(nr):
+                    vals = " ".join(e(inp.fm_m[q, j, i]) for q in range(n))
+                    f.write(f"{i} {j} {e(xs[i] * cfg.unit_length)} {e(ts[j])} {vals}\n")
+
+
 def c3_wind(nr: int = 512, nt: int = 256, cooling: int = COOL_PLAIN) -> WindConfig:
     return WindConfig(nx=(nr, nt), cooling=cooling)
 

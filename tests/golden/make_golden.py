@@ -100,6 +100,37 @@ def cvidl_cases():
            setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7)), 8)
 
 
+def restart_cases():
+    """A restarted SIROCCO cycle (t0 = 10 > 3: the cooling reads the
+    files' xi and T_r, some below the reader's floors) with per-cell
+    prefactors, for both cooling modes; the tabulated force multiplier
+    (krad = alpharad = 999, fits over t = 1e-4 .. 1e-2, so that the
+    line-force t falls below, inside and above them); and the production
+    set-up with all of it.  Yields (name, cfg, V0, nsteps, inputs, t0)."""
+    for cool, tag in ((setups.COOL_PLAIN, "plain"), (setups.COOL_BLONDIN, "blondin")):
+        cfg = setups.WindConfig(nx=(32, 16), cooling=cool)
+        V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
+        yield (f"wind_{tag}_restart_32x16", cfg, V0, 8,
+               setups.sirocco_inputs(cfg, V0, heatcool=True, prefactors=True), 10.0)
+    cfg = setups.WindConfig(nx=(32, 16), krad=999.0, alpharad=999.0)
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
+    yield ("wind_plain_tabulated_32x16", cfg, V0, 8,
+           setups.sirocco_inputs(cfg, V0, heatcool=False, prefactors=False, fm_points=12,
+                                 fm_range=(1.e-4, 1.e-2)), None)
+    cfg = dataclasses.replace(setups.cvidl_wind(32, 16, dfloor=1.e-15), first_dt=1.e-3,
+                              shock_flattening=True, char_limiting=True, entropy_switch=True,
+                              solver=setups.HLL, krad=999.0, alpharad=999.0)
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
+    yield ("wind_cvidl_prod_restart_32x16", cfg, V0, 8,
+           setups.sirocco_inputs(cfg, V0, heatcool=True, prefactors=True, fm_points=12,
+                                 fm_range=(1.e-4, 1.e-2)), 10.0)
+
+
+def sirocco_arrays(inp):
+    """The SiroccoInputs fields as fixture arrays (prefix sir_)."""
+    return {f"sir_{k}": v for k, v in dataclasses.asdict(inp).items() if v is not None}
+
+
 def wind_inputs(cfg):
     """Grid and flux arrays exactly as handed to the reference."""
     gr, gt = setups.wind_grid(cfg)
@@ -109,7 +140,7 @@ def wind_inputs(cfg):
             "flux_r": fr, "flux_t": ft, "flux_p": fp}
 
 
-def main(which=("hydro", "wind", "cvidl")):
+def main(which=("hydro", "wind", "cvidl", "restart")):
     refrun.build(with_reference=True)
     wind = list(wind_cases()) if "wind" in which else []
     wind += list(cvidl_cases()) if "cvidl" in which else []
@@ -122,6 +153,17 @@ def main(which=("hydro", "wind", "cvidl")):
                             dt_seq=r["dt_seq"], t=np.array(r["t"]), dt=np.array(r["dt"]),
                             nsteps=np.array(n), config=np.array(json.dumps(meta)), **inp)
         print(f"{name}: {n} steps, t={r['t']:.6e}, next dt={r['dt']:.6e}")
+    for name, cfg, V0, n, sir, t0 in (restart_cases() if "restart" in which else ()):
+        inp = wind_inputs(cfg)
+        fl = (inp["flux_r"], inp["flux_t"], inp["flux_p"])
+        r = refrun.run_reference(cfg, V0, n, fluxes=fl, sirocco=sir, t0=t0)
+        meta = dataclasses.asdict(cfg)
+        np.savez_compressed(OUT / f"{name}.npz", V0=V0, V=r["V"], U=r["U"],
+                            dt_seq=r["dt_seq"], t=np.array(r["t"]), dt=np.array(r["dt"]),
+                            nsteps=np.array(n), config=np.array(json.dumps(meta)),
+                            t0=np.array(0.0 if t0 is None else t0), **inp,
+                            **sirocco_arrays(sir))
+        print(f"{name}: {n} steps, t={r['t']:.6e}, next dt={r['dt']:.6e}")
     for name, cfg, V0, n in (cases() if "hydro" in which else ()):
         r = refrun.run_reference(cfg, V0, n)
         meta = dataclasses.asdict(cfg)
@@ -132,4 +174,4 @@ def main(which=("hydro", "wind", "cvidl")):
 
 
 if __name__ == "__main__":
-    main(tuple(sys.argv[1:]) or ("hydro", "wind", "cvidl"))
+    main(tuple(sys.argv[1:]) or ("hydro", "wind", "cvidl", "restart"))

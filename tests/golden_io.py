@@ -30,3 +30,13 @@ def load(name):
     meta["xend"] = tuple(meta["xend"])
     cfg = setups.HydroConfig(**meta)
     return cfg, {k: z[k] for k in z.files if k != "config"}
+
+
+def sirocco(g):
+    """The SIROCCO restart inputs stored with a wind fixture (sir_* arrays)
+    as a setups.SiroccoInputs, or None; and its start time (0 if none)."""
+    keys = [k for k in g if k.startswith("sir_")]
+    t0 = float(g["t0"]) if "t0" in g else 0.0
+    if not keys:
+        return None, t0
+    return setups.SiroccoInputs(**{k[4:]: g[k] for k in keys}), t0

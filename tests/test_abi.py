@@ -53,24 +53,29 @@ def test_header_compiles_as_c():
     assert r.returncode == 0, r.stderr
 
 
-def test_descriptor_layout_matches_header(tmp_path):
-    """The ctypes mirror of pmx_wind_desc has the C struct's size and the
-    offset of its last field."""
+@pytest.mark.parametrize("struct,cname", [("WindDesc", "pmx_wind_desc"),
+                                          ("HydroDesc", "pmx_hydro_desc")])
+def test_descriptor_layout_matches_header(tmp_path, struct, cname):
+    """The ctypes mirrors of the descriptors have the C structs' size and
+    the offset of every field."""
     import shutil
     import subprocess
     cc = shutil.which("gcc") or shutil.which("cc")
     if cc is None:
         pytest.skip("no C compiler")
+    py = getattr(plutomx, struct)
+    names = [f[0] for f in py._fields_]
     src = tmp_path / "sz.c"
+    body = "".join(f' printf("%zu\\n", offsetof({cname}, {n}));' for n in names)
     src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "plutomx.h"\n'
-                   'int main(void){printf("%zu %zu\\n", sizeof(pmx_wind_desc),'
-                   ' offsetof(pmx_wind_desc, dfloor)); return 0;}\n')
+                   f'int main(void){{printf("%zu\\n", sizeof({cname}));{body} return 0;}}\n')
     exe = tmp_path / "sz"
     subprocess.run([cc, "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
-    size, off = map(int, subprocess.run([str(exe)], capture_output=True, text=True,
-                                        check=True).stdout.split())
-    assert ctypes.sizeof(plutomx.WindDesc) == size
-    assert plutomx.WindDesc.dfloor.offset == off
+    vals = list(map(int, subprocess.run([str(exe)], capture_output=True, text=True,
+                                        check=True).stdout.split()))
+    assert ctypes.sizeof(py) == vals[0]
+    for n, off in zip(names, vals[1:]):
+        assert getattr(py, n).offset == off, n
 
 
 def test_wind_descriptor_validation_without_gpu():
@@ -81,8 +86,10 @@ def test_wind_descriptor_validation_without_gpu():
     bad = [dataclasses.replace(setups.cvidl_wind(16, 8), dfloor=-1.0),
            dataclasses.replace(setups.cvidl_wind(16, 8),
                                bc=(setups.BC_WIND_POLE, 1, setups.BC_WIND_INNER, 2)),
-           dataclasses.replace(setups.cvidl_wind(16, 8), bc=(9, 1, 2, 2))]
-    msgs = ["dfloor must be >= 0", "is an r-beg", "unknown boundary type"]
+           dataclasses.replace(setups.cvidl_wind(16, 8), bc=(9, 1, 2, 2)),
+           dataclasses.replace(setups.cvidl_wind(16, 8), krad=999.0, alpharad=999.0)]
+    msgs = ["dfloor must be >= 0", "is an r-beg", "unknown boundary type",
+            "needs the force multiplier fits"]
     for cfg, msg in zip(bad, msgs):
         with pytest.raises(plutomx.PmxError, match=msg):
             plutomx.Wind(cfg)

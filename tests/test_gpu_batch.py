@@ -33,10 +33,12 @@ def states(cfgs, rho0=1e-11):
     return [setups.wind_ic_mixed(c, seed=30 + g, rho0_cgs=rho0) for g, c in enumerate(cfgs)]
 
 
-def run_single(pmx, cfg, V0, nsteps, mlp=None):
-    with pmx.Wind(cfg) as w:
+def run_single(pmx, cfg, V0, nsteps, mlp=None, sirocco=None, t0=0.0):
+    with pmx.Wind(cfg, sirocco=sirocco) as w:
         w.set_prim(V0)
-        w.set_time(0.0, cfg.first_dt, 0)
+        if sirocco is not None:
+            w.set_sirocco(sirocco)
+        w.set_time(t0, cfg.first_dt, 0)
         if mlp is not None:
             w.set_surrogate(mlp)
         w.run(nsteps)
@@ -44,11 +46,15 @@ def run_single(pmx, cfg, V0, nsteps, mlp=None):
         return w.get_prim(), w.get_cons(), w.get_time()
 
 
-def run_batch(pmx, cfgs, V0s, nsteps, graph=False, tune=False, mlp=None):
-    with pmx.WindBatch(cfgs) as b:
-        for w, cfg, V0 in zip(b.grids, cfgs, V0s):
+def run_batch(pmx, cfgs, V0s, nsteps, graph=False, tune=False, mlp=None, sirocco=None,
+              t0=0.0):
+    sirocco = sirocco or [None] * len(cfgs)
+    with pmx.WindBatch(cfgs, sirocco=sirocco) as b:
+        for w, cfg, V0, sir in zip(b.grids, cfgs, V0s, sirocco):
             w.set_prim(V0)
-            w.set_time(0.0, cfg.first_dt, 0)
+            if sir is not None:
+                w.set_sirocco(sir)
+            w.set_time(t0, cfg.first_dt, 0)
         b.set_step(0)
         if mlp is not None:
             b.set_surrogate(mlp)
@@ -113,6 +119,27 @@ def test_batch_with_production_boundaries_matches_single_grids(pmx):
     batch = run_batch(pmx, cfgs, V0s, 6, graph=True)
     for cfg, V0, (Vb, Ub, tb) in zip(cfgs, V0s, batch):
         Vs, Us, ts = run_single(pmx, cfg, V0, 6)
+        assert np.array_equal(Vb, Vs) and np.array_equal(Ub, Us)
+        assert tb["dt"] == ts["dt"]
+
+
+def test_batch_with_sirocco_restart_inputs_matches_single_grids(pmx):
+    """A restarted cycle (t0 = 10: the cooling reads each grid's xi / T_r,
+    with per-cell prefactors) over plain grids, a grid with the tabulated
+    force multiplier and a production grid with it: every grid as on its
+    own, bit for bit, with graph replay."""
+    tab = setups.WindConfig(nx=(40, 24), krad=999.0, alpharad=999.0, first_dt=1e-3)
+    prod = dataclasses.replace(setups.cvidl_wind(32, 24, dfloor=1.e-15), first_dt=1.e-3,
+                               shock_flattening=True, char_limiting=True, entropy_switch=True,
+                               solver=setups.HLL, krad=999.0, alpharad=999.0)
+    cfgs = small_batch()[:2] + [tab, prod]
+    V0s = [setups.wind_ic_mixed(c, seed=60 + g, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
+           for g, c in enumerate(cfgs)]
+    sirs = [setups.sirocco_inputs(c, V, seed=70 + g, fm_points=10 if c.krad == 999.0 else 0,
+                                  fm_range=(1e-4, 1e-2)) for g, (c, V) in enumerate(zip(cfgs, V0s))]
+    batch = run_batch(pmx, cfgs, V0s, 6, graph=True, sirocco=sirs, t0=10.0)
+    for cfg, V0, sir, (Vb, Ub, tb) in zip(cfgs, V0s, sirs, batch):
+        Vs, Us, ts = run_single(pmx, cfg, V0, 6, sirocco=sir, t0=10.0)
         assert np.array_equal(Vb, Vs) and np.array_equal(Ub, Us)
         assert tb["dt"] == ts["dt"]
 

@@ -129,3 +129,41 @@ def test_non_stiff_wind_state_is_untouched(pmx, trained):
             out.append((w.get_prim(), w.surrogate_stats()))
     assert np.array_equal(out[0][0], out[1][0])
     assert out[1][1]["listed"] == 0
+
+
+def test_non_unit_prefactors_take_the_plain_integrator(pmx, trained):
+    """The network is trained on unit prefactors: cells whose prefactors
+    (prefactors.dat) differ from 1 are never handed to it; the surrogate
+    path then reproduces the plain integrator bit for bit.  The control run
+    with unit prefactors on the same stiff state does list cells."""
+    cfg = setups.c3_wind(64, 32)
+    V0 = setups.wind_ic(cfg, seed=8)
+    nt, nr = cfg.nx[1], cfg.nx[0]
+    # rho = 5e-11 g/cm^3, T = 5e5 .. 2e6 K, xi = 100 and dt = 10: every cell
+    # is stiff (|dE/dt| > 0.5 / dt in code units; the oracle sends all of
+    # them to Cash-Karp) and dt / t_cool stays inside the training hull
+    kelvin = cfg.unit_velocity ** 2 * setups.CONST["amu"] / setups.CONST["kB"]
+    T = 5e5 * 4.0 ** setups.uniform01(9, np.arange(nt * nr, dtype=np.uint64)).reshape(nt, nr)
+    V0[setups.RHO, 0] = 5e-11 / cfg.unit_density
+    V0[setups.PRS, 0] = V0[setups.RHO, 0] * T / (kelvin * cfg.mu)
+    V0[setups.TRC] = 1.0
+    xi, t_r = np.full((nt, nr), 100.0), np.full((nt, nr), cfg.t_x)
+    pre = setups.sirocco_inputs(cfg, V0, heatcool=False).pre
+
+    def run(pre_, use):
+        with pmx.Wind(cfg) as w:
+            w.set_prim(V0)
+            w.set_heatcool(xi, t_r)
+            w.set_prefactors(pre_)
+            w.set_time(10.0, 10.0, 0)
+            if use:
+                w.set_surrogate(trained)
+            w.split_source()
+            w.sync()
+            return w.get_prim(), w.surrogate_stats()
+
+    assert run(None, True)[1]["listed"] > 0
+    plain, surr = run(pre, False), run(pre, True)
+    assert surr[1]["listed"] == 0
+    assert np.array_equal(plain[0], surr[0])
+

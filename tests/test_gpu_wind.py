@@ -42,10 +42,13 @@ def assert_close(a, b, what, rtol=RTOL):
                                int(bad.sum()))
 
 
-def gpu_run(pmx, cfg, V0, nsteps, fluxes=None, grid=None, stepwise=False):
-    with pmx.Wind(cfg, fluxes=fluxes, grid=grid) as w:
+def gpu_run(pmx, cfg, V0, nsteps, fluxes=None, grid=None, stepwise=False, sirocco=None,
+            t0=0.0):
+    with pmx.Wind(cfg, fluxes=fluxes, grid=grid, sirocco=sirocco) as w:
         w.set_prim(V0)
-        w.set_time(0.0, cfg.first_dt, 0)
+        if sirocco is not None:
+            w.set_sirocco(sirocco)
+        w.set_time(t0, cfg.first_dt, 0)
         dts = []
         for _ in range(nsteps):
             dts.append(w.get_time()["dt"])
@@ -64,7 +67,9 @@ def fixture_inputs(g):
 def test_wind_gpu_matches_reference_fixture(pmx, name):
     cfg, g = golden_io.load(name)
     fluxes, grid = fixture_inputs(g)
-    V, U, dts, tm = gpu_run(pmx, cfg, g["V0"], int(g["nsteps"]), fluxes, grid)
+    sir, t0 = golden_io.sirocco(g)
+    V, U, dts, tm = gpu_run(pmx, cfg, g["V0"], int(g["nsteps"]), fluxes, grid, sirocco=sir,
+                            t0=t0)
     assert_close(V, g["V"], name + " V")
     assert_close(U, g["U"], name + " U")
     np.testing.assert_allclose(dts, g["dt_seq"], rtol=RTOL, atol=0)

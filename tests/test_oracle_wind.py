@@ -9,6 +9,7 @@ the dt sequence, t and the next dt bit for bit.  Each fixture also states
 which branches of the cooling and time-step logic it reaches, so a fixture
 that silently stops covering (say) the stiff Cash-Karp path fails here.
 """
+import dataclasses
 import math
 
 import numpy as np
@@ -28,8 +29,10 @@ def run_fixture(name):
     cfg, g = golden_io.load(name)
     grid = ({"x": g["x1"], "xl": g["x1l"], "xr": g["x1r"], "dx": g["dx1"]},
             {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
-    r = refrun.run_wind_oracle(cfg, g["V0"], int(g["nsteps"]),
-                               fluxes=(g["flux_r"], g["flux_t"], g["flux_p"]), grid=grid)
+    sir, t0 = golden_io.sirocco(g)
+    r = refrun.run_wind_oracle(cfg, g["V0"], int(g["nsteps"]), t0=t0,
+                               fluxes=(g["flux_r"], g["flux_t"], g["flux_p"]), grid=grid,
+                               sirocco=sir)
     return cfg, g, r, refrun.wind_counters()
 
 
@@ -54,6 +57,10 @@ COVERAGE = {
     "wind_cvidl_char_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
     "wind_cvidl_prod_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
     "wind_cvidl_prod_hll_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
+    "wind_plain_restart_32x16": ["rkf12_ok", "ck45_calls", "dt_cool"],
+    "wind_blondin_restart_32x16": ["bl_solve", "below_minT"],
+    "wind_plain_tabulated_32x16": ["rkf12_ok", "ck45_calls"],
+    "wind_cvidl_prod_restart_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
 }
 
 
@@ -138,6 +145,92 @@ def test_hll_fixture_pins_the_solver():
     r = refrun.run_wind_oracle(dataclasses.replace(cfg, solver=setups.HLLC), g["V0"],
                                int(g["nsteps"]), fluxes=fl, grid=grid)
     assert not np.array_equal(r["V"], g["V"])
+
+
+def _restart_variant(name, cfg_change=None, **sir_change):
+    """The oracle on a restart fixture with its config or SIROCCO inputs
+    changed; True if the bitwise match with the reference survives."""
+    cfg, g = golden_io.load(name)
+    if cfg_change:
+        cfg = dataclasses.replace(cfg, **cfg_change)
+    sir, t0 = golden_io.sirocco(g)
+    sir = dataclasses.replace(sir, **sir_change)
+    grid = ({"x": g["x1"], "xl": g["x1l"], "xr": g["x1r"], "dx": g["dx1"]},
+            {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
+    r = refrun.run_wind_oracle(cfg, g["V0"], int(g["nsteps"]), t0=t0, sirocco=sir,
+                               fluxes=(g["flux_r"], g["flux_t"], g["flux_p"]), grid=grid)
+    return np.array_equal(r["V"], g["V"])
+
+
+@pytest.mark.parametrize("name", ["wind_plain_restart_32x16", "wind_blondin_restart_32x16"])
+def test_restart_fixtures_pin_prefactors_and_heatcool(name):
+    """Each prefactor, the files' xi / T_r and the reader's floors change
+    the restart fixtures: dropping the prefactors, swapping two of their
+    columns (the two cooling terms brem_c / line_c, the Compton pair
+    comp_h / comp_c), dropping xi / T_r, or handing the cooling the
+    unfloored values all break the bitwise match with the reference."""
+    cfg, g = golden_io.load(name)
+    sir, _ = golden_io.sirocco(g)
+    assert _restart_variant(name)
+    assert not _restart_variant(name, pre=None)
+    for a, b in ((3, 4), (0, 1), (0, 2)):
+        pre = sir.pre.copy()
+        pre[[a, b]] = pre[[b, a]]
+        assert not _restart_variant(name, pre=pre), (a, b)
+    assert not _restart_variant(name, xi=None, t_r=None)
+    # the reader's floors reach cells of these fixtures
+    assert (sir.xi < 1.0).any() and (sir.t_r < 1.e3).any()
+
+    class NoFloor(setups.SiroccoInputs):
+        def heatcool_floored(self):
+            return self.xi, self.t_r
+    cfg2, g2 = golden_io.load(name)
+    grid = ({"x": g2["x1"], "xl": g2["x1l"], "xr": g2["x1r"], "dx": g2["dx1"]},
+            {"x": g2["x2"], "xl": g2["x2l"], "xr": g2["x2r"], "dx": g2["dx2"]})
+    r = refrun.run_wind_oracle(cfg2, g2["V0"], int(g2["nsteps"]), t0=float(g2["t0"]),
+                               sirocco=NoFloor(xi=sir.xi, t_r=sir.t_r, pre=sir.pre),
+                               fluxes=(g2["flux_r"], g2["flux_t"], g2["flux_p"]), grid=grid)
+    assert not np.array_equal(r["V"], g2["V"])
+
+
+def test_tabulated_fixture_pins_linterp():
+    """The tabulated force multiplier fixture pins linterp(): the k, alpha
+    form, the fits of another cell, the first or the last fit point (the
+    values used below and above the fit range) and the interior points all
+    change the result."""
+    name = "wind_plain_tabulated_32x16"
+    cfg, g = golden_io.load(name)
+    sir, _ = golden_io.sirocco(g)
+    assert _restart_variant(name)
+    assert not _restart_variant(name, cfg_change={"krad": 0.2, "alpharad": -0.6})
+    assert not _restart_variant(name, fm_m=np.roll(sir.fm_m, 1, axis=2))
+    for q in (0, len(sir.fm_t) // 2, len(sir.fm_t) - 1):
+        m = sir.fm_m.copy()
+        m[q] *= 1.5
+        assert not _restart_variant(name, fm_m=m), q
+
+
+def test_linterp_closed_forms():
+    """linterp() on a power law: with M = k t^a at the fit points the
+    log-log interpolation is exact, so the tabulated force multiplier of a
+    cell equals the k, alpha one wherever t lies inside the fit range; the
+    tabulated run with fits of exactly k t^alpha over a range covering
+    every t therefore agrees with the k, alpha run to rounding."""
+    name = "wind_plain_tabulated_32x16"
+    cfg, g = golden_io.load(name)
+    sir, _ = golden_io.sirocco(g)
+    t = np.logspace(-12.0, 8.0, 41)
+    m = 0.2 * t[:, None, None] ** -0.6 * np.ones((1,) + sir.fm_m.shape[1:])
+    grid = ({"x": g["x1"], "xl": g["x1l"], "xr": g["x1r"], "dx": g["dx1"]},
+            {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
+    fl = (g["flux_r"], g["flux_t"], g["flux_p"])
+    a = refrun.run_wind_oracle(cfg, g["V0"], 4, fluxes=fl, grid=grid,
+                               sirocco=setups.SiroccoInputs(fm_t=t, fm_m=m))
+    b = refrun.run_wind_oracle(dataclasses.replace(cfg, krad=0.2, alpharad=-0.6), g["V0"], 4,
+                               fluxes=fl, grid=grid)
+    for nv in range(6):
+        scale = np.abs(b["V"][nv]).max()
+        np.testing.assert_allclose(a["V"][nv], b["V"][nv], rtol=1e-9, atol=1e-9 * scale)
 
 
 def test_wind_fixture_inputs_match_generators():
