@@ -252,6 +252,50 @@ int pmx_wind_timing(pmx_wind *w, int enable, int reset, double *stage_ms, int *s
                     double *cool_ms, int *cool_launches);
 
 /* ------------------------------------------------------------------ */
+/* Output: snapshots of the device state for the .dbl files a SIROCCO    */
+/* cycle reads back, with the production problem's user variables.       */
+/* ------------------------------------------------------------------ */
+/* ComputeUserVar of Test_Problems/LineDrivenWind/cv_idl/userdef_output.c
+ * (:20-260) for the production uservar list (cv_idl/pluto_sirocco_sub.py:86),
+ * in this order: */
+#define PMX_NUV 23
+#define PMX_UV_NAMES "XI T T_r ch cc lc bc xh ch_pre cc_pre lc_pre bc_pre xh_pre XI_pre " \
+                     "ne nh gr gt gp dv_ds t M_max1 M_max2"
+
+/* Take an output snapshot, asynchronously: on the context's stream (after
+ * the work queued before it) V is copied to a staging buffer in device
+ * memory and, with user_vars, the 23 user variables are computed next to
+ * it (as WriteData calls ComputeUserVar before writing, Src/write_data.c:
+ * 96-101: the line-force quantities use the dv/ds of the last stage's
+ * input, so user_vars needs a step first); a copy stream then moves the
+ * snapshot to pinned host memory while later steps run.  A second snapshot
+ * waits on the device for the first one's transfer.  For a grid of a
+ * batch, call pmx_wind_batch_sync first. */
+int pmx_wind_snapshot(pmx_wind *w, int user_vars);
+/* Wait for the last snapshot and copy it to out[6 (+ 23)][nx2][nx1] (V,
+ * then the user variables); t, dt: g_time and g_dt at the snapshot; step:
+ * the context's step counter. */
+int pmx_wind_snapshot_wait(pmx_wind *w, double *out, double *t, double *dt, long *step);
+
+/* WriteData for DBL_OUTPUT in single_file mode (Src/write_data.c:29-200,
+ * :420-450, Src/bin_io.c:216-258): dir/data.NNNN.dbl holds the nvar arrays
+ * data[nvar][nx3][nx2][nx1] (interior cells, host byte order) one after
+ * another; dir/dbl.out gets the line "nfile t dt step single_file little
+ * names..." after its first nfile lines (the file is rewritten from there,
+ * "w" for nfile 0).  names: nvar names, e.g. "rho vx1 vx2 vx3 prs tr1" and
+ * PMX_UV_NAMES split at spaces. */
+int pmx_write_dbl(const char *dir, int nfile, const double *data, int nvar,
+                  const char *const *names, int nx1, int nx2, int nx3, double t, double dt,
+                  long step);
+/* RestartDump (Src/restart.c:276-315): record `counter` (0, 1, ...) of
+ * dir/restart.out, the reference's Restart structure (Src/structs.h:256-
+ * 262: int nstep, int nfile[16], double t, double dt, 40 fill bytes; 128
+ * bytes), nfile[16] being each output type's file counter.  The fill bytes
+ * are written as zeros (the reference writes whatever its stack held). */
+int pmx_write_restart(const char *dir, int counter, long nstep, const int *nfile, double t,
+                      double dt);
+
+/* ------------------------------------------------------------------ */
 /* SIROCCO file interface (Src/LineDriven/line_connect.c:43-497): host-  */
 /* side readers of the files a SIROCCO cycle writes.  Records are matched */
 /* to cells of the grid of `grid` (nx1, nx2, x1, x2 with ghosts,          */

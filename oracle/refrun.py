@@ -66,7 +66,7 @@ def read_dump(path: pathlib.Path, shape) -> dict:
 
 def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
                   timeout: float = 3600.0, fluxes=None, sirocco=None,
-                  t0: float | None = None, files_dir=None) -> dict:
+                  t0: float | None = None, files_dir=None, output: bool = False) -> dict:
     """Run the reference for ``nsteps`` (>= 2) steps from ``V0``.
 
     PLUTO's ``-maxsteps N`` integrates N+1 steps (Src/main.c:233-240), so
@@ -77,7 +77,10 @@ def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
     SIROCCO cycle; with heat-cool or prefactor data the run reads them as a
     restart does (restart_flag = 1).  ``t0`` starts the clock there, as a
     restart from a file at that time would.  ``files_dir``: take the
-    SIROCCO files (*.dat) from there instead, as a restart."""
+    SIROCCO files (*.dat) from there instead, as a restart.  ``output``
+    (wind): the production user variables and a .dbl file every step; the
+    result then holds "dbl" (the bytes of every data.NNNN.dbl, in order),
+    "dbl_out" (the text of dbl.out) and "restart_out" (its bytes)."""
     if nsteps < 2:
         raise ValueError("the reference driver needs nsteps >= 2")
     binary = reference_binary(cfg.reference_problem())
@@ -85,7 +88,7 @@ def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
         raise FileNotFoundError(f"{binary} not built (make -C oracle/ref)")
     with tempfile.TemporaryDirectory(prefix="pmxref_") as tmp:
         tmp = pathlib.Path(tmp)
-        (tmp / "pluto.ini").write_text(cfg.pluto_ini())
+        (tmp / "pluto.ini").write_text(cfg.pluto_ini(output=True) if output else cfg.pluto_ini())
         ic = np.ascontiguousarray(V0, dtype="<f8")
         if getattr(cfg, "entropy_switch", False):
             # the build carries Vc[ENTR] too; Boundary() sets it from rho, p
@@ -127,6 +130,10 @@ def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
         # 0..N-1 followed by the next dt.
         dts = np.loadtxt(tmp / "dump.bin.dt", ndmin=2)
         out["dt_seq"] = dts[:-1, 2]
+        if output:
+            out["dbl"] = [f.read_bytes() for f in sorted(tmp.glob("data.*.dbl"))]
+            out["dbl_out"] = (tmp / "dbl.out").read_text()
+            out["restart_out"] = (tmp / "restart.out").read_bytes()
         return out
 
 
@@ -233,6 +240,9 @@ def _wind_struct(cfg, fluxes=None, grid=None, sirocco=None):
         L.orc_wind_run.argtypes = [ctypes.POINTER(_OrcWindCfg), dp, ctypes.c_int, dp, dp,
                                    dp, dp]
         L.orc_wind_run.restype = ctypes.c_int
+        L.orc_wind_run_uv.argtypes = [ctypes.POINTER(_OrcWindCfg), dp, ctypes.c_int, dp, dp,
+                                      dp, dp, dp]
+        L.orc_wind_run_uv.restype = ctypes.c_int
         L.orc_wind_cool.argtypes = [ctypes.POINTER(_OrcWindCfg), dp, ctypes.c_double,
                                     ctypes.c_double, dp, dp, dp]
         L.orc_wind_cool.restype = ctypes.c_int
@@ -270,7 +280,7 @@ def _wind_struct(cfg, fluxes=None, grid=None, sirocco=None):
             xi, t_r = sirocco.heatcool_floored()
             c.xi, c.t_r = arr(xi), arr(t_r)
         if sirocco.pre is not None:
-            c.pre = arr(sirocco.pre[:5])
+            c.pre = arr(sirocco.pre[:6])
         if sirocco.fm_t is not None:
             c.fm_points = len(sirocco.fm_t)
             c.fm_t, c.fm_m = arr(sirocco.fm_t), arr(sirocco.fm_m)
@@ -278,24 +288,30 @@ def _wind_struct(cfg, fluxes=None, grid=None, sirocco=None):
 
 
 def run_wind_oracle(cfg, V0: np.ndarray, nsteps: int, dt0: float | None = None,
-                    t0: float = 0.0, fluxes=None, grid=None, sirocco=None) -> dict:
+                    t0: float = 0.0, fluxes=None, grid=None, sirocco=None,
+                    user_vars: bool = False) -> dict:
     """Run the wind restatement (orc_wind_run) for nsteps steps from step
     number 0.  ``fluxes`` = (flux_r, flux_t, flux_p) [nangles][1][nt][nr]
     (default: setups.wind_fluxes); ``grid`` = (radial, polar) dicts with
     x, xl, xr, dx including ghosts (default: setups.wind_grid);
-    ``sirocco``: setups.SiroccoInputs of a restarted cycle."""
+    ``sirocco``: setups.SiroccoInputs of a restarted cycle; ``user_vars``:
+    also ComputeUserVar on the final state ("uv", [23][nt][nr])."""
     c, keep = _wind_struct(cfg, fluxes, grid, sirocco)
     V = np.ascontiguousarray(V0, dtype=np.float64).copy()
     U = np.empty_like(V)
     dt = np.array([cfg.first_dt if dt0 is None else dt0])
     t = np.array([t0])
     hist = np.empty(max(nsteps, 1))
-    rc = lib().orc_wind_run(ctypes.byref(c), _ptr(V), nsteps, _ptr(dt), _ptr(t), _ptr(U),
-                            _ptr(hist))
+    uv = np.zeros((23, cfg.nx[1], cfg.nx[0])) if user_vars else None
+    rc = lib().orc_wind_run_uv(ctypes.byref(c), _ptr(V), nsteps, _ptr(dt), _ptr(t), _ptr(U),
+                               _ptr(hist), _ptr(uv) if uv is not None else None)
     if rc != 0:
         raise RuntimeError(f"orc_wind_run failed ({rc})")
-    return {"V": V, "U": U, "t": float(t[0]), "dt": float(dt[0]),
-            "dt_seq": hist[:nsteps].copy()}
+    out = {"V": V, "U": U, "t": float(t[0]), "dt": float(dt[0]),
+           "dt_seq": hist[:nsteps].copy()}
+    if uv is not None:
+        out["uv"] = uv
+    return out
 
 
 def run_wind_cool_oracle(cfg, V0: np.ndarray, dt: float, t: float = 0.0, xi=None, t_r=None,

@@ -87,8 +87,8 @@ typedef struct {
    * (line_connect.c:308-497); NULL = its start-up values:                   */
   const double *xi, *t_r;     /* [nx2][nx1] sirocco_xi, sirocco_t_r (read by
                                  the cooling once g_time > 3)                */
-  const double *pre;          /* [5][nx2][nx1] comp_h, comp_c, xray_h, brem_c,
-                                 line_c prefactors (NULL: all 1.0)           */
+  const double *pre;          /* [6][nx2][nx1] comp_h, comp_c, xray_h, brem_c,
+                                 line_c, xi_ion prefactors (NULL: all 1.0)   */
   /* krad = alpharad = 999: the tabulated force multiplier
    * (line_connect.c:183-257, :858-887)                                      */
   int    fm_points;
@@ -102,6 +102,15 @@ typedef struct {
  * at step number 0.  dt/t in/out as in orc_run.  U_out optional. */
 int orc_wind_run(const orc_wind_cfg *cfg, double *V, int nsteps, double *dt,
                  double *t, double *U_out, double *dt_hist);
+
+/* orc_wind_run, and then the wind problem's ComputeUserVar
+ * (cv_idl/userdef_output.c:20-260) on the final state: uv[23][nx2][nx1] in
+ * the order of ORC_UV_NAMES (the production problem's uservar list). */
+#define ORC_NUV 23
+#define ORC_UV_NAMES "XI T T_r ch cc lc bc xh ch_pre cc_pre lc_pre bc_pre xh_pre XI_pre " \
+                     "ne nh gr gt gp dv_ds t M_max1 M_max2"
+int orc_wind_run_uv(const orc_wind_cfg *cfg, double *V, int nsteps, double *dt, double *t,
+                    double *U_out, double *dt_hist, double *uv);
 
 /* One SplitSource (cooling) call on V ([6][nx2][nx1], in/out) at time t
  * with step dt.  xi / t_r ([nx2][nx1]) are the SIROCCO arrays, or NULL for

@@ -1303,8 +1303,111 @@ static void w_out(const W *w, double *Vout, double *U_out)
       }
 }
 
+/* ComputeUserVar, cv_idl/userdef_output.c:20-260, on the current state
+ * with the dv/ds of the last VGradCalc (the stage-2 input of the last
+ * step).  M_UV, t_UV, t_UV2 and dvds2 start at 0 where the reference
+ * leaves them uninitialised (read only for cells whose first flux-carrying
+ * bins all have dv/ds <= 0). */
+static void user_vars(W *w, double *uv)
+{
+  const orc_wind_cfg *c = w->c;
+  const size_t nint = (size_t)w->n1*w->n2;
+  const double mu = c->mu, lx = c->l_star*c->f_x, tx = c->t_x, M_max = 4400.;
+  const double sigma_e = C_sigmaT/(C_amu*1.18);          /* userdef_output.c:107 */
+  const int tab = (c->krad == 999 && c->alpharad == 999);
+  int i, j, a;
+  enum { XI, T_, TR, CH, CC, LC, BC, XH, CHP, CCP, LCP, BCP, XHP, XIP, NE, NH, GR, GT, GP,
+         DVDS, TT, M1, M2 };
+  for (j = NG; j < NG + w->n2; j++)
+  for (i = NG; i < NG + w->n1; i++) {
+    size_t o = (size_t)(j - NG)*w->n1 + (i - NG);
+    double v[NVW], grad[3], pre[5], xip;
+    double r = w->x1[i]*w->UL, rho, T, nH, ne, ne_ratio, v_th, xi, sqrt_T, sqxi, sqsqxi;
+    double M_UV = 0.0, t_UV = 0.0, t_UV2 = 0.0, dvds2 = 0.0, M_temp1 = -1.0, M_temp2 = -1.0;
+    double fmax = 1e-99;
+    int nv;
+    for (nv = 0; nv < NVW; nv++) v[nv] = w->V[VI(w, nv, j, i)];
+    rho = v[RHO]*w->UD;
+    T = uv[T_*nint + o] = v[PRS]/v[RHO]*w->KELVIN*mu;
+    nH = rho/(1.43*C_mp);
+    uv[NH*nint + o] = nH;
+    if (T < 1.5e4) ne_ratio = 1e-2 + pow(10, (-51.59417133 + 12.27740153*log10(T)));
+    else if (T >= 1.5e4 && T < 3.3e4) ne_ratio = pow(10, (-3.80749689 + 0.86092628*log10(T)));
+    else ne_ratio = 1.21;
+    ne = nH*ne_ratio;
+    uv[NE*nint + o] = ne;
+    v_th = sqrt(2.0*C_kB*T/C_mp);
+    if (w->t <= 1.0) {
+      xi = lx/(nH*r*r);
+      uv[XI*nint + o] = xi;
+      uv[TR*nint + o] = tx;
+    } else {
+      xi = w->xi_s[CI(w, j, i)];
+      uv[XI*nint + o] = xi;
+      uv[TR*nint + o] = w->tr_s[CI(w, j, i)];
+    }
+    sqrt_T = sqrt(T);
+    sqxi = sqrt(xi);
+    sqsqxi = pow(xi, 0.25);
+    prefactors(w, j, i, pre);
+    xip = c->pre ? c->pre[(size_t)5*nint + o] : 1.0;
+    uv[CHP*nint + o] = pre[0]; uv[CCP*nint + o] = pre[1]; uv[XHP*nint + o] = pre[2];
+    uv[LCP*nint + o] = pre[4]; uv[BCP*nint + o] = pre[3]; uv[XIP*nint + o] = xip;
+    if (w->t < 3.0) {
+      uv[CH*nint + o] = (8.9e-36*xi*tx);
+      uv[CC*nint + o] = (8.9e-36*xi*(4.0*T));
+      uv[XH*nint + o] = (1.5e-21*(sqsqxi/sqrt_T));
+      uv[LC*nint + o] = ((1e-16*exp(-1.3e5/T)/sqxi/T) + fmin(fmin(1e-24, 5e-27*sqrt_T), 1.5e-17/T));
+      uv[BC*nint + o] = (3.3e-27*sqrt_T);
+    } else {
+      uv[CH*nint + o] = pre[0]*(8.9e-36*xi*tx);
+      uv[CC*nint + o] = pre[1]*(8.9e-36*xi*(4.0*T));
+      uv[XH*nint + o] = pre[2]*(1.5e-21*(sqsqxi/sqrt_T));
+      uv[LC*nint + o] = pre[4]*((1e-16*exp(-1.3e5/T)/sqxi/T) +
+                                fmin(fmin(1e-24, 5e-27*sqrt_T), 1.5e-17/T));
+      uv[BC*nint + o] = pre[3]*(3.3e-27*sqrt_T);
+    }
+    line_force(w, v, grad, j, i);
+    uv[GR*nint + o] = grad[0];
+    uv[GT*nint + o] = grad[1];
+    uv[GP*nint + o] = grad[2];
+    T = v[PRS]/v[RHO]*w->KELVIN*mu;
+    for (a = 0; a < w->na; a++) {
+      size_t qa = AI(w, a, j, i);
+      double dv = w->dvds[qa], fr = w->fr[qa], ft = w->ft[qa];
+      if (dv > 0.0) {
+        t_UV = sigma_e*rho*v_th/dv;
+        if (tab) {
+          const double *mcol = w->fm_lm + o;
+          M_UV = linterp(log10(t_UV), w->fm_lt, mcol, nint, c->fm_points);
+        } else {
+          M_UV = c->krad*pow(t_UV, c->alpharad);
+        }
+        if (M_UV > M_max) M_UV = M_max;
+        if (M_UV > M_temp1) M_temp1 = M_UV;
+      }
+      if (pow(fr, 2) + pow(ft, 2) > fmax) {
+        fmax = pow(M_UV*fr, 2) + pow(M_UV*ft, 2);
+        M_temp2 = M_UV;
+        t_UV2 = t_UV;
+        dvds2 = dv;
+      }
+    }
+    uv[M1*nint + o] = M_temp1;
+    uv[M2*nint + o] = M_temp2;
+    uv[DVDS*nint + o] = dvds2;
+    uv[TT*nint + o] = t_UV2;
+  }
+}
+
 int orc_wind_run(const orc_wind_cfg *c, double *Vin, int nsteps, double *dt, double *t,
                  double *U_out, double *dt_hist)
+{
+  return orc_wind_run_uv(c, Vin, nsteps, dt, t, U_out, dt_hist, NULL);
+}
+
+int orc_wind_run_uv(const orc_wind_cfg *c, double *Vin, int nsteps, double *dt, double *t,
+                    double *U_out, double *dt_hist, double *uv)
 {
   W w;
   int step;
@@ -1341,6 +1444,7 @@ int orc_wind_run(const orc_wind_cfg *c, double *Vin, int nsteps, double *dt, dou
     }
   }
 
+  if (uv) user_vars(&w, uv);
   w_out(&w, Vin, U_out);
   w_free(&w);
   return 0;

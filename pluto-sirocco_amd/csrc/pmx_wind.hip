@@ -144,6 +144,67 @@ __device__ __forceinline__ void ld_vel(const StageArgs& a, int jj, int ii, doubl
   if (ft) v2 = -v2;
 }
 
+// VGradCalc for one cell (line_connect.c:628-738) from the stage input:
+// the velocity averages at the four cell corners (:638-660) and the
+// centre velocity by bilinear interpolation (:746-777), in the (x, z)
+// frame; vgrad_dvds then gives one bin's |dv/ds| between the centre and
+// that bin's offset point, or -999 for a bin without flux.
+struct VGrad {
+  double v11_0, v11_1, v12_0, v12_1, v22_0, v22_1, v21_0, v21_1;
+  double vx1, vz1;
+};
+
+__device__ __forceinline__ void vgrad_cell(const StageArgs& a, int j, int i, VGrad& g) {
+  const Const& k = a.k;
+  const Tables& T = a.T;
+  double P1[3][3], P2[3][3];
+#pragma unroll
+  for (int dj = 0; dj < 3; dj++)
+#pragma unroll
+    for (int di = 0; di < 3; di++) ld_vel(a, j + dj - 1, i + di - 1, P1[dj][di], P2[dj][di]);
+  // P[dj][di] = V(j+dj-1, i+di-1)
+  g.v11_0 = (P1[0][0] + P1[0][1] + P1[1][0] + P1[1][1]) / 4.0;
+  g.v11_1 = (P2[0][0] + P2[0][1] + P2[1][0] + P2[1][1]) / 4.0;
+  g.v12_0 = (P1[2][0] + P1[1][0] + P1[2][1] + P1[1][1]) / 4.0;
+  g.v12_1 = (P2[2][0] + P2[1][0] + P2[2][1] + P2[1][1]) / 4.0;
+  g.v22_0 = (P1[2][1] + P1[2][2] + P1[1][2] + P1[1][1]) / 4.0;
+  g.v22_1 = (P2[2][1] + P2[2][2] + P2[1][2] + P2[1][1]) / 4.0;
+  g.v21_0 = (P1[1][2] + P1[0][2] + P1[0][1] + P1[1][1]) / 4.0;
+  g.v21_1 = (P2[1][2] + P2[0][2] + P2[0][1] + P2[1][1]) / 4.0;
+  double f1 = T.f1c[i], f2 = T.f2c[j], t1, t2;
+  t1 = (1.0 - f1) * g.v11_0 + f1 * g.v21_0;
+  t2 = (1.0 - f1) * g.v12_0 + f1 * g.v22_0;
+  double ans0 = (1.0 - f2) * t1 + f2 * t2;
+  t1 = (1.0 - f1) * g.v11_1 + f1 * g.v21_1;
+  t2 = (1.0 - f1) * g.v12_1 + f1 * g.v22_1;
+  double ans1 = (1.0 - f2) * t1 + f2 * t2;
+  const double sj = T.sin2[j + NG], cj = T.cos2[j + NG];
+  g.vx1 = (ans0 * k.UV * sj + ans1 * k.UV * cj);
+  g.vz1 = (ans0 * k.UV * cj - ans1 * k.UV * sj);
+}
+
+// q: the cell's line-force table entries of bin `ang`; sa: table stride.
+__device__ __forceinline__ double vgrad_dvds(const StageArgs& a, const VGrad& g,
+                                             const double* q, long long sa, int ang) {
+  const Const& k = a.k;
+  double ds = q[LF_DS * sa];
+  if (ds == -999) return -999;
+  double g1 = q[LF_F1 * sa], g2 = q[LF_F2 * sa], t1, t2;
+  t1 = (1.0 - g1) * g.v11_0 + g1 * g.v21_0;
+  t2 = (1.0 - g1) * g.v12_0 + g1 * g.v22_0;
+  double b0 = (1.0 - g2) * t1 + g2 * t2;
+  t1 = (1.0 - g1) * g.v11_1 + g1 * g.v21_1;
+  t2 = (1.0 - g1) * g.v12_1 + g1 * g.v22_1;
+  double b1 = (1.0 - g2) * t1 + g2 * t2;
+  double so = q[LF_SIN * sa], co = q[LF_COS * sa];
+  double vx2 = (b0 * k.UV * so + b1 * k.UV * co);
+  double vz2 = (b0 * k.UV * co - b1 * k.UV * so);
+  double st = a.T.sta[ang], ctt = a.T.cta[ang];
+  double u1 = st * g.vx1 + ctt * g.vz1;
+  double u2 = st * vx2 + ctt * vz2;
+  return fabs((u2 - u1) / ds);
+}
+
 __device__ __forceinline__ void ldc(const double* tab, int n, int idx, double* c) {
 #pragma unroll
   for (int q = 0; q < 6; q++) c[q] = tab[q * (n + 2 * NG) + idx];
@@ -267,31 +328,8 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
     //      :815-903): x1 uses vc = (vp + vm)/2, x2 the cell centre ------
     double g1r = 0.0, g2t = 0.0;
     {
-      double P1[3][3], P2[3][3];
-#pragma unroll
-      for (int dj = 0; dj < 3; dj++)
-#pragma unroll
-        for (int di = 0; di < 3; di++) ld_vel(a, j + dj - 1, i + di - 1, P1[dj][di], P2[dj][di]);
-      // corner averages, line_connect.c:638-660 (P[dj][di] = V(j+dj-1, i+di-1))
-      double v11_0 = (P1[0][0] + P1[0][1] + P1[1][0] + P1[1][1]) / 4.0;
-      double v11_1 = (P2[0][0] + P2[0][1] + P2[1][0] + P2[1][1]) / 4.0;
-      double v12_0 = (P1[2][0] + P1[1][0] + P1[2][1] + P1[1][1]) / 4.0;
-      double v12_1 = (P2[2][0] + P2[1][0] + P2[2][1] + P2[1][1]) / 4.0;
-      double v22_0 = (P1[2][1] + P1[2][2] + P1[1][2] + P1[1][1]) / 4.0;
-      double v22_1 = (P2[2][1] + P2[2][2] + P2[1][2] + P2[1][1]) / 4.0;
-      double v21_0 = (P1[1][2] + P1[0][2] + P1[0][1] + P1[1][1]) / 4.0;
-      double v21_1 = (P2[1][2] + P2[0][2] + P2[0][1] + P2[1][1]) / 4.0;
-      // bilinear at the cell centre (line_connect.c:746-777)
-      double f1 = T.f1c[i], f2 = T.f2c[j], t1, t2;
-      t1 = (1.0 - f1) * v11_0 + f1 * v21_0;
-      t2 = (1.0 - f1) * v12_0 + f1 * v22_0;
-      double ans0 = (1.0 - f2) * t1 + f2 * t2;
-      t1 = (1.0 - f1) * v11_1 + f1 * v21_1;
-      t2 = (1.0 - f1) * v12_1 + f1 * v22_1;
-      double ans1 = (1.0 - f2) * t1 + f2 * t2;
-      const double sj = T.sin2[J], cj = T.cos2[J];
-      double vx1 = (ans0 * k.UV * sj + ans1 * k.UV * cj);
-      double vz1 = (ans0 * k.UV * cj - ans1 * k.UV * sj);
+      VGrad vg;
+      vgrad_cell(a, j, i, vg);
       // thermal speeds of the two LineForce calls (line_connect.c:848-866)
       double vc1[NVW];
 #pragma unroll
@@ -308,26 +346,7 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       const double* lf = T.lf + cell;
       for (int ang = 0; ang < a.na; ang++) {
         const double* q = lf + (long long)ang * a.sv;
-        double ds = q[LF_DS * sa];
-        double dvds;
-        if (ds == -999) {
-          dvds = -999;
-        } else {
-          double g1 = q[LF_F1 * sa], g2 = q[LF_F2 * sa];
-          t1 = (1.0 - g1) * v11_0 + g1 * v21_0;
-          t2 = (1.0 - g1) * v12_0 + g1 * v22_0;
-          double b0 = (1.0 - g2) * t1 + g2 * t2;
-          t1 = (1.0 - g1) * v11_1 + g1 * v21_1;
-          t2 = (1.0 - g1) * v12_1 + g1 * v22_1;
-          double b1 = (1.0 - g2) * t1 + g2 * t2;
-          double so = q[LF_SIN * sa], co = q[LF_COS * sa];
-          double vx2 = (b0 * k.UV * so + b1 * k.UV * co);
-          double vz2 = (b0 * k.UV * co - b1 * k.UV * so);
-          double st = T.sta[ang], ctt = T.cta[ang];
-          double u1 = st * vx1 + ctt * vz1;
-          double u2 = st * vx2 + ctt * vz2;
-          dvds = fabs((u2 - u1) / ds);
-        }
+        const double dvds = vgrad_dvds(a, vg, q, sa, ang);
         double fr = q[LF_FR * sa], ft = q[LF_FT * sa];
         double fm1, fm2;
         if (tab) {   // both calls read the fit of this cell (line_connect.c:858-862)
@@ -691,6 +710,121 @@ __global__ void wind_end_kernel(Scalars* s, int next, double cfl, double cfl_max
   }
 }
 
+// ComputeUserVar of the production problem (cv_idl/userdef_output.c:20-260)
+// for one cell: uv[q * sv + cell] in PMX_UV_* order, from the state Vf, the
+// SIROCCO arrays, the prefactors pre[6][sv] and, for the line-force
+// quantities, the dv/ds of the last stage's input a.Vin (the reference's
+// dvds_array, set by the last VGradCalc).  The reference's LineForce and
+// its M loop use two differently rounded sigma_e (sigmaT/amu/1.18 and
+// sigmaT/(amu*1.18)); M_UV, t_UV, t_UV2 and dvds2 start at 0 where the
+// reference leaves them uninitialised.
+enum { UV_XI, UV_T, UV_TR, UV_CH, UV_CC, UV_LC, UV_BC, UV_XH, UV_CHP, UV_CCP, UV_LCP, UV_BCP,
+       UV_XHP, UV_XIP, UV_NE, UV_NH, UV_GR, UV_GT, UV_GP, UV_DVDS, UV_TT, UV_M1, UV_M2,
+       UV_COUNT };
+static_assert(UV_COUNT == PMX_NUV, "user variable count");
+
+__global__ __launch_bounds__(128) void wind_uservar_kernel(StageArgs a, const double* Vf,
+                                                           const double* xi_s,
+                                                           const double* tr_s,
+                                                           const double* pre, double* uv) {
+  const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= a.sv) return;
+  const int i = (int)(cell % a.nr), j = (int)(cell / a.nr);
+  const Const& k = a.k;
+  const Tables& T = a.T;
+  const long long sv = a.sv;
+  double* o = uv + cell;
+  double v[NVW];
+#pragma unroll
+  for (int nv = 0; nv < NVW; nv++) v[nv] = Vf[nv * sv + cell];
+  const double t = a.s->t;
+  const double r = T.x1[i + NG] * k.UL;
+  const double rho = v[RHO] * k.UD;
+  const double Tk = v[PRS] / v[RHO] * k.KELVIN * k.mu;
+  o[UV_T * sv] = Tk;
+  const double nH = rho / (1.43 * C_mp);
+  o[UV_NH * sv] = nH;
+  double ne_ratio;
+  if (Tk < 1.5e4) ne_ratio = 1e-2 + pow(10.0, (-51.59417133 + 12.27740153 * log10(Tk)));
+  else if (Tk >= 1.5e4 && Tk < 3.3e4) ne_ratio = pow(10.0, (-3.80749689 + 0.86092628 * log10(Tk)));
+  else ne_ratio = 1.21;
+  o[UV_NE * sv] = nH * ne_ratio;
+  const double v_th = sqrt(2.0 * C_kB * Tk / C_mp);
+  double xi;
+  if (t <= 1.0) {
+    xi = k.lx / (nH * r * r);
+    o[UV_TR * sv] = k.t_x;
+  } else {
+    xi = xi_s[cell];
+    o[UV_TR * sv] = tr_s[cell];
+  }
+  o[UV_XI * sv] = xi;
+  const double sqrt_T = sqrt(Tk), sqxi = sqrt(xi), sqsqxi = pow(xi, 0.25);
+  double p6[6];
+#pragma unroll
+  for (int q = 0; q < 6; q++) p6[q] = pre[q * sv + cell];
+  o[UV_CHP * sv] = p6[0]; o[UV_CCP * sv] = p6[1]; o[UV_XHP * sv] = p6[2];
+  o[UV_BCP * sv] = p6[3]; o[UV_LCP * sv] = p6[4]; o[UV_XIP * sv] = p6[5];
+  const double lc = ((1e-16 * exp(-1.3e5 / Tk) / sqxi / Tk) +
+                     fmin(fmin(1e-24, 5e-27 * sqrt_T), 1.5e-17 / Tk));
+  if (t < 3.0) {
+    o[UV_CH * sv] = (8.9e-36 * xi * k.t_x);
+    o[UV_CC * sv] = (8.9e-36 * xi * (4.0 * Tk));
+    o[UV_XH * sv] = (1.5e-21 * (sqsqxi / sqrt_T));
+    o[UV_LC * sv] = lc;
+    o[UV_BC * sv] = (3.3e-27 * sqrt_T);
+  } else {
+    o[UV_CH * sv] = p6[0] * (8.9e-36 * xi * k.t_x);
+    o[UV_CC * sv] = p6[1] * (8.9e-36 * xi * (4.0 * Tk));
+    o[UV_XH * sv] = p6[2] * (1.5e-21 * (sqsqxi / sqrt_T));
+    o[UV_LC * sv] = p6[4] * lc;
+    o[UV_BC * sv] = p6[3] * (3.3e-27 * sqrt_T);
+  }
+  // LineForce(Vc, grad) and the M loop over the bins
+  VGrad vg;
+  vgrad_cell(a, j, i, vg);
+  const bool tab = k.fm_points > 0;
+  const double sig_u = C_sigmaT / (C_amu * 1.18);
+  const long long sa = a.sv * a.na;
+  const double* lf = T.lf + cell;
+  double gr = 0.0, gt = 0.0;
+  double M_UV = 0.0, t_UV = 0.0, t_UV2 = 0.0, dvds2 = 0.0, M1 = -1.0, M2 = -1.0, fmax = 1e-99;
+  for (int ang = 0; ang < a.na; ang++) {
+    const double* q = lf + (long long)ang * a.sv;
+    const double dvds = vgrad_dvds(a, vg, q, sa, ang);
+    const double fr = q[LF_FR * sa], ft = q[LF_FT * sa];
+    double M = 0.0;
+    if (dvds > 0.0) {
+      const double tl = k.sigma_e * rho * v_th / dvds;
+      M = tab ? linterp_fm(log10(tl), T.fm_lt, T.fm_lm + cell, sv, k.fm_points)
+              : k.krad * pow(tl, k.alpharad);
+      if (M > 4400.) M = 4400.;
+    }
+    gr += ((1.0 + M) * k.sigma_e * fr / C_c) / k.UA;
+    gt += ((1.0 + M) * k.sigma_e * ft / C_c) / k.UA;
+    if (dvds > 0.0) {
+      t_UV = sig_u * rho * v_th / dvds;
+      M_UV = tab ? linterp_fm(log10(t_UV), T.fm_lt, T.fm_lm + cell, sv, k.fm_points)
+                 : k.krad * pow(t_UV, k.alpharad);
+      if (M_UV > 4400.) M_UV = 4400.;
+      if (M_UV > M1) M1 = M_UV;
+    }
+    if (pow(fr, 2.0) + pow(ft, 2.0) > fmax) {
+      fmax = pow(M_UV * fr, 2.0) + pow(M_UV * ft, 2.0);
+      M2 = M_UV;
+      t_UV2 = t_UV;
+      dvds2 = dvds;
+    }
+  }
+  o[UV_GR * sv] = gr;
+  o[UV_GT * sv] = gt;
+  o[UV_GP * sv] = 0.0;
+  o[UV_M1 * sv] = M1;
+  o[UV_M2 * sv] = M2;
+  o[UV_DVDS * sv] = dvds2;
+  o[UV_TT * sv] = t_UV2;
+}
+
 __global__ void heatcool_from_state_kernel(const double* V, int nr, long long sv,
                                            const double* x1, Const k, double* xi_s,
                                            double* tr_s) {
@@ -719,7 +853,7 @@ struct pmx_wind {
   double* U;
   double* xi_s;
   double* tr_s;
-  double* pre;        // [5][sv] cooling prefactors (1.0 until set)
+  double* pre;        // [6][sv] prefactors (1.0 until set); the cooling reads the first 5
   double* fm;         // tabulated force multiplier: log10 t, then log10 M; or null
   double* tab;        // one allocation for all geometry tables
   double* lf;
@@ -738,12 +872,21 @@ struct pmx_wind {
   SurrWork sw;
   CoolArgs* cargs;     // device copy of this grid's cooling arguments (1 entry)
   long long* coff;     // {0, sv}
+  bool stepped;        // a step has run: V[1] holds the last stage-2 input
+  // output snapshot: staging copy in HBM, pinned host copy, copy stream
+  hipStream_t copy_stream;
+  double* snap_dev;
+  double* snap_host;
+  hipEvent_t snap_ready, snap_done;
+  bool snap_pending;
+  int snap_nvar;
+  long snap_step;
 };
 
 namespace {
 
 constexpr double C_AH = 1.008, C_AHe = 4.004, C_AZ = 30.0;   // Src/pluto.h:296-301
-constexpr double C_G = 6.6726e-8, C_PI = 3.14159265358979, C_sigmaT = 6.6524e-25;
+constexpr double C_G = 6.6726e-8, C_PI = 3.14159265358979;
 
 int timed_begin(pmx_wind* w, hipEvent_t* e) {
   if (!w->timing) return 0;
@@ -829,6 +972,7 @@ void launch_stage(pmx_wind* w, dim3 grid, dim3 block, const StageArgs& a) {
 
 int advance(pmx_wind* w) {
   dim3 grid((w->nr + BX - 1) / BX, (w->nt + BY - 1) / BY), block(BX, BY);
+  w->stepped = true;
   hipEvent_t e[2];
   if (flag_shock(w, w->stream)) return -1;
   if (timed_begin(w, e)) return -1;
@@ -946,6 +1090,14 @@ int pmx_wind_destroy(pmx_wind* w) {
   hipFree(w->pre); hipFree(w->fm); hipFree(w->tab); hipFree(w->lf); hipFree(w->s); hipFree(w->shock); hipFree(w->Ue);
   hipFree(w->sw.list); hipFree(w->sw.fb); hipFree(w->sw.feat); hipFree(w->sw.yhat);
   hipFree(w->sw.f0); hipFree(w->sw.cnt); hipFree(w->cargs); hipFree(w->coff);
+  if (w->copy_stream) {
+    hipStreamSynchronize(w->copy_stream);
+    hipStreamDestroy(w->copy_stream);
+    hipEventDestroy(w->snap_ready);
+    hipEventDestroy(w->snap_done);
+  }
+  hipFree(w->snap_dev);
+  if (w->snap_host) hipHostFree(w->snap_host);
   if (w->stream) hipStreamDestroy(w->stream);
   delete w;
   return 0;
@@ -1104,9 +1256,9 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
   CK(hipMalloc(&w->s, sizeof(Scalars)));
   if (d->shock_flattening) CK(hipMalloc(&w->shock, (size_t)(w->nr + 4) * (w->nt + 4)));
   if (d->entropy_switch) CK(hipMalloc(&w->Ue, sizeof(double) * w->sv));
-  CK(hipMalloc(&w->pre, sizeof(double) * 5 * w->sv));
+  CK(hipMalloc(&w->pre, sizeof(double) * 6 * w->sv));
   {
-    std::vector<double> ones((size_t)5 * w->sv, 1.0);
+    std::vector<double> ones((size_t)6 * w->sv, 1.0);
     CK(hipMemcpy(w->pre, ones.data(), sizeof(double) * ones.size(), hipMemcpyHostToDevice));
   }
   if (tabulated) {   // the reader keeps log10 of t and M (line_connect.c:215, :228)
@@ -1198,15 +1350,71 @@ int pmx_wind_set_heatcool(pmx_wind* w, const double* xi, const double* t_r) {
 int pmx_wind_set_prefactors(pmx_wind* w, const double* pre) {
   if (!w) { set_error("null argument"); return -1; }
   if (pre) {
-    PMX_HIP_CHECK(hipMemcpyAsync(w->pre, pre, sizeof(double) * 5 * w->sv,
+    PMX_HIP_CHECK(hipMemcpyAsync(w->pre, pre, sizeof(double) * 6 * w->sv,
                                  hipMemcpyHostToDevice, w->stream));
     PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
   } else {
-    std::vector<double> ones((size_t)5 * w->sv, 1.0);
+    std::vector<double> ones((size_t)6 * w->sv, 1.0);
     PMX_HIP_CHECK(hipMemcpyAsync(w->pre, ones.data(), sizeof(double) * ones.size(),
                                  hipMemcpyHostToDevice, w->stream));
     PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
   }
+  return 0;
+}
+
+int pmx_wind_snapshot(pmx_wind* w, int user_vars) {
+  if (!w) { set_error("null argument"); return -1; }
+  if (user_vars && !w->stepped) {
+    set_error("user variables need a step first: the line-force quantities use the dv/ds "
+              "of the last stage");
+    return -1;
+  }
+  const size_t nsnap = (size_t)(NVW + PMX_NUV) * w->sv + 2;
+  if (!w->snap_dev) {
+    PMX_HIP_CHECK(hipStreamCreateWithFlags(&w->copy_stream, hipStreamNonBlocking));
+    PMX_HIP_CHECK(hipEventCreateWithFlags(&w->snap_ready, hipEventDisableTiming));
+    PMX_HIP_CHECK(hipEventCreateWithFlags(&w->snap_done, hipEventDisableTiming));
+    PMX_HIP_CHECK(hipMalloc(&w->snap_dev, sizeof(double) * nsnap));
+    PMX_HIP_CHECK(hipHostMalloc(&w->snap_host, sizeof(double) * nsnap));
+  } else {
+    // the staging copy is reused: the previous transfer must be out of it
+    PMX_HIP_CHECK(hipStreamWaitEvent(w->stream, w->snap_done, 0));
+  }
+  const size_t nv = NVW + (user_vars ? PMX_NUV : 0);
+  double* tail = w->snap_dev + (size_t)(NVW + PMX_NUV) * w->sv;
+  PMX_HIP_CHECK(hipMemcpyAsync(w->snap_dev, w->V[0], sizeof(double) * NVW * w->sv,
+                               hipMemcpyDeviceToDevice, w->stream));
+  if (user_vars) {
+    hipLaunchKernelGGL(wind_uservar_kernel, dim3((unsigned)((w->sv + 127) / 128)), dim3(128), 0,
+                       w->stream, stage_args(w, w->V[1], w->V[0]), w->V[0], w->xi_s, w->tr_s,
+                       w->pre, w->snap_dev + (size_t)NVW * w->sv);
+    PMX_HIP_CHECK(hipGetLastError());
+  }
+  // t and dt: the first two doubles of the device scalars
+  PMX_HIP_CHECK(hipMemcpyAsync(tail, w->s, sizeof(double) * 2, hipMemcpyDeviceToDevice,
+                               w->stream));
+  PMX_HIP_CHECK(hipEventRecord(w->snap_ready, w->stream));
+  PMX_HIP_CHECK(hipStreamWaitEvent(w->copy_stream, w->snap_ready, 0));
+  PMX_HIP_CHECK(hipMemcpyAsync(w->snap_host, w->snap_dev, sizeof(double) * nv * w->sv,
+                               hipMemcpyDeviceToHost, w->copy_stream));
+  PMX_HIP_CHECK(hipMemcpyAsync(w->snap_host + (size_t)(NVW + PMX_NUV) * w->sv, tail,
+                               sizeof(double) * 2, hipMemcpyDeviceToHost, w->copy_stream));
+  PMX_HIP_CHECK(hipEventRecord(w->snap_done, w->copy_stream));
+  w->snap_pending = true;
+  w->snap_nvar = (int)nv;
+  w->snap_step = w->step;
+  return 0;
+}
+
+int pmx_wind_snapshot_wait(pmx_wind* w, double* out, double* t, double* dt, long* step) {
+  if (!w) { set_error("null argument"); return -1; }
+  if (!w->snap_pending) { set_error("no snapshot taken"); return -1; }
+  PMX_HIP_CHECK(hipEventSynchronize(w->snap_done));
+  const double* tail = w->snap_host + (size_t)(NVW + PMX_NUV) * w->sv;
+  if (out) memcpy(out, w->snap_host, sizeof(double) * (size_t)w->snap_nvar * w->sv);
+  if (t) *t = tail[0];
+  if (dt) *dt = tail[1];
+  if (step) *step = w->snap_step;
   return 0;
 }
 
@@ -1709,6 +1917,8 @@ int pmx_wind_batch_set_shape(pmx_wind_batch* b, int shape, int blocks) {
 int pmx_wind_batch_run(pmx_wind_batch* b, int nsteps, int use_graph) {
   if (!b || nsteps < 0) { set_error("bad argument"); return -1; }
   int n = 0;
+  if (nsteps > 0)
+    for (auto* g : b->grids) g->stepped = true;
   if (use_graph && !b->timing) {
     // a captured pair of steps (even, odd) replayed
     if (b->step % 2 == 1 && n < nsteps) {

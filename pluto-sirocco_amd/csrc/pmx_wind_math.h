@@ -35,6 +35,7 @@ constexpr double C_amu = 1.66053886e-24;
 constexpr double C_c = 2.99792458e10;
 constexpr double C_kB = 1.3806505e-16;
 constexpr double C_mp = 1.67262171e-24;
+constexpr double C_sigmaT = 6.6524e-25;
 
 // Problem constants (host-evaluated with the reference's expressions).
 struct Const {

@@ -47,7 +47,8 @@ EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro
            "pmx_wind_batch_timing", "pmx_wind_batch_cells",
            "pmx_wind_set_prefactors", "pmx_sirocco_flux_angles", "pmx_sirocco_read_fluxes",
            "pmx_sirocco_fm_points", "pmx_sirocco_read_force_multiplier",
-           "pmx_sirocco_read_heatcool", "pmx_sirocco_read_prefactors"]
+           "pmx_sirocco_read_heatcool", "pmx_sirocco_read_prefactors",
+           "pmx_wind_snapshot", "pmx_wind_snapshot_wait", "pmx_write_dbl", "pmx_write_restart"]
 
 
 class PmxError(RuntimeError):
@@ -173,6 +174,13 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     lib.pmx_sirocco_read_force_multiplier.argtypes = [cp, wdp, ctypes.c_int, dp, dp, ip, lp]
     lib.pmx_sirocco_read_heatcool.argtypes = [cp, wdp, dp, dp, dp, ip, lp]
     lib.pmx_sirocco_read_prefactors.argtypes = [cp, wdp, dp, ip, lp, lp]
+    lib.pmx_wind_snapshot.argtypes = [vp, ctypes.c_int]
+    lib.pmx_wind_snapshot_wait.argtypes = [vp, dp, dp, dp, lp]
+    lib.pmx_write_dbl.argtypes = [cp, ctypes.c_int, dp, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_long]
+    lib.pmx_write_restart.argtypes = [cp, ctypes.c_int, ctypes.c_long, ip, ctypes.c_double,
+                                      ctypes.c_double]
     for name in EXPORTS:
         if name != "pmx_last_error":
             getattr(lib, name).restype = ctypes.c_int
@@ -438,6 +446,32 @@ def wind_desc(cfg: setups.WindConfig, fluxes=None, grid=None, keep=None,
     return d
 
 
+PRIM_NAMES = ("rho", "vx1", "vx2", "vx3", "prs", "tr1")   # Src/var_names.c:33-95
+
+
+def write_dbl(directory, nfile: int, data: np.ndarray, names, t: float, dt: float,
+              step: int) -> None:
+    """data.NNNN.dbl (data[nvar][nz][ny][nx], single_file) and its dbl.out
+    line, as the reference's WriteData writes them (pmx_write_dbl)."""
+    data = np.ascontiguousarray(data, dtype=np.float64)
+    nvar = len(names)
+    if data.shape[0] != nvar:
+        raise PmxError("one name per variable")
+    shape = data.shape[1:]
+    nz, ny, nx = (1,) * (3 - len(shape)) + tuple(shape)
+    cn = (ctypes.c_char_p * nvar)(*[n.encode() for n in names])
+    _check(load_library().pmx_write_dbl(str(directory).encode(), nfile, _dptr(data), nvar, cn,
+                                        nx, ny, nz, t, dt, step), "write_dbl")
+
+
+def write_restart(directory, counter: int, nstep: int, nfile, t: float, dt: float) -> None:
+    """Record `counter` of restart.out (pmx_write_restart); nfile: 16 file
+    counters, one per output type."""
+    nf = (ctypes.c_int * 16)(*[int(v) for v in nfile])
+    _check(load_library().pmx_write_restart(str(directory).encode(), counter, nstep, nf, t, dt),
+           "write_restart")
+
+
 def _grid_desc(cfg: setups.WindConfig, keep: list) -> WindDesc:
     """Descriptor with the scalars and the cell centres of a wind grid (what
     the SIROCCO file readers use)."""
@@ -586,6 +620,25 @@ class Wind:
         out = np.empty(self.cfg.shape)
         _check(self._lib.pmx_wind_get_prim(self._h, _dptr(out)), "get_prim")
         return out
+
+    def snapshot(self, user_vars: bool = True) -> None:
+        """Queue an output snapshot (V and the production user variables);
+        later steps may be queued before snapshot_wait()."""
+        self._snap_uv = bool(user_vars)
+        _check(self._lib.pmx_wind_snapshot(self._h, int(user_vars)), "snapshot")
+
+    def snapshot_wait(self) -> dict:
+        """The last snapshot: V [6][1][nt][nr], uv [23][1][nt][nr] (or None),
+        t, dt, step."""
+        nt, nr = self.cfg.nx[1], self.cfg.nx[0]
+        nv = 6 + (23 if self._snap_uv else 0)
+        out = np.empty((nv, 1, nt, nr))
+        t, dt, st = ctypes.c_double(), ctypes.c_double(), ctypes.c_long()
+        _check(self._lib.pmx_wind_snapshot_wait(self._h, _dptr(out), ctypes.byref(t),
+                                                ctypes.byref(dt), ctypes.byref(st)),
+               "snapshot_wait")
+        return {"V": out[:6], "uv": out[6:] if self._snap_uv else None, "t": t.value,
+                "dt": dt.value, "step": st.value}
 
     def get_cons(self) -> np.ndarray:
         out = np.empty(self.cfg.shape)

@@ -280,7 +280,9 @@ class WindConfig:
                              "cv_idl boundaries only")
         return {COOL_BLONDIN: "wind_blondin", COOL_PLAIN: "wind_plain"}[self.cooling]
 
-    def pluto_ini(self) -> str:
+    def pluto_ini(self, output: bool = False) -> str:
+        """pluto.ini of the reference build; ``output``: the production
+        problem's user variables and a .dbl file every step."""
         lines = ["[Grid]", "",
                  f"X1-grid    1  {self.r_range[0]!r}  {self.nx[0]}  l+  {self.r_range[1]!r}",
                  f"X2-grid    1  {self.theta_range[0]!r}  {self.nx[1]}  u  "
@@ -298,8 +300,8 @@ class WindConfig:
             lines.append(f"{lab}        {_BC_NAMES[self.bc[s]]}")
         lines += ["X3-beg        outflow", "X3-end        outflow",
                   "", "[Static Grid Output]", "",
-                  "uservar    0",
-                  "dbl       -1.0  -1   single_file",
+                  f"uservar    {len(USER_VARS)} {' '.join(USER_VARS)}" if output else "uservar    0",
+                  "dbl       -1.0   1   single_file" if output else "dbl       -1.0  -1   single_file",
                   "flt       -1.0  -1   single_file",
                   "vtk       -1.0  -1   single_file",
                   "tab       -1.0  -1", "ppm       -1.0  -1", "png       -1.0  -1",
@@ -454,6 +456,14 @@ def write_flux_files(cfg: WindConfig, directory, fluxes=None) -> None:
                 for i in range(cfg.nx[0]):
                     vals = " ".join(f"{v:.17e}" for v in arr[:, 0, j, i])
                     f.write(f"{i} {j} 0 {xs[i] * cfg.unit_length:.17e} {ts[j]:.17e} {vals}\n")
+
+
+# The production problem's user variables, in its uservar order
+# (cv_idl/pluto_sirocco_sub.py:86), as ComputeUserVar fills them
+# (cv_idl/userdef_output.c:20-260).
+USER_VARS = ("XI", "T", "T_r", "ch", "cc", "lc", "bc", "xh", "ch_pre", "cc_pre", "lc_pre",
+             "bc_pre", "xh_pre", "XI_pre", "ne", "nh", "gr", "gt", "gp", "dv_ds", "t",
+             "M_max1", "M_max2")
 
 
 @dataclasses.dataclass

@@ -126,6 +126,30 @@ def restart_cases():
                                  fm_range=(1.e-4, 1.e-2)), 10.0)
 
 
+def output_cases():
+    """Runs that write a .dbl file every step with the production problem's
+    user variables: the start-up heat-cool values (t < 1), and a restarted
+    production set-up with the tabulated force multiplier (t0 = 10)."""
+    cfg = setups.WindConfig(nx=(32, 16))
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
+    yield "wind_output_plain_32x16", cfg, V0, 6, None, None
+    cfg = dataclasses.replace(setups.cvidl_wind(32, 16, dfloor=1.e-15), first_dt=1.e-3,
+                              shock_flattening=True, char_limiting=True, entropy_switch=True,
+                              solver=setups.HLL, krad=999.0, alpharad=999.0)
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
+    yield ("wind_output_prod_32x16", cfg, V0, 6,
+           setups.sirocco_inputs(cfg, V0, fm_points=12, fm_range=(1.e-4, 1.e-2)), 10.0)
+
+
+def restart_records(b):
+    """The Restart records of restart.out (Src/structs.h:256-262): nstep,
+    nfile[16], t, dt per record (the fill bytes are not kept)."""
+    rec = np.dtype([("nstep", "<i4"), ("nfile", "<i4", 16), ("pad", "<i4"), ("t", "<f8"),
+                    ("dt", "<f8"), ("fill", "V40")])
+    r = np.frombuffer(b, dtype=rec)
+    return r["nstep"].copy(), r["nfile"].copy(), r["t"].copy(), r["dt"].copy()
+
+
 def sirocco_arrays(inp):
     """The SiroccoInputs fields as fixture arrays (prefix sir_)."""
     return {f"sir_{k}": v for k, v in dataclasses.asdict(inp).items() if v is not None}
@@ -140,7 +164,7 @@ def wind_inputs(cfg):
             "flux_r": fr, "flux_t": ft, "flux_p": fp}
 
 
-def main(which=("hydro", "wind", "cvidl", "restart")):
+def main(which=("hydro", "wind", "cvidl", "restart", "output")):
     refrun.build(with_reference=True)
     wind = list(wind_cases()) if "wind" in which else []
     wind += list(cvidl_cases()) if "cvidl" in which else []
@@ -164,6 +188,22 @@ def main(which=("hydro", "wind", "cvidl", "restart")):
                             t0=np.array(0.0 if t0 is None else t0), **inp,
                             **sirocco_arrays(sir))
         print(f"{name}: {n} steps, t={r['t']:.6e}, next dt={r['dt']:.6e}")
+    for name, cfg, V0, n, sir, t0 in (output_cases() if "output" in which else ()):
+        inp = wind_inputs(cfg)
+        fl = (inp["flux_r"], inp["flux_t"], inp["flux_p"])
+        r = refrun.run_reference(cfg, V0, n, fluxes=fl, sirocco=sir, t0=t0, output=True)
+        nvar = 6 + len(setups.USER_VARS)
+        dbl = np.frombuffer(r["dbl"][-1], dtype="<f8").reshape(nvar, cfg.nx[1], cfg.nx[0])
+        nstep, nfile, rt, rdt = restart_records(r["restart_out"])
+        meta = dataclasses.asdict(cfg)
+        extra = sirocco_arrays(sir) if sir is not None else {}
+        np.savez_compressed(OUT / f"{name}.npz", V0=V0, V=r["V"], U=r["U"],
+                            dt_seq=r["dt_seq"], t=np.array(r["t"]), dt=np.array(r["dt"]),
+                            nsteps=np.array(n), config=np.array(json.dumps(meta)),
+                            t0=np.array(0.0 if t0 is None else t0), dbl_final=dbl,
+                            dbl_out=np.array(r["dbl_out"]), rs_nstep=nstep, rs_nfile=nfile,
+                            rs_t=rt, rs_dt=rdt, **inp, **extra)
+        print(f"{name}: {len(r['dbl'])} dbl files, {len(nstep)} restart records")
     for name, cfg, V0, n in (cases() if "hydro" in which else ()):
         r = refrun.run_reference(cfg, V0, n)
         meta = dataclasses.asdict(cfg)
@@ -174,4 +214,4 @@ def main(which=("hydro", "wind", "cvidl", "restart")):
 
 
 if __name__ == "__main__":
-    main(tuple(sys.argv[1:]) or ("hydro", "wind", "cvidl", "restart"))
+    main(tuple(sys.argv[1:]) or ("hydro", "wind", "cvidl", "restart", "output"))
