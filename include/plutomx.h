@@ -294,6 +294,17 @@ int pmx_write_dbl(const char *dir, int nfile, const double *data, int nvar,
  * are written as zeros (the reference writes whatever its stack held). */
 int pmx_write_restart(const char *dir, int counter, long nstep, const int *nfile, double t,
                       double dt);
+/* RestartFromFile / RestartGet for .dbl (Src/restart.c:17-92, :195-274):
+ * the byte order of output nrestart from its dbl.out line (*swap = 1 when
+ * it is not the host's), then the first restart.out record whose nfile[0]
+ * (the .dbl slot of the output table, Src/runtime_setup.c:244-249) equals
+ * nrestart: its step number, the 16 file counters, t and dt. */
+int pmx_read_restart(const char *dir, int nrestart, long *nstep, int *nfile, double *t,
+                     double *dt, int *swap);
+/* The first nvar arrays of dir/data.NNNN.dbl (nfile NNNN), data[nvar][nx3]
+ * [nx2][nx1], byte-swapped when swap (Src/restart.c:104-128). */
+int pmx_read_dbl(const char *dir, int nfile, int nvar, int nx1, int nx2, int nx3, int swap,
+                 double *data);
 
 /* ------------------------------------------------------------------ */
 /* SIROCCO file interface (Src/LineDriven/line_connect.c:43-497): host-  */

@@ -66,7 +66,8 @@ def read_dump(path: pathlib.Path, shape) -> dict:
 
 def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
                   timeout: float = 3600.0, fluxes=None, sirocco=None,
-                  t0: float | None = None, files_dir=None, output: bool = False) -> dict:
+                  t0: float | None = None, files_dir=None, output: bool = False,
+                  restart=None) -> dict:
     """Run the reference for ``nsteps`` (>= 2) steps from ``V0``.
 
     PLUTO's ``-maxsteps N`` integrates N+1 steps (Src/main.c:233-240), so
@@ -80,7 +81,11 @@ def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
     SIROCCO files (*.dat) from there instead, as a restart.  ``output``
     (wind): the production user variables and a .dbl file every step; the
     result then holds "dbl" (the bytes of every data.NNNN.dbl, in order),
-    "dbl_out" (the text of dbl.out) and "restart_out" (its bytes)."""
+    "dbl_out" (the text of dbl.out) and "restart_out" (its bytes).
+    ``restart`` = (nrestart, nmore), with output: then restart the
+    reference from output nrestart of that run in the same directory
+    (``-restart``) for nmore steps; "restarted" holds that run's V, U, t,
+    dt and "dbl" (the files it wrote)."""
     if nsteps < 2:
         raise ValueError("the reference driver needs nsteps >= 2")
     binary = reference_binary(cfg.reference_problem())
@@ -134,6 +139,24 @@ def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
             out["dbl"] = [f.read_bytes() for f in sorted(tmp.glob("data.*.dbl"))]
             out["dbl_out"] = (tmp / "dbl.out").read_text()
             out["restart_out"] = (tmp / "restart.out").read_bytes()
+        if output and restart is not None:
+            nrestart, nmore = restart
+            rec = np.frombuffer(out["restart_out"], dtype=np.dtype(
+                [("nstep", "<i4"), ("nfile", "<i4", 16), ("pad", "<i4"), ("t", "<f8"),
+                 ("dt", "<f8"), ("fill", "V40")]))
+            s0 = int(rec["nstep"][list(rec["nfile"][:, 0]).index(nrestart)])
+            before = {f.name for f in tmp.glob("data.*.dbl")}
+            env["PMX_DUMP"] = str(tmp / "dump2.bin")
+            proc = subprocess.run([str(binary), "-i", "pluto.ini", "-restart", str(nrestart),
+                                   "-maxsteps", str(s0 + nmore - 1)], cwd=tmp, env=env,
+                                  capture_output=True, text=True, timeout=timeout)
+            if proc.returncode != 0:
+                raise RuntimeError(f"restart failed:\n{proc.stdout[-2000:]}\n{proc.stderr[-2000:]}")
+            rr = read_dump(tmp / "dump2.bin", cfg.shape)
+            rr["dbl"] = [f.read_bytes() for f in sorted(tmp.glob("data.*.dbl"))]
+            rr["new_dbl"] = sorted(f.name for f in tmp.glob("data.*.dbl") if f.name not in before)
+            rr["step0"] = s0
+            out["restarted"] = rr
         return out
 
 

@@ -103,4 +103,89 @@ int pmx_write_restart(const char* dir, int counter, long nstep, const int* nfile
   return 0;
 }
 
+int pmx_read_restart(const char* dir, int nrestart, long* nstep, int* nfile, double* t,
+                     double* dt, int* swap) {
+  if (nrestart < 0) {                           // restart.c:206-209
+    pmx::set_error("negative restart numbers are not supported (as in the reference)");
+    return -1;
+  }
+  // byte order from the dbl.out line of output nrestart, restart.c:60-83
+  const std::string lfile = path_in(dir, "dbl.out");
+  FILE* fb = std::fopen(lfile.c_str(), "r");
+  if (!fb) { pmx::set_error("cannot find " + lfile); return -1; }
+  char str[512];
+  int nlines = 0;
+  while (std::fgets(str, 512, fb) != nullptr) nlines++;
+  std::rewind(fb);
+  if (nrestart > nlines - 1) {
+    std::fclose(fb);
+    pmx::set_error("output #" + std::to_string(nrestart) + " does not exist in " + lfile);
+    return -1;
+  }
+  for (int q = nrestart; q--;) {
+    int c;
+    while ((c = std::fgetc(fb)) != '\n' && c != EOF) {}
+  }
+  int inv, ist;
+  double dd;
+  char mode[512], order[512];
+  const int got = std::fscanf(fb, "%d  %lf  %lf  %d  %511s  %511s\n", &inv, &dd, &dd, &ist,
+                              mode, order);
+  std::fclose(fb);
+  if (got != 6) { pmx::set_error("malformed line in " + lfile); return -1; }
+  const int sw = (!std::strcmp(order, "big") && little_endian()) ||
+                 (!std::strcmp(order, "little") && !little_endian());
+  // the first record whose .dbl counter is nrestart, restart.c:220-244
+  const std::string rfile = path_in(dir, "restart.out");
+  FILE* fr = std::fopen(rfile.c_str(), "rb");
+  if (!fr) { pmx::set_error("cannot find " + rfile); return -1; }
+  RestartRecord r;
+  bool found = false;
+  while (std::fread(&r, sizeof(r), 1, fr) == 1) {
+    int nf0 = r.nfile[0];
+    if (sw) nf0 = (int)__builtin_bswap32((unsigned)nf0);
+    if (nf0 == nrestart) { found = true; break; }
+  }
+  std::fclose(fr);
+  if (!found) { pmx::set_error("no record of output #" + std::to_string(nrestart)); return -1; }
+  if (sw) {
+    r.nstep = (int)__builtin_bswap32((unsigned)r.nstep);
+    for (int q = 0; q < 16; q++) r.nfile[q] = (int)__builtin_bswap32((unsigned)r.nfile[q]);
+    unsigned long long u;
+    std::memcpy(&u, &r.t, 8); u = __builtin_bswap64(u); std::memcpy(&r.t, &u, 8);
+    std::memcpy(&u, &r.dt, 8); u = __builtin_bswap64(u); std::memcpy(&r.dt, &u, 8);
+  }
+  if (nstep) *nstep = r.nstep;
+  if (nfile) for (int q = 0; q < 16; q++) nfile[q] = r.nfile[q];
+  if (t) *t = r.t;
+  if (dt) *dt = r.dt;
+  if (swap) *swap = sw;
+  return 0;
+}
+
+int pmx_read_dbl(const char* dir, int nfile, int nvar, int nx1, int nx2, int nx3, int swap,
+                 double* data) {
+  if (!data || nvar < 1 || nx1 < 1 || nx2 < 1 || nx3 < 1 || nfile < 0) {
+    pmx::set_error("pmx_read_dbl: bad argument");
+    return -1;
+  }
+  char fname[64];
+  std::snprintf(fname, sizeof(fname), "data.%04d.dbl", nfile);
+  const std::string dfile = path_in(dir, fname);
+  FILE* f = std::fopen(dfile.c_str(), "rb");
+  if (!f) { pmx::set_error("cannot open " + dfile); return -1; }
+  const size_t n = (size_t)nx1 * nx2 * nx3 * nvar;
+  const size_t got = std::fread(data, sizeof(double), n, f);
+  std::fclose(f);
+  if (got != n) { pmx::set_error("short read from " + dfile); return -1; }
+  if (swap)
+    for (size_t q = 0; q < n; q++) {
+      unsigned long long u;
+      std::memcpy(&u, &data[q], 8);
+      u = __builtin_bswap64(u);
+      std::memcpy(&data[q], &u, 8);
+    }
+  return 0;
+}
+
 }  // extern "C"

@@ -48,7 +48,8 @@ EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro
            "pmx_wind_set_prefactors", "pmx_sirocco_flux_angles", "pmx_sirocco_read_fluxes",
            "pmx_sirocco_fm_points", "pmx_sirocco_read_force_multiplier",
            "pmx_sirocco_read_heatcool", "pmx_sirocco_read_prefactors",
-           "pmx_wind_snapshot", "pmx_wind_snapshot_wait", "pmx_write_dbl", "pmx_write_restart"]
+           "pmx_wind_snapshot", "pmx_wind_snapshot_wait", "pmx_write_dbl", "pmx_write_restart",
+           "pmx_read_restart", "pmx_read_dbl"]
 
 
 class PmxError(RuntimeError):
@@ -181,6 +182,9 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
                                   ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_long]
     lib.pmx_write_restart.argtypes = [cp, ctypes.c_int, ctypes.c_long, ip, ctypes.c_double,
                                       ctypes.c_double]
+    lib.pmx_read_restart.argtypes = [cp, ctypes.c_int, lp, ip, dp, dp, ip]
+    lib.pmx_read_dbl.argtypes = [cp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_int, ctypes.c_int, dp]
     for name in EXPORTS:
         if name != "pmx_last_error":
             getattr(lib, name).restype = ctypes.c_int
@@ -470,6 +474,24 @@ def write_restart(directory, counter: int, nstep: int, nfile, t: float, dt: floa
     nf = (ctypes.c_int * 16)(*[int(v) for v in nfile])
     _check(load_library().pmx_write_restart(str(directory).encode(), counter, nstep, nf, t, dt),
            "write_restart")
+
+
+def read_restart(directory, nrestart: int, nvar: int, shape) -> dict:
+    """Restart from output nrestart as the reference does (pmx_read_restart,
+    pmx_read_dbl): its step, file counters, t, dt and the first nvar arrays
+    of its .dbl file, data[nvar][*shape]."""
+    lib = load_library()
+    st, t, dt, sw = ctypes.c_long(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    nf = (ctypes.c_int * 16)()
+    d = str(directory).encode()
+    _check(lib.pmx_read_restart(d, nrestart, ctypes.byref(st), nf, ctypes.byref(t),
+                                ctypes.byref(dt), ctypes.byref(sw)), "read_restart")
+    shape = tuple(shape)
+    nz, ny, nx = (1,) * (3 - len(shape)) + shape
+    data = np.empty((nvar,) + shape)
+    _check(lib.pmx_read_dbl(d, nf[0], nvar, nx, ny, nz, sw.value, _dptr(data)), "read_dbl")
+    return {"nstep": st.value, "nfile": list(nf), "t": t.value, "dt": dt.value,
+            "swap": sw.value, "data": data}
 
 
 def _grid_desc(cfg: setups.WindConfig, keep: list) -> WindDesc:

@@ -191,9 +191,15 @@ def main(which=("hydro", "wind", "cvidl", "restart", "output")):
     for name, cfg, V0, n, sir, t0 in (output_cases() if "output" in which else ()):
         inp = wind_inputs(cfg)
         fl = (inp["flux_r"], inp["flux_t"], inp["flux_p"])
-        r = refrun.run_reference(cfg, V0, n, fluxes=fl, sirocco=sir, t0=t0, output=True)
+        r = refrun.run_reference(cfg, V0, n, fluxes=fl, sirocco=sir, t0=t0, output=True,
+                                 restart=(3, 3))
         nvar = 6 + len(setups.USER_VARS)
         dbl = np.frombuffer(r["dbl"][-1], dtype="<f8").reshape(nvar, cfg.nx[1], cfg.nx[0])
+        dbl3 = np.frombuffer(r["dbl"][3], dtype="<f8").reshape(nvar, cfg.nx[1], cfg.nx[0])
+        # the reference restarted from output 3 (step 3) reaches the same
+        # final state as the uninterrupted run
+        rst = np.frombuffer(r["restarted"]["dbl"][-1], dtype="<f8").reshape(dbl.shape)
+        assert np.array_equal(rst, dbl), name
         nstep, nfile, rt, rdt = restart_records(r["restart_out"])
         meta = dataclasses.asdict(cfg)
         extra = sirocco_arrays(sir) if sir is not None else {}
@@ -201,6 +207,7 @@ def main(which=("hydro", "wind", "cvidl", "restart", "output")):
                             dt_seq=r["dt_seq"], t=np.array(r["t"]), dt=np.array(r["dt"]),
                             nsteps=np.array(n), config=np.array(json.dumps(meta)),
                             t0=np.array(0.0 if t0 is None else t0), dbl_final=dbl,
+                            dbl_3=dbl3[:6],
                             dbl_out=np.array(r["dbl_out"]), rs_nstep=nstep, rs_nfile=nfile,
                             rs_t=rt, rs_dt=rdt, **inp, **extra)
         print(f"{name}: {len(r['dbl'])} dbl files, {len(nstep)} restart records")

@@ -90,3 +90,34 @@ def test_user_vars_need_a_step(pmx):
             w.snapshot(user_vars=True)
         w.snapshot(user_vars=False)
         assert np.array_equal(w.snapshot_wait()["V"], g["V0"])
+
+
+@pytest.mark.parametrize("name", ["wind_output_plain_32x16", "wind_output_prod_32x16"])
+def test_restart_from_output_continues_like_the_reference(pmx, tmp_path, name):
+    """A run restarted from output 3 (read back with the restart readers
+    from files the writers produced) reaches the reference's final output:
+    the reference restarted from the same output reaches the state of its
+    uninterrupted run, which the fixture holds."""
+    cfg, g = golden_io.load(name)
+    sir, _ = golden_io.sirocco(g)
+    names = list(pmx.PRIM_NAMES)
+    for nf, (st, t, dt) in enumerate(zip(g["rs_nstep"][:4], g["rs_t"][:4], g["rs_dt"][:4])):
+        pmx.write_dbl(tmp_path, nf, g["dbl_3"][:, None], names, t, dt, int(st))
+        pmx.write_restart(tmp_path, nf, int(st), g["rs_nfile"][nf], t, dt)
+    r = pmx.read_restart(tmp_path, 3, 6, (1, cfg.nx[1], cfg.nx[0]))
+    grid = ({"x": g["x1"], "xl": g["x1l"], "xr": g["x1r"], "dx": g["dx1"]},
+            {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
+    with pmx.Wind(cfg, fluxes=(g["flux_r"], g["flux_t"], g["flux_p"]), grid=grid,
+                  sirocco=sir) as w:
+        w.set_prim(r["data"])
+        if sir is not None:
+            w.set_sirocco(sir)
+        w.set_time(r["t"], r["dt"], r["nstep"])
+        w.run(int(g["rs_nstep"][-1]) - r["nstep"])
+        w.snapshot(user_vars=True)
+        s = w.snapshot_wait()
+    dbl = g["dbl_final"]
+    for nv in range(6):
+        assert_close(s["V"][nv, 0], dbl[nv], f"V{nv}")
+    for q, nm in enumerate(setups.USER_VARS):
+        assert_close(s["uv"][q, 0], dbl[6 + q], nm)

@@ -124,3 +124,39 @@ def test_dbl_out_rewrites_from_nfile(pmx, tmp_path):
     assert lines[0].startswith("0 0.000000e+00 2.500000e-01 0 single_file little rho")
     assert lines[1].startswith("1 9.000000e+00 2.500000e-01 7 single_file little rho")
     assert lines[2].startswith("2 1.000000e+00")
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_restart_readers_round_trip(pmx, tmp_path, name):
+    """Files written by the writers read back through the restart readers:
+    output 3's record (step, t, dt) and data."""
+    cfg, g = golden_io.load(name)
+    names = list(pmx.PRIM_NAMES) + list(setups.USER_VARS)
+    for nf, (st, t, dt) in enumerate(zip(g["rs_nstep"], g["rs_t"], g["rs_dt"])):
+        data = g["dbl_final"] if nf == 5 else np.concatenate([g["dbl_3"], g["dbl_final"][6:]])
+        pmx.write_dbl(tmp_path, nf, data[:, None], names, t, dt, int(st))
+        pmx.write_restart(tmp_path, nf, int(st), g["rs_nfile"][nf], t, dt)
+    r = pmx.read_restart(tmp_path, 3, 6, (1, cfg.nx[1], cfg.nx[0]))
+    assert r["nstep"] == g["rs_nstep"][3] and r["t"] == g["rs_t"][3] and r["dt"] == g["rs_dt"][3]
+    assert r["swap"] == 0 and r["nfile"][0] == 3
+    assert np.array_equal(r["data"][:, 0], g["dbl_3"])
+    with pytest.raises(pmx.PmxError, match="does not exist"):
+        pmx.read_restart(tmp_path, 9, 6, (1, cfg.nx[1], cfg.nx[0]))
+
+
+def test_restart_readers_read_the_reference_files(pmx, tmp_path):
+    """The readers on the files the compiled reference wrote itself (skipped
+    where oracle/_ref is not built)."""
+    name = "wind_output_plain_32x16"
+    cfg, g = golden_io.load(name)
+    if not refrun.reference_binary(cfg.reference_problem()).exists():
+        pytest.skip("reference build (make -C oracle/ref) not present")
+    r = refrun.run_reference(cfg, g["V0"], int(g["nsteps"]), output=True)
+    d = tmp_path
+    for q, b in enumerate(r["dbl"]):
+        (d / f"data.{q:04d}.dbl").write_bytes(b)
+    (d / "dbl.out").write_text(r["dbl_out"])
+    (d / "restart.out").write_bytes(r["restart_out"])
+    rr = pmx.read_restart(d, 3, 6, (1, cfg.nx[1], cfg.nx[0]))
+    assert rr["nstep"] == g["rs_nstep"][3] and rr["t"] == g["rs_t"][3]
+    assert np.array_equal(rr["data"][:, 0], g["dbl_3"])
