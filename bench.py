@@ -132,6 +132,36 @@ def wind_legs(steps: int = 200) -> dict:
         "stage_ms_per_launch": round(tm["stage_ms"] / max(tm["stage_launches"], 1), 4),
         "workload": "C3 with cv_idl boundaries and floors, SHOCK_FLATTENING MULTID, "
                     f"CHAR_LIMITING, ENTROPY_SWITCH ALWAYS, HLL, {psteps} steps"}
+    # output every 5 steps (V + the 23 user variables, 30 MB): no output,
+    # the snapshot collected at once (blocking), and collected after the
+    # next 5 steps were queued (its transfer overlapping them)
+    def output_run(mode, nblk=4, per=5):
+        with plutomx.Wind(prod_cfg) as w:
+            w.set_prim(V0)
+            w.set_time(0.0, prod_cfg.first_dt, 0)
+            w.run(2)
+            w.sync()
+            t0 = time.perf_counter()
+            for _ in range(nblk):
+                if mode == "blocking":
+                    w.snapshot(user_vars=True)
+                    w.snapshot_wait()
+                    w.run(per)
+                elif mode == "overlapped":
+                    w.snapshot(user_vars=True)
+                    w.run(per)
+                    w.snapshot_wait()
+                else:
+                    w.run(per)
+            w.sync()
+            return (time.perf_counter() - t0) / (nblk * per)
+    ms = {mode: 1e3 * output_run(mode) for mode in ("none", "blocking", "overlapped")}
+    out["c3_production_output"] = {
+        "ms_per_step_no_output": round(ms["none"], 4),
+        "ms_per_step_blocking": round(ms["blocking"], 4),
+        "ms_per_step_overlapped": round(ms["overlapped"], 4),
+        "workload": "C3 production set-up, a snapshot of V + 23 user variables "
+                    "(30 MB) every 5 steps, 20 steps"}
     m = plutomx.Mlp(seed=11)
     t0 = time.perf_counter()
     m.sample(cfg, 2_000_000, seed=5)

@@ -958,15 +958,27 @@ int features(const pmx_wind* w) {
          (w->k.fm_points > 0 ? FEAT_TAB : 0);
 }
 
+// Stage 1 of the feature kernels needs the 2-wave cap (it also carries the
+// CFL reduction), and so does stage 2 with the tabulated force multiplier;
+// stage 2 of the others fits 2 waves by itself, where the cap costs ~10%
+// (measured on the production set-up: 137 vs 123 us).
+template <int STAGE, int FEAT>
+void launch_feat(pmx_wind* w, dim3 grid, dim3 block, const StageArgs& a) {
+  if (STAGE == 1 || (FEAT & FEAT_TAB))
+    hipLaunchKernelGGL((wind_stage_kernel_w<STAGE, FEAT>), grid, block, 0, w->stream, a);
+  else
+    hipLaunchKernelGGL((wind_stage_kernel<STAGE, FEAT>), grid, block, 0, w->stream, a);
+}
+
 template <int STAGE>
 void launch_stage(pmx_wind* w, dim3 grid, dim3 block, const StageArgs& a) {
   switch (features(w)) {
     case 0: hipLaunchKernelGGL((wind_stage_kernel<STAGE, 0>), grid, block, 0, w->stream, a); break;
-    case 1: hipLaunchKernelGGL((wind_stage_kernel_w<STAGE, 1>), grid, block, 0, w->stream, a); break;
-    case 2: hipLaunchKernelGGL((wind_stage_kernel_w<STAGE, 2>), grid, block, 0, w->stream, a); break;
-    case 3: hipLaunchKernelGGL((wind_stage_kernel_w<STAGE, 3>), grid, block, 0, w->stream, a); break;
+    case 1: launch_feat<STAGE, 1>(w, grid, block, a); break;
+    case 2: launch_feat<STAGE, 2>(w, grid, block, a); break;
+    case 3: launch_feat<STAGE, 3>(w, grid, block, a); break;
     // the tabulated force multiplier: one kernel with every feature
-    default: hipLaunchKernelGGL((wind_stage_kernel_w<STAGE, 7>), grid, block, 0, w->stream, a);
+    default: launch_feat<STAGE, 7>(w, grid, block, a);
   }
 }
 
