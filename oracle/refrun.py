@@ -303,7 +303,9 @@ def _wind_struct(cfg, fluxes=None, grid=None, sirocco=None):
             xi, t_r = sirocco.heatcool_floored()
             c.xi, c.t_r = arr(xi), arr(t_r)
         if sirocco.pre is not None:
-            c.pre = arr(sirocco.pre[:6])
+            pre = np.ones((6,) + tuple(np.shape(sirocco.pre)[1:]))
+            pre[:len(sirocco.pre)] = sirocco.pre[:6]
+            c.pre = arr(pre)
         if sirocco.fm_t is not None:
             c.fm_points = len(sirocco.fm_t)
             c.fm_t, c.fm_m = arr(sirocco.fm_t), arr(sirocco.fm_m)
@@ -338,9 +340,12 @@ def run_wind_oracle(cfg, V0: np.ndarray, nsteps: int, dt0: float | None = None,
 
 
 def run_wind_cool_oracle(cfg, V0: np.ndarray, dt: float, t: float = 0.0, xi=None, t_r=None,
-                         fluxes=None, grid=None) -> dict:
-    """One SplitSource (cooling) call of the restatement on V0."""
-    c, keep = _wind_struct(cfg, fluxes, grid)
+                         fluxes=None, grid=None, pre=None) -> dict:
+    """One SplitSource (cooling) call of the restatement on V0; ``pre``:
+    the cooling prefactors [>=5][nt][nr] (default 1)."""
+    import setups
+    c, keep = _wind_struct(cfg, fluxes, grid,
+                           None if pre is None else setups.SiroccoInputs(pre=np.asarray(pre)))
     V = np.ascontiguousarray(V0, dtype=np.float64).copy()
     dtc = np.array([0.0])
     xi_p = t_p = None

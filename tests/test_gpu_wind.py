@@ -263,3 +263,26 @@ def test_wind_run_is_deterministic(pmx):
         out.append((V, U, dts))
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
     assert np.array_equal(out[0][2], out[1][2])
+
+
+@pytest.mark.parametrize("dt", [1e-2, 1.0, 30.0])
+def test_brem_only_cooling_closed_forms_on_device(pmx, dt):
+    """Bremsstrahlung only (prefactors 0 except brem_c): the device's
+    sub-cycled integrator follows sqrt(E) = sqrt(E0) - B t / 2 to 1e-4, and
+    its BLONDIN step lands within 1 K of the trapezoid root
+    (closed forms in tests/test_cooling_closed_form.py)."""
+    import test_cooling_closed_form as cf
+    for cooling in (setups.COOL_PLAIN, setups.COOL_BLONDIN):
+        cfg, V, pre, kelvin = cf.brem_case(cooling)
+        with pmx.Wind(cfg) as w:
+            w.set_prim(V)
+            w.set_prefactors(pre)
+            w.set_time(10.0, dt, 0)
+            w.split_source()
+            w.sync()
+            p = w.get_prim()[setups.PRS, 0]
+        if cooling == setups.COOL_PLAIN:
+            np.testing.assert_allclose(p, cf.analytic_plain(cfg, V, dt, kelvin), rtol=1e-4)
+        else:
+            p_an, Tf = cf.analytic_blondin(cfg, V, dt)
+            assert (np.abs(p / p_an - 1.0) * Tf).max() <= 1.0
