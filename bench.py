@@ -132,9 +132,10 @@ def wind_legs(steps: int = 200) -> dict:
         "stage_ms_per_launch": round(tm["stage_ms"] / max(tm["stage_launches"], 1), 4),
         "workload": "C3 with cv_idl boundaries and floors, SHOCK_FLATTENING MULTID, "
                     f"CHAR_LIMITING, ENTROPY_SWITCH ALWAYS, HLL, {psteps} steps"}
-    # output every 5 steps (V + the 23 user variables, 30 MB): no output,
-    # the snapshot collected at once (blocking), and collected after the
-    # next 5 steps were queued (its transfer overlapping them)
+    # output every 5 steps (V + the 23 user variables, 30 MB, read in place
+    # from pinned memory): no output, the snapshot collected at once
+    # (blocking), and collected after the next 5 steps were queued (its
+    # transfer overlapping them)
     def output_run(mode, nblk=4, per=5):
         with plutomx.Wind(prod_cfg) as w:
             w.set_prim(V0)
@@ -145,12 +146,12 @@ def wind_legs(steps: int = 200) -> dict:
             for _ in range(nblk):
                 if mode == "blocking":
                     w.snapshot(user_vars=True)
-                    w.snapshot_wait()
+                    w.snapshot_wait(copy=False)
                     w.run(per)
                 elif mode == "overlapped":
                     w.snapshot(user_vars=True)
                     w.run(per)
-                    w.snapshot_wait()
+                    w.snapshot_wait(copy=False)
                 else:
                     w.run(per)
             w.sync()

@@ -276,6 +276,10 @@ int pmx_wind_snapshot(pmx_wind *w, int user_vars);
  * then the user variables); t, dt: g_time and g_dt at the snapshot; step:
  * the context's step counter. */
 int pmx_wind_snapshot_wait(pmx_wind *w, double *out, double *t, double *dt, long *step);
+/* The last snapshot in place, after pmx_wind_snapshot_wait: *data points at
+ * the pinned host copy, [6 (+ 23)][nx2][nx1] as above, valid until the next
+ * snapshot or pmx_wind_destroy (no copy; out may then be NULL above). */
+int pmx_wind_snapshot_host(pmx_wind *w, const double **data);
 
 /* WriteData for DBL_OUTPUT in single_file mode (Src/write_data.c:29-200,
  * :420-450, Src/bin_io.c:216-258): dir/data.NNNN.dbl holds the nvar arrays

@@ -1430,6 +1430,14 @@ int pmx_wind_snapshot_wait(pmx_wind* w, double* out, double* t, double* dt, long
   return 0;
 }
 
+int pmx_wind_snapshot_host(pmx_wind* w, const double** data) {
+  if (!w || !data) { set_error("null argument"); return -1; }
+  if (!w->snap_pending) { set_error("no snapshot taken"); return -1; }
+  PMX_HIP_CHECK(hipEventSynchronize(w->snap_done));
+  *data = w->snap_host;
+  return 0;
+}
+
 int pmx_wind_set_time(pmx_wind* w, double t, double dt, long step) {
   if (!w) { set_error("null argument"); return -1; }
   Scalars s0;

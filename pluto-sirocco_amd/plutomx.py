@@ -49,7 +49,7 @@ EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro
            "pmx_sirocco_fm_points", "pmx_sirocco_read_force_multiplier",
            "pmx_sirocco_read_heatcool", "pmx_sirocco_read_prefactors",
            "pmx_wind_snapshot", "pmx_wind_snapshot_wait", "pmx_write_dbl", "pmx_write_restart",
-           "pmx_read_restart", "pmx_read_dbl"]
+           "pmx_read_restart", "pmx_read_dbl", "pmx_wind_snapshot_host"]
 
 
 class PmxError(RuntimeError):
@@ -177,6 +177,7 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     lib.pmx_sirocco_read_prefactors.argtypes = [cp, wdp, dp, ip, lp, lp]
     lib.pmx_wind_snapshot.argtypes = [vp, ctypes.c_int]
     lib.pmx_wind_snapshot_wait.argtypes = [vp, dp, dp, dp, lp]
+    lib.pmx_wind_snapshot_host.argtypes = [vp, ctypes.POINTER(dp)]
     lib.pmx_write_dbl.argtypes = [cp, ctypes.c_int, dp, ctypes.c_int,
                                   ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, ctypes.c_int,
                                   ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_long]
@@ -649,16 +650,22 @@ class Wind:
         self._snap_uv = bool(user_vars)
         _check(self._lib.pmx_wind_snapshot(self._h, int(user_vars)), "snapshot")
 
-    def snapshot_wait(self) -> dict:
+    def snapshot_wait(self, copy: bool = True) -> dict:
         """The last snapshot: V [6][1][nt][nr], uv [23][1][nt][nr] (or None),
-        t, dt, step."""
+        t, dt, step.  copy=False: read-only views of the pinned host copy,
+        valid until the next snapshot of this context."""
         nt, nr = self.cfg.nx[1], self.cfg.nx[0]
         nv = 6 + (23 if self._snap_uv else 0)
-        out = np.empty((nv, 1, nt, nr))
+        out = np.empty((nv, 1, nt, nr)) if copy else None
         t, dt, st = ctypes.c_double(), ctypes.c_double(), ctypes.c_long()
-        _check(self._lib.pmx_wind_snapshot_wait(self._h, _dptr(out), ctypes.byref(t),
-                                                ctypes.byref(dt), ctypes.byref(st)),
-               "snapshot_wait")
+        _check(self._lib.pmx_wind_snapshot_wait(self._h, _dptr(out) if copy else None,
+                                                ctypes.byref(t), ctypes.byref(dt),
+                                                ctypes.byref(st)), "snapshot_wait")
+        if not copy:
+            ptr = ctypes.POINTER(ctypes.c_double)()
+            _check(self._lib.pmx_wind_snapshot_host(self._h, ctypes.byref(ptr)), "snapshot_host")
+            out = np.ctypeslib.as_array(ptr, shape=(nv, 1, nt, nr))
+            out.flags.writeable = False
         return {"V": out[:6], "uv": out[6:] if self._snap_uv else None, "t": t.value,
                 "dt": dt.value, "step": st.value}
 

@@ -79,8 +79,9 @@ def test_snapshot_is_taken_when_queued(pmx):
         assert not np.array_equal(V7, V3)
         w.snapshot(user_vars=False)
         w.run(1)
-        s2 = w.snapshot_wait()
+        s2 = w.snapshot_wait(copy=False)
         assert np.array_equal(s2["V"], V7) and s2["uv"] is None
+        assert not s2["V"].flags.writeable
 
 
 def test_user_vars_need_a_step(pmx):
