@@ -69,7 +69,28 @@ def cpu_baseline(n_sample: int = 128, steps: int = 3) -> dict | None:
                       f"{os.cpu_count()} host cores ({wall:.2f} s)"}
 
 
-def wind_legs(steps: int = 200) -> dict:
+def wind_cpu_baseline(nr: int = 256, nt: int = 128, steps: int = 4) -> dict | None:
+    """The compiled reference on a bounded sample of the C3 wind (a quarter
+    of its cells, the same physics); the plain-C restatement where the
+    reference build is absent."""
+    from oracle import refrun
+    cfg = setups.c3_wind(nr, nt, setups.COOL_PLAIN)
+    V0 = setups.wind_ic(cfg)
+    binary = refrun.reference_binary(cfg.reference_problem())
+    if binary.exists():
+        r = refrun.run_reference(cfg, V0, steps, time_only=True)
+        kind, wall, nst = "reference", r["wall"], r["steps_timed"]
+    else:
+        t0 = time.perf_counter()
+        refrun.run_wind_oracle(cfg, V0, steps)
+        kind, wall, nst = "port", time.perf_counter() - t0, steps
+    return {"value": round(cfg.ncells * nst / wall / 1e6, 4), "unit": "Mcell-updates/s",
+            "cores": 1, "kind": kind,
+            "sample": f"wind {nr}x{nt}, PLM+HLLC+RK2, line force, plain cooling, {nst} steps, "
+                      f"1 process ({wall:.2f} s)"}
+
+
+def wind_legs(steps: int = 200, cpu: bool = True) -> dict:
     """The wind workloads: C3 (512 x 256 spherical wind, plain sub-cycled
     cooling), C4 (the same with the MLP surrogate, trained here on the GPU
     on 2e6 samples of the plain integrator), the cooling step alone on
@@ -106,6 +127,8 @@ def wind_legs(steps: int = 200) -> dict:
     plain, Tp = run_wind()
     out["c3_plain"] = dict(plain, workload=f"wind 512x256, PLM+HLLC+RK2, line force, plain "
                                             f"cooling, {steps} steps")
+    if cpu:
+        out["c3_plain"]["cpu_baseline"] = wind_cpu_baseline()
     # the same wind with the production problem's set-up (cv_idl boundaries
     # and floors, MULTID shock flattening, characteristic limiting, entropy
     # switch, HLL), 20 steps: from this start state the set-up reaches a zone
@@ -243,7 +266,7 @@ def main() -> int:
 
     import plutomx
     if args.wind_only:
-        print(json.dumps({"wind": wind_legs(args.wind_steps)}), flush=True)
+        print(json.dumps({"wind": wind_legs(args.wind_steps, cpu=not args.no_cpu)}), flush=True)
         return 0
     if args.copy_probe:
         nbytes = args.copy_probe * 1 << 20
@@ -309,7 +332,7 @@ def main() -> int:
                                       "pageable host memory (not the headline value)"},
            "cpu_baseline": None if args.no_cpu else cpu_baseline()}
     if not args.no_wind:
-        out["wind"] = wind_legs(args.wind_steps)
+        out["wind"] = wind_legs(args.wind_steps, cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)
     return 0
 
