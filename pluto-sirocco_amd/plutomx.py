@@ -627,10 +627,13 @@ class Wind:
         if pre is None:
             _check(self._lib.pmx_wind_set_prefactors(self._h, None), "set_prefactors")
             return
-        pre = np.ascontiguousarray(pre, dtype=np.float64)
-        if pre.shape[0] < 5 or pre[0].size != self.cfg.nx[0] * self.cfg.nx[1]:
-            raise PmxError("prefactors must be [>=5][nt][nr]")
-        _check(self._lib.pmx_wind_set_prefactors(self._h, _dptr(pre)), "set_prefactors")
+        pre = np.asarray(pre, dtype=np.float64)
+        n = self.cfg.nx[0] * self.cfg.nx[1]
+        if pre.shape[0] not in (5, 6) or pre[0].size != n:
+            raise PmxError("prefactors must be [5 or 6][nt][nr]")
+        full = np.ones((6, n))                  # the library reads all six planes
+        full[:pre.shape[0]] = pre.reshape(pre.shape[0], n)
+        _check(self._lib.pmx_wind_set_prefactors(self._h, _dptr(full)), "set_prefactors")
 
     def set_sirocco(self, inp: "setups.SiroccoInputs") -> None:
         """The heat-cool arrays (after the reader's floors) and prefactors of
