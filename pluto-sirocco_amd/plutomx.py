@@ -261,7 +261,16 @@ class Hydro:
     # --- state -----------------------------------------------------------
     def set_prim(self, V: np.ndarray, ghosts: bool = False) -> None:
         V = np.ascontiguousarray(V, dtype=np.float64)
+        if V.size != self._prim_size(ghosts):
+            raise PmxError(f"set_prim: {V.size} values, the grid holds {self._prim_size(ghosts)}")
         _check(self._lib.pmx_hydro_set_prim(self._h, _dptr(V), int(ghosts)), "set_prim")
+
+    def _prim_size(self, ghosts: bool) -> int:
+        nv, nz, ny, nx = self.cfg.shape
+        if not ghosts:
+            return nv * nz * ny * nx
+        g = self.cfg.nghost
+        return nv * (nz + (2 * g if self.cfg.dims == 3 else 0)) * (ny + 2 * g) * (nx + 2 * g)
 
     def get_prim(self, ghosts: bool = False) -> np.ndarray:
         shape = self.cfg.shape
@@ -612,6 +621,8 @@ class Wind:
         """Upload V[6][1][nt][nr]; with init_heatcool the SIROCCO xi / T_r
         arrays are set from this state as the reference does at start-up."""
         V = np.ascontiguousarray(V, dtype=np.float64)
+        if V.size != int(np.prod(self.cfg.shape)):
+            raise PmxError(f"set_prim: {V.size} values, the grid holds {np.prod(self.cfg.shape)}")
         _check(self._lib.pmx_wind_set_prim(self._h, _dptr(V)), "set_prim")
         if init_heatcool:
             _check(self._lib.pmx_wind_set_heatcool(self._h, None, None), "set_heatcool")
@@ -619,6 +630,9 @@ class Wind:
     def set_heatcool(self, xi: np.ndarray, t_r: np.ndarray) -> None:
         xi = np.ascontiguousarray(xi, dtype=np.float64)
         t_r = np.ascontiguousarray(t_r, dtype=np.float64)
+        n = self.cfg.nx[0] * self.cfg.nx[1]
+        if xi.size != n or t_r.size != n:
+            raise PmxError(f"set_heatcool: xi and t_r need {n} values each")
         _check(self._lib.pmx_wind_set_heatcool(self._h, _dptr(xi), _dptr(t_r)), "set_heatcool")
 
     def set_prefactors(self, pre: np.ndarray | None) -> None:
