@@ -245,6 +245,11 @@ int pmx_wind_split_source(pmx_wind *w);
  * invDt / dt_cool reset and NextTimeStep every second step, all on the
  * device.  Asynchronous; pmx_wind_sync() waits. */
 int pmx_wind_run(pmx_wind *w, int nsteps);
+/* Same as pmx_wind_run, but replays a captured HIP graph of a step pair
+ * (an even step, then an odd one; built on first use, rebuilt after
+ * pmx_wind_set_surrogate).  Bit-identical to pmx_wind_run; with timing
+ * enabled it runs eagerly. */
+int pmx_wind_run_graph(pmx_wind *w, int nsteps);
 int pmx_wind_sync(pmx_wind *w);
 /* Per-kernel timing (HIP events on the context's stream) of the stage and
  * cooling kernels while enabled: summed milliseconds and launch counts. */

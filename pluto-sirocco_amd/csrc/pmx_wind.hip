@@ -889,6 +889,9 @@ struct pmx_wind {
   bool snap_pending;
   int snap_nvar;
   long snap_step;
+  // pmx_wind_run_graph: a captured (even, odd) step pair
+  hipGraphExec_t graph;
+  bool have_graph;
 };
 
 namespace {
@@ -1118,6 +1121,7 @@ int pmx_wind_destroy(pmx_wind* w) {
   }
   hipFree(w->snap_dev);
   if (w->snap_host) hipHostFree(w->snap_host);
+  if (w->have_graph) hipGraphExecDestroy(w->graph);
   if (w->stream) hipStreamDestroy(w->stream);
   delete w;
   return 0;
@@ -1501,21 +1505,58 @@ int pmx_wind_split_source(pmx_wind* w) {
   return cool(w);
 }
 
+// One iteration of the main loop for step number st (Strang order by
+// parity, NextTimeStep every second step with cooling).
+int wind_step(pmx_wind* w, long st) {
+  if (st % 2 == 0) {
+    if (advance(w) || cool(w)) return -1;
+  } else {
+    if (cool(w) || advance(w)) return -1;
+  }
+  const int next = (w->cooling == PMX_COOL_NONE) || (st % 2 == 1);
+  hipLaunchKernelGGL(wind_end_kernel, dim3(1), dim3(1), 0, w->stream, w->s, next, w->cfl,
+                     w->cfl_max_var, w->k.first_dt);
+  PMX_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
 int pmx_wind_run(pmx_wind* w, int nsteps) {
   if (!w || nsteps < 0) { set_error("bad argument"); return -1; }
   for (int n = 0; n < nsteps; n++) {
-    const long st = w->step;
-    if (st % 2 == 0) {
-      if (advance(w) || cool(w)) return -1;
-    } else {
-      if (cool(w) || advance(w)) return -1;
-    }
-    const int next = (w->cooling == PMX_COOL_NONE) || (st % 2 == 1);
-    hipLaunchKernelGGL(wind_end_kernel, dim3(1), dim3(1), 0, w->stream, w->s, next, w->cfl,
-                       w->cfl_max_var, w->k.first_dt);
-    PMX_HIP_CHECK(hipGetLastError());
+    if (wind_step(w, w->step)) return -1;
     w->step++;
   }
+  return 0;
+}
+
+int pmx_wind_run_graph(pmx_wind* w, int nsteps) {
+  if (!w || nsteps < 0) { set_error("bad argument"); return -1; }
+  if (w->timing) return pmx_wind_run(w, nsteps);
+  int n = 0;
+  if (w->step % 2 == 1 && n < nsteps) {       // align to an even step
+    if (wind_step(w, w->step)) return -1;
+    w->step++;
+    n++;
+  }
+  if (nsteps - n >= 2 && !w->have_graph) {
+    hipGraph_t gph;
+    PMX_HIP_CHECK(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = wind_step(w, 0) | wind_step(w, 1);
+    const hipError_t e = hipStreamEndCapture(w->stream, &gph);
+    if (rc || e != hipSuccess) { set_error("graph capture failed"); return -1; }
+    PMX_HIP_CHECK(hipGraphInstantiate(&w->graph, gph, nullptr, nullptr, 0));
+    PMX_HIP_CHECK(hipGraphDestroy(gph));
+    w->have_graph = true;
+  }
+  for (; nsteps - n >= 2; n += 2) {
+    PMX_HIP_CHECK(hipGraphLaunch(w->graph, w->stream));
+    w->step += 2;
+  }
+  for (; n < nsteps; n++) {
+    if (wind_step(w, w->step)) return -1;
+    w->step++;
+  }
+  if (nsteps > 0) w->stepped = true;
   return 0;
 }
 
@@ -1541,6 +1582,10 @@ int pmx_wind_set_surrogate(pmx_wind* w, pmx_mlp* m, int guess) {
   }
   w->mlp = m;
   w->sw.guess = guess;
+  if (w->have_graph) {                        // the captured step used the old cooling
+    PMX_HIP_CHECK(hipGraphExecDestroy(w->graph));
+    w->have_graph = false;
+  }
   return 0;
 }
 

@@ -49,7 +49,7 @@ EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro
            "pmx_sirocco_fm_points", "pmx_sirocco_read_force_multiplier",
            "pmx_sirocco_read_heatcool", "pmx_sirocco_read_prefactors",
            "pmx_wind_snapshot", "pmx_wind_snapshot_wait", "pmx_write_dbl", "pmx_write_restart",
-           "pmx_read_restart", "pmx_read_dbl", "pmx_wind_snapshot_host"]
+           "pmx_read_restart", "pmx_read_dbl", "pmx_wind_snapshot_host", "pmx_wind_run_graph"]
 
 
 class PmxError(RuntimeError):
@@ -134,6 +134,7 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     lib.pmx_wind_advance_step.argtypes = [vp]
     lib.pmx_wind_split_source.argtypes = [vp]
     lib.pmx_wind_run.argtypes = [vp, ctypes.c_int]
+    lib.pmx_wind_run_graph.argtypes = [vp, ctypes.c_int]
     lib.pmx_wind_sync.argtypes = [vp]
     lib.pmx_wind_timing.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ip, dp, ip]
     fp = ctypes.POINTER(ctypes.c_float)
@@ -712,8 +713,13 @@ class Wind:
     def split_source(self) -> None:
         _check(self._lib.pmx_wind_split_source(self._h), "split_source")
 
-    def run(self, nsteps: int) -> None:
-        _check(self._lib.pmx_wind_run(self._h, nsteps), "run")
+    def run(self, nsteps: int, graph: bool = False) -> None:
+        """nsteps iterations of the main loop; graph=True replays a captured
+        HIP graph of a step pair (bit-identical)."""
+        if graph:
+            _check(self._lib.pmx_wind_run_graph(self._h, nsteps), "run_graph")
+        else:
+            _check(self._lib.pmx_wind_run(self._h, nsteps), "run")
 
     def sync(self) -> None:
         _check(self._lib.pmx_wind_sync(self._h), "sync")

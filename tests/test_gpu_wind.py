@@ -286,3 +286,38 @@ def test_brem_only_cooling_closed_forms_on_device(pmx, dt):
         else:
             p_an, Tf = cf.analytic_blondin(cfg, V, dt)
             assert (np.abs(p / p_an - 1.0) * Tf).max() <= 1.0
+
+
+@pytest.mark.parametrize("kind", ["plain", "production"])
+def test_wind_graph_replay_matches_eager(pmx, kind):
+    """pmx_wind_run_graph replays a captured (even, odd) step pair: every
+    replay, with host reads and synchronisations between them, matches the
+    eager run bit for bit, from an odd start step too."""
+    import dataclasses
+    if kind == "plain":
+        cfg = setups.WindConfig(nx=(64, 32), first_dt=1e-3)
+    else:
+        cfg = dataclasses.replace(setups.cvidl_wind(64, 32, dfloor=1.e-15), first_dt=1.e-3,
+                                  shock_flattening=True, char_limiting=True,
+                                  entropy_switch=True, solver=setups.HLL)
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
+    eager = []
+    with pmx.Wind(cfg) as w:
+        w.set_prim(V0)
+        w.set_time(0.0, cfg.first_dt, 0)
+        w.run(1)
+        for _ in range(4):
+            w.run(2)
+            w.sync()
+            eager.append((w.get_prim(), w.get_time()))
+    with pmx.Wind(cfg) as w:
+        w.set_prim(V0)
+        w.set_time(0.0, cfg.first_dt, 0)
+        w.run(1, graph=True)
+        for q in range(4):
+            w.run(2, graph=True)
+            w.sync()
+            V, tm = w.get_prim(), w.get_time()
+            assert np.array_equal(V, eager[q][0]), q
+            assert (tm["t"], tm["dt"], tm["step"]) == (eager[q][1]["t"], eager[q][1]["dt"],
+                                                       eager[q][1]["step"])
