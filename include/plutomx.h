@@ -175,7 +175,10 @@ typedef struct {
    * reconstructed and advected with the mass flux (Src/adv_flux.c:131-134);
    * cons->prim takes the pressure from it (Src/HD/mappers.c:186-204). */
   int entropy_switch;
-  int solver;                   /* PMX_SOLVER_HLLC (0) or PMX_SOLVER_HLL */
+  int solver;                   /* PMX_SOLVER_HLLC (0), PMX_SOLVER_HLL or PMX_SOLVER_ROE
+                                   (Src/HD/roe.c, with its 2-D strong-shock switch and,
+                                   next to flagged zones, its HLL branch :105-131; a
+                                   batch takes HLLC and HLL grids only) */
   double first_dt;              /* as in pmx_hydro_desc (PMX_FLAG_DT_TOO_SMALL) */
   /* krad == alpharad == 999: the tabulated force multiplier of
    * M_UV_data.dat (line_connect.c:183-257, :858-887).  Per angle bin,

@@ -82,7 +82,7 @@ typedef struct {
   int    char_limiting;       /* 1: CHAR_LIMITING YES (plm_states.c:479-746)   */
   int    entropy_switch;      /* 1: ENTROPY_SWITCH ALWAYS (pressure from the
                                  advected entropy, HD/mappers.c:186-204)      */
-  int    solver;              /* 0: HLLC, 2: HLL (Src/HD/hll.c)               */
+  int    solver;              /* 0: HLLC, 1: Roe, 2: HLL (Src/HD/hll.c)       */
   /* What read_sirocco_heatcool(flag = 1) leaves on a restart
    * (line_connect.c:308-497); NULL = its start-up values:                   */
   const double *xi, *t_r;     /* [nx2][nx1] sirocco_xi, sirocco_t_r (read by

@@ -411,14 +411,48 @@ static void plm_char(double gamma, int d, const double *v, const double *dvp,
   }
 }
 
-/* HLLC with the FLAG_HLL switch of SHOCK_FLATTENING MULTID
- * (Src/HD/hllc.c:57-104): the same speeds and supersonic cases, then the
- * HLL flux for a subsonic interface. */
-static void riemann_face(double gamma, int n, int hll, const double *vL, const double *uL,
+/* Roe's HLL switch next to a shocked zone (Src/HD/roe.c:105-131): Davis
+ * speeds, bmin = min(0, SL), bmax = max(0, SR), the HLL formula for every
+ * interface. */
+static void roe_hll_face(double gamma, int n, const double *vL, const double *uL,
                          const double *vR, const double *uR, double *flux, double *press,
                          double *cmax)
 {
+  double fL[NVW], fR[NVW], aL, aR, bmin, bmax, scrh;
+  int nv;
+  orc_phys_flux(n, vL, uL, fL);
+  orc_phys_flux(n, vR, uR, fR);
+  aL = sqrt(gamma*vL[PRS]/vL[RHO]);
+  aR = sqrt(gamma*vR[PRS]/vR[RHO]);
+  bmin = MIND(vL[VX1 + n] - aL, vR[VX1 + n] - aR);         /* hll_speed.c:76-91 */
+  bmax = MAXD(vL[VX1 + n] + aL, vR[VX1 + n] + aR);
+  *cmax = MAXD(fabs(bmin), fabs(bmax));
+  bmin = MIND(0.0, bmin);
+  bmax = MAXD(0.0, bmax);
+  scrh = 1.0/(bmax - bmin);
+  for (nv = TRC; nv--; ) {
+    flux[nv] = bmin*bmax*(uR[nv] - uL[nv]) + bmax*fL[nv] - bmin*fR[nv];
+    flux[nv] *= scrh;
+  }
+  *press = (bmax*vL[PRS] - bmin*vR[PRS])*scrh;
+}
+
+/* The interface flux of the configured solver: HLLC with the FLAG_HLL
+ * switch of SHOCK_FLATTENING MULTID (Src/HD/hllc.c:57-104: the same speeds
+ * and supersonic cases, then the HLL flux for a subsonic interface; the
+ * [Solver] hll is that branch everywhere), or Roe (Src/HD/roe.c:86-345,
+ * with its own HLL branch at flagged interfaces). */
+static void riemann_face(double gamma, int n, int solver, int flagged, const double *vL,
+                         const double *uL, const double *vR, const double *uR, double *flux,
+                         double *press, double *cmax)
+{
   double mach = 0.0;
+  const int hll = solver == 2 || flagged;
+  if (solver == 1) {
+    if (flagged) roe_hll_face(gamma, n, vL, uL, vR, uR, flux, press, cmax);
+    else orc_roe(gamma, 2, n, vL, uL, vR, uR, flux, press, cmax, &mach);
+    return;
+  }
   if (!hll) {
     orc_hllc(gamma, n, vL, uL, vR, uR, flux, press, cmax, &mach);
     return;
@@ -737,8 +771,7 @@ static void update_stage(W *w, double dt)
         int f1 = d ? w->flag[CI(w, ip + 1, i)] : w->flag[CI(w, j, ip + 1)];
         p2c(gamma, vp[ip], uL);
         p2c(gamma, vm[ip + 1], uR);
-        /* HLL_Solver (Src/HD/hll.c:29-103) is HLLC's FLAG_HLL branch */
-        riemann_face(gamma, d, w->c->solver == 2 || ((f0 | f1) & FLAG_HLL), vp[ip], uL,
+        riemann_face(gamma, d, w->c->solver, ((f0 | f1) & FLAG_HLL) != 0, vp[ip], uL,
                      vm[ip + 1], uR, flux[ip], &press[ip], &cmax[ip]);
         flux[ip][TRC] = flux[ip][RHO]*(flux[ip][RHO] > 0.0 ? vp[ip][TRC] : vm[ip + 1][TRC]);
         if (ent) {                                /* adv_flux.c:131-134 */

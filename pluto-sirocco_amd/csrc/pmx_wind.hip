@@ -247,14 +247,48 @@ __device__ __forceinline__ void hll_face(const double* vL, const double* vR, dou
   }
 }
 
-// HLLC (or HLL next to a shocked zone) at one interface plus the tracer's
-// upwind flux (Src/adv_flux.c:42-52)
+// Roe's HLL switch next to a shocked zone (Src/HD/roe.c:105-131): Davis
+// speeds, bmin = min(0, SL), bmax = max(0, SR) and the HLL formula for every
+// interface, without HLLC's supersonic shortcut.
 template <int DIR>
+__device__ __forceinline__ void roe_hll_face(const double* vL, const double* vR, double gamma,
+                                             double gmm1, double* flux, double& press,
+                                             double& cmax) {
+  const int n = VX1 + DIR;
+  double uL[NV], uR[NV], fL[NV], fR[NV];
+  prim_to_cons(vL, uL, gmm1);
+  prim_to_cons(vR, uR, gmm1);
+  phys_flux<DIR>(vL, uL, fL);
+  phys_flux<DIR>(vR, uR, fR);
+  double aL = sqrt(gamma * vL[PRS] / vL[RHO]), aR = sqrt(gamma * vR[PRS] / vR[RHO]);
+  double bmin = dmin(vL[n] - aL, vR[n] - aR);
+  double bmax = dmax(vL[n] + aL, vR[n] + aR);
+  cmax = dmax(fabs(bmin), fabs(bmax));
+  bmin = dmin(0.0, bmin);
+  bmax = dmax(0.0, bmax);
+  const double scrh = 1.0 / (bmax - bmin);
+#pragma unroll
+  for (int q = 0; q < NV; q++) {
+    flux[q] = bmin * bmax * (uR[q] - uL[q]) + bmax * fL[q] - bmin * fR[q];
+    flux[q] *= scrh;
+  }
+  press = (bmax * vL[PRS] - bmin * vR[PRS]) * scrh;
+}
+
+// HLLC (or HLL next to a shocked zone), or with ROE the Roe solver, at one
+// interface plus the tracer's upwind flux (Src/adv_flux.c:42-52)
+template <int DIR, bool ROE = false>
 __device__ __forceinline__ void face(const double* vL, const double* vR, const Const& k,
                                      double* F, double& p, double& cmax, bool hll = false) {
-  // HLL_Solver (Src/HD/hll.c:29-103) is HLLC's FLAG_HLL branch
-  if (hll || k.hll) hll_face<DIR>(vL, vR, k.gamma, k.gmm1, F, p, cmax);
-  else hllc<DIR>(vL, vR, k.gamma, k.gmm1, F, p, cmax);
+  if (ROE) {
+    if (hll) roe_hll_face<DIR>(vL, vR, k.gamma, k.gmm1, F, p, cmax);
+    else roe<DIR, 2>(vL, vR, k.gamma, k.gmm1, F, p, cmax);
+  } else if (hll || k.hll) {
+    // HLL_Solver (Src/HD/hll.c:29-103) is HLLC's FLAG_HLL branch
+    hll_face<DIR>(vL, vR, k.gamma, k.gmm1, F, p, cmax);
+  } else {
+    hllc<DIR>(vL, vR, k.gamma, k.gmm1, F, p, cmax);
+  }
   F[TRC] = F[RHO] * (F[RHO] > 0.0 ? vL[TRC] : vR[TRC]);
 }
 
@@ -294,7 +328,7 @@ __device__ __forceinline__ void plm_zone(const StageArgs& a, int j, int i, const
 // switch, bit 2 the tabulated force multiplier (kernels for grids without
 // them leave them out: they cost registers); grids of a kernel built with
 // a feature still check it.
-enum { FEAT_CHAR = 1, FEAT_ENT = 2, FEAT_TAB = 4 };
+enum { FEAT_CHAR = 1, FEAT_ENT = 2, FEAT_TAB = 4, FEAT_ROE = 8 };
 
 template <int STAGE, int FEAT>
 __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, double& cdt,
@@ -380,8 +414,8 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       plm_zone<0, FEAT & FEAT_CHAR>(a, j, i - 1, w2, w1, vc, cl, vpl, vml);
       ldc(T.pr, a.nr, I + 1, cr);
       plm_zone<0, FEAT & FEAT_CHAR>(a, j, i + 1, vc, e1, e2, cr, vpr, vmr);
-      face<0>(vpl, vmi, k, Fm, pm, cm1, hll_interface(a, j, i - 1, j, i));
-      face<0>(vpi, vmr, k, Fp, pp, cp1, hll_interface(a, j, i, j, i + 1));
+      face<0, (FEAT & FEAT_ROE) != 0>(vpl, vmi, k, Fm, pm, cm1, hll_interface(a, j, i - 1, j, i));
+      face<0, (FEAT & FEAT_ROE) != 0>(vpi, vmr, k, Fp, pp, cp1, hll_interface(a, j, i, j, i + 1));
       if (ent) {
         double spl, sml, spi, smi, spr, smr;
         plm_scalar(entropy_of(w2, k.gamma), sw1, sc, cl, smm && minmod_zone(a, j, i - 1), spl, sml);
@@ -441,8 +475,8 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       plm_zone<1, FEAT & FEAT_CHAR>(a, j - 1, i, s2, s1, vc, cl, vpl, vml);
       ldc(T.pt, a.nt, J + 1, cr);
       plm_zone<1, FEAT & FEAT_CHAR>(a, j + 1, i, vc, n1, n2, cr, vpr, vmr);
-      face<1>(vpl, vmc, k, Gm, qm, cm2, hll_interface(a, j - 1, i, j, i));
-      face<1>(vpc, vmr, k, Gp, qp, cp2, hll_interface(a, j, i, j + 1, i));
+      face<1, (FEAT & FEAT_ROE) != 0>(vpl, vmc, k, Gm, qm, cm2, hll_interface(a, j - 1, i, j, i));
+      face<1, (FEAT & FEAT_ROE) != 0>(vpc, vmr, k, Gp, qp, cp2, hll_interface(a, j, i, j + 1, i));
       if (ent) {
         const double ss1 = entropy_of(s1, k.gamma), sn1 = entropy_of(n1, k.gamma);
         double spl, sml, spc, smc, spr, smr;
@@ -966,7 +1000,7 @@ int flag_shock(pmx_wind* w, hipStream_t st) {
 
 int features(const pmx_wind* w) {
   return (w->k.char_lim ? FEAT_CHAR : 0) | (w->k.ent ? FEAT_ENT : 0) |
-         (w->k.fm_points > 0 ? FEAT_TAB : 0);
+         (w->k.fm_points > 0 ? FEAT_TAB : 0) | (w->k.roe ? FEAT_ROE : 0);
 }
 
 // Stage 1 of the feature kernels needs the 2-wave cap (it also carries the
@@ -988,8 +1022,12 @@ void launch_stage(pmx_wind* w, dim3 grid, dim3 block, const StageArgs& a) {
     case 1: launch_feat<STAGE, 1>(w, grid, block, a); break;
     case 2: launch_feat<STAGE, 2>(w, grid, block, a); break;
     case 3: launch_feat<STAGE, 3>(w, grid, block, a); break;
-    // the tabulated force multiplier: one kernel with every feature
-    default: launch_feat<STAGE, 7>(w, grid, block, a);
+    case FEAT_ROE: launch_feat<STAGE, FEAT_ROE>(w, grid, block, a); break;
+    // the tabulated force multiplier, or Roe with other features: one
+    // kernel with every feature (grids still check each)
+    default:
+      if (features(w) & FEAT_ROE) launch_feat<STAGE, 15>(w, grid, block, a);
+      else launch_feat<STAGE, 7>(w, grid, block, a);
   }
 }
 
@@ -1094,6 +1132,7 @@ Const make_const(const pmx_wind_desc* d) {
   k.char_lim = d->char_limiting != 0;
   k.ent = d->entropy_switch != 0;
   k.hll = d->solver == PMX_SOLVER_HLL;
+  k.roe = d->solver == PMX_SOLVER_ROE;
   k.first_dt = d->first_dt;
   k.fm_points = (d->krad == 999 && d->alpharad == 999) ? d->fm_points : 0;
   return k;
@@ -1142,8 +1181,10 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
     return -1;
   }
   if (!(d->dfloor >= 0.0)) { set_error("dfloor must be >= 0"); return -1; }
-  if (d->solver != PMX_SOLVER_HLLC && d->solver != PMX_SOLVER_HLL) {
-    set_error("the wind's solver is PMX_SOLVER_HLLC or PMX_SOLVER_HLL"); return -1;
+  if (d->solver != PMX_SOLVER_HLLC && d->solver != PMX_SOLVER_HLL &&
+      d->solver != PMX_SOLVER_ROE) {
+    set_error("the wind's solver is PMX_SOLVER_HLLC, PMX_SOLVER_HLL or PMX_SOLVER_ROE");
+    return -1;
   }
   if (!d->x1 || !d->x1l || !d->x1r || !d->dx1 || !d->x2 || !d->x2l || !d->x2r || !d->dx2 ||
       !d->flux_r || !d->flux_t || !d->flux_p) {
@@ -1865,6 +1906,11 @@ int pmx_wind_batch_create(const pmx_wind_desc* descs, int ngrids, pmx_wind_batch
   for (int g = 0; g < ngrids; g++) {
     if (descs[g].cooling != b->cooling || descs[g].cooling == PMX_COOL_BLONDIN) {
       set_error("batched grids share one cooling mode: PMX_COOL_NONE or PMX_COOL_PLAIN");
+      pmx_wind_batch_destroy(b);
+      return -1;
+    }
+    if (descs[g].solver == PMX_SOLVER_ROE) {
+      set_error("the batch runs HLLC and HLL grids (a Roe grid runs on its own)");
       pmx_wind_batch_destroy(b);
       return -1;
     }

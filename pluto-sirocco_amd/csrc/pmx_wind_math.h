@@ -53,6 +53,7 @@ struct Const {
   int char_lim;                            // CHAR_LIMITING YES
   int ent;                                 // ENTROPY_SWITCH ALWAYS
   int hll;                                 // [Solver] hll instead of hllc
+  int roe;                                 // [Solver] roe
   double first_dt;                         // runtime->first_dt (dt-too-small stop)
   int fm_points;                           // > 0: tabulated force multiplier (krad = alpharad = 999)
 };

@@ -100,6 +100,20 @@ def cvidl_cases():
            setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7)), 8)
 
 
+def roe_cases():
+    """The wind with [Solver] roe (Src/HD/roe.c: its 2-D strong-shock switch
+    to HLL), plain and on the production set-up (Roe's own HLL branch next
+    to the shocked zones MULTID flags)."""
+    cfg = setups.WindConfig(nx=(32, 16), solver=setups.ROE)
+    yield ("wind_plain_roe_mixed_32x16", cfg,
+           setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7)), 8)
+    cfg = dataclasses.replace(setups.cvidl_wind(32, 16, dfloor=1.e-15), first_dt=1.e-3,
+                              shock_flattening=True, char_limiting=True, entropy_switch=True,
+                              solver=setups.ROE)
+    yield ("wind_cvidl_prod_roe_mixed_32x16", cfg,
+           setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7)), 8)
+
+
 def restart_cases():
     """A restarted SIROCCO cycle (t0 = 10 > 3: the cooling reads the
     files' xi and T_r, some below the reader's floors) with per-cell
@@ -164,10 +178,11 @@ def wind_inputs(cfg):
             "flux_r": fr, "flux_t": ft, "flux_p": fp}
 
 
-def main(which=("hydro", "wind", "cvidl", "restart", "output")):
+def main(which=("hydro", "wind", "cvidl", "roe", "restart", "output")):
     refrun.build(with_reference=True)
     wind = list(wind_cases()) if "wind" in which else []
     wind += list(cvidl_cases()) if "cvidl" in which else []
+    wind += list(roe_cases()) if "roe" in which else []
     for name, cfg, V0, n in wind:
         inp = wind_inputs(cfg)
         fl = (inp["flux_r"], inp["flux_t"], inp["flux_p"])
@@ -221,4 +236,4 @@ def main(which=("hydro", "wind", "cvidl", "restart", "output")):
 
 
 if __name__ == "__main__":
-    main(tuple(sys.argv[1:]) or ("hydro", "wind", "cvidl", "restart", "output"))
+    main(tuple(sys.argv[1:]) or ("hydro", "wind", "cvidl", "roe", "restart", "output"))

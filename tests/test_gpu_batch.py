@@ -156,3 +156,10 @@ def test_batch_surrogate_matches_single_grids(pmx):
             Vs, Us, ts = run_single(pmx, cfg, V0, 4, mlp=m)
             assert np.array_equal(Vb, Vs) and np.array_equal(Ub, Us)
             assert tb["dt"] == ts["dt"]
+
+
+def test_batch_refuses_roe_grids(pmx):
+    """A Roe grid runs on its own context; the batch says so."""
+    cfgs = small_batch()[:1] + [dataclasses.replace(small_batch()[1], solver=setups.ROE)]
+    with pytest.raises(pmx.PmxError, match="Roe"):
+        pmx.WindBatch(cfgs)

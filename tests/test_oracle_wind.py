@@ -57,6 +57,8 @@ COVERAGE = {
     "wind_cvidl_char_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
     "wind_cvidl_prod_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
     "wind_cvidl_prod_hll_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
+    "wind_plain_roe_mixed_32x16": ["rkf12_ok", "ck45_calls"],
+    "wind_cvidl_prod_roe_mixed_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
     "wind_plain_restart_32x16": ["rkf12_ok", "ck45_calls", "dt_cool"],
     "wind_blondin_restart_32x16": ["bl_solve", "below_minT"],
     "wind_plain_tabulated_32x16": ["rkf12_ok", "ck45_calls"],
@@ -271,3 +273,19 @@ def test_wind_mass_budget_without_sources():
     # the grid volume is the closed form (r1^3 - r0^3)/3 * (1 - cos(pi/2))
     assert math.isclose(dV.sum(), (100.0 ** 3 - 1.0) / 3.0 * (1.0 - math.cos(tr[-1])),
                         rel_tol=1e-12)
+
+
+def test_roe_fixtures_pin_the_solver():
+    """The Roe fixtures differ from the same runs with HLLC and with HLL:
+    the solver choice is visible."""
+    import dataclasses
+    for name in ("wind_plain_roe_mixed_32x16", "wind_cvidl_prod_roe_mixed_32x16"):
+        cfg, g = golden_io.load(name)
+        grid = ({"x": g["x1"], "xl": g["x1l"], "xr": g["x1r"], "dx": g["dx1"]},
+                {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
+        fl = (g["flux_r"], g["flux_t"], g["flux_p"])
+        for other in (setups.HLLC, setups.HLL):
+            r = refrun.run_wind_oracle(dataclasses.replace(cfg, solver=other), g["V0"],
+                                       int(g["nsteps"]), fluxes=fl, grid=grid)
+            assert not np.array_equal(r["V"], g["V"]), (name, other)
+
