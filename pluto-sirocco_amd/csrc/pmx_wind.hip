@@ -27,7 +27,7 @@ namespace wind {
 
 constexpr int NG = 2;          // ghost cells of the reference grid (PLM)
 constexpr int BX = 64, BY = 2; // stage-kernel workgroup
-constexpr int NLF = 7;         // line-force tables per (angle, cell)
+constexpr int NLF = 5;         // line-force tables per (angle, cell)
 constexpr int FM_POINTS_MAX = 4096;   // fit points of a tabulated force multiplier
 
 // Waves per SIMD that the batch stage kernel's register allocation targets.
@@ -38,7 +38,12 @@ constexpr int FM_POINTS_MAX = 4096;   // fit points of a tabulated force multipl
 #ifndef PMX_WIND_WAVES
 #define PMX_WIND_WAVES 2
 #endif
-enum { LF_F1, LF_F2, LF_SIN, LF_COS, LF_DS, LF_FR, LF_FT };
+// Per (angle, cell): the offset point's polar bilinear weight (-999 for a
+// bin without flux, which the reference skips), sin / cos of its polar
+// angle, and the two flux components.  The radial weight and ds are
+// recomputed per bin from the cell's geometry with the same IEEE
+// operations as the host set-up (bit-identical, 2 fewer tables to read).
+enum { LF_F2, LF_SIN, LF_COS, LF_FR, LF_FT };
 
 // Geometry and line-force tables on the device.  Radial and polar 1-D
 // arrays include the 2 ghost cells (index = interior index + 2).
@@ -152,6 +157,9 @@ __device__ __forceinline__ void ld_vel(const StageArgs& a, int jj, int ii, doubl
 struct VGrad {
   double v11_0, v11_1, v12_0, v12_1, v22_0, v22_1, v21_0, v21_1;
   double vx1, vz1;
+  // the cell's offset geometry (line_connect.c:562-607): centre (x, z),
+  // the offset length maxds and the radial corner positions
+  double x, z, maxds, x11_0, dx0;
 };
 
 __device__ __forceinline__ void vgrad_cell(const StageArgs& a, int j, int i, VGrad& g) {
@@ -181,15 +189,31 @@ __device__ __forceinline__ void vgrad_cell(const StageArgs& a, int j, int i, VGr
   const double sj = T.sin2[j + NG], cj = T.cos2[j + NG];
   g.vx1 = (ans0 * k.UV * sj + ans1 * k.UV * cj);
   g.vz1 = (ans0 * k.UV * cj - ans1 * k.UV * sj);
+  // the set-up's expressions (as in pmx_wind_create, line_connect.c:562-607)
+  const int I = i + NG, J = j + NG;
+  const double UL = k.UL;
+  const double x11_0 = (T.x1[I - 1] + T.x1[I]) / 2.0 * UL, x11_1 = (T.x2[J - 1] + T.x2[J]) / 2.0;
+  const double x22_0 = (T.x1[I + 1] + T.x1[I]) / 2.0 * UL, x22_1 = (T.x2[J + 1] + T.x2[J]) / 2.0;
+  double maxds = fabs(x22_0 - x11_0);
+  if (maxds > (T.x1[I] * UL * fabs(x22_1 - x11_1))) maxds = T.x1[I] * UL * fabs(x22_1 - x11_1);
+  maxds /= 2.0;
+  g.maxds = maxds;
+  g.x11_0 = x11_0;
+  g.dx0 = x22_0 - x11_0;
+  g.x = T.x1[I] * sj * UL;
+  g.z = T.x1[I] * cj * UL;
 }
 
 // q: the cell's line-force table entries of bin `ang`; sa: table stride.
 __device__ __forceinline__ double vgrad_dvds(const StageArgs& a, const VGrad& g,
                                              const double* q, long long sa, int ang) {
   const Const& k = a.k;
-  double ds = q[LF_DS * sa];
-  if (ds == -999) return -999;
-  double g1 = q[LF_F1 * sa], g2 = q[LF_F2 * sa], t1, t2;
+  double g2 = q[LF_F2 * sa];
+  if (g2 == -999) return -999;
+  const double ddx1 = g.maxds * a.T.sta[ang], ddx2 = g.maxds * a.T.cta[ang];
+  const double ds = sqrt(ddx1 * ddx1 + ddx2 * ddx2);
+  const double roff = sqrt((g.x + ddx1) * (g.x + ddx1) + (g.z + ddx2) * (g.z + ddx2));
+  double g1 = (roff - g.x11_0) / g.dx0, t1, t2;
   t1 = (1.0 - g1) * g.v11_0 + g1 * g.v21_0;
   t2 = (1.0 - g1) * g.v12_0 + g1 * g.v22_0;
   double b0 = (1.0 - g2) * t1 + g2 * t2;
@@ -1004,12 +1028,14 @@ int features(const pmx_wind* w) {
 }
 
 // Stage 1 of the feature kernels needs the 2-wave cap (it also carries the
-// CFL reduction), and so does stage 2 with the tabulated force multiplier;
-// stage 2 of the others fits 2 waves by itself, where the cap costs ~10%
-// (measured on the production set-up: 137 vs 123 us).
+// CFL reduction), and so do the stage-2 kernels that would otherwise take
+// accumulation registers and drop to one wave per SIMD (characteristic
+// limiting with the entropy switch, and the tabulated force multiplier);
+// stage 2 of the others fits 2 waves by itself, where the cap only adds
+// spills.
 template <int STAGE, int FEAT>
 void launch_feat(pmx_wind* w, dim3 grid, dim3 block, const StageArgs& a) {
-  if (STAGE == 1 || (FEAT & FEAT_TAB))
+  if (STAGE == 1 || (FEAT & FEAT_TAB) || (FEAT & 3) == 3)
     hipLaunchKernelGGL((wind_stage_kernel_w<STAGE, FEAT>), grid, block, 0, w->stream, a);
   else
     hipLaunchKernelGGL((wind_stage_kernel<STAGE, FEAT>), grid, block, 0, w->stream, a);
@@ -1273,20 +1299,18 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
         lf[LF_FR * sa + q] = fr;
         lf[LF_FT * sa + q] = ft;
         if (mod_flux == 0.0) {
-          lf[LF_DS * sa + q] = -999;
+          lf[LF_F2 * sa + q] = -999;
           continue;
         }
         double x = x1[I] * sin(x2[J]) * UL, z = x1[I] * cos(x2[J]) * UL;
         double ta = (a + 0.5) * (2.0 * C_PI) / 36.0;
         double ddx1 = maxds * sin(ta), ddx2 = maxds * cos(ta);
-        double ds = sqrt(ddx1 * ddx1 + ddx2 * ddx2);
-        double roff = sqrt((x + ddx1) * (x + ddx1) + (z + ddx2) * (z + ddx2));
+        // ds and the radial weight (roff - x11_0) / (x22_0 - x11_0) are
+        // recomputed per bin by the stage (vgrad_dvds), bit for bit
         double toff = atan((x + ddx1) / (z + ddx2));
-        lf[LF_F1 * sa + q] = (roff - x11_0) / (x22_0 - x11_0);
         lf[LF_F2 * sa + q] = (toff - x11_1) / (x22_1 - x11_1);
         lf[LF_SIN * sa + q] = sin(toff);
         lf[LF_COS * sa + q] = cos(toff);
-        lf[LF_DS * sa + q] = ds;
       }
 
   // pack the geometry tables into one buffer
