@@ -34,6 +34,9 @@ enum { MODE_FIRST = 0, MODE_WEIGHTED = 1, MODE_THIRD = 2 };
 #define PMX_WAVES_PER_EU 2   // occupancy the register allocation targets: 2 fits the stage
                              // without spills (~215 VGPRs) and measured 3% faster than 3
 #endif
+#ifndef PMX_MERGE_XY
+#define PMX_MERGE_XY 1     // 3-D: x and y sweeps share phases, 4 barriers per plane instead of 6
+#endif
 #ifndef PMX_PREFETCH_U
 #define PMX_PREFETCH_U 0   // load U and V0 of the cell at the top of the plane loop
 #endif
@@ -150,7 +153,13 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
   // LDS arrays are component-major so that consecutive lanes touch
   // consecutive doubles (no bank conflicts).
   __shared__ double sPlane[NPLANE];    // V plane [nv][LH][LW]; later y-fluxes [7][NFY]
-  __shared__ double sSt[NST * 10];     // states [vp(5) vm(5)][item]
+  // states [vp(5) vm(5)][item]; x and y each have their own when the two
+  // directions' phases share barriers (3-D: 72 KB, still 2 workgroups per
+  // CU; the 2-D kernel keeps the smaller footprint)
+  constexpr bool MERGE = PMX_MERGE_XY && DIMS == 3;
+  __shared__ double sSt[(MERGE ? 2 : 1) * NST * 10];
+  double* const sStX = sSt;
+  double* const sStY = sSt + (MERGE ? NST * 10 : 0);
   __shared__ double sFx[NFX * 7];      // x-fluxes [flux(5) press cmax][face]
   __shared__ double sRed[NT / 64];
 
@@ -266,8 +275,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
     }
     __syncthreads();
 
-    // ---- P1: x-states of cells -1 .. TX (per row) ----------------------
-    for (int it = tid; it < NSX; it += NT) {
+    // x-states of cells -1 .. TX (per row)
+    auto x_state = [&](int it) {
       int r = it / (TX + 2), c = it % (TX + 2) - 1;
       double s[5][NV];
 #pragma unroll
@@ -281,30 +290,26 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
       cell_states<REC>(s, vp, vm);
 #pragma unroll
       for (int nv = 0; nv < NV; nv++) {
-        sSt[nv * NST + it] = vp[nv];
-        sSt[(5 + nv) * NST + it] = vm[nv];
+        sStX[nv * NST + it] = vp[nv];
+        sStX[(5 + nv) * NST + it] = vm[nv];
       }
-    }
-    __syncthreads();
-
-    // ---- P2: x-faces ----------------------------------------------------
-    for (int f = tid; f < NFX; f += NT) {
+    };
+    // x-faces
+    auto x_face = [&](int f) {
       int r = f / (TX + 1), fi = f % (TX + 1);  // face between cells fi-1 and fi
       int sl = r * (TX + 2) + fi, sr = sl + 1;
       double vL[NV], vR[NV], F[7];
 #pragma unroll
       for (int nv = 0; nv < NV; nv++) {
-        vL[nv] = sSt[nv * NST + sl];
-        vR[nv] = sSt[(5 + nv) * NST + sr];
+        vL[nv] = sStX[nv * NST + sl];
+        vR[nv] = sStX[(5 + nv) * NST + sr];
       }
       riemann<SOLVER, 0, DIMS>(vL, vR, gamma, gmm1, F, F[5], F[6]);
 #pragma unroll
       for (int q = 0; q < 7; q++) sFx[q * NFX + f] = F[q];
-    }
-    __syncthreads();
-
-    // ---- P3: y-states of rows -1 .. TY (per column) ---------------------
-    for (int it = tid; it < NSY; it += NT) {
+    };
+    // y-states of rows -1 .. TY (per column)
+    auto y_state = [&](int it) {
       int c = it % TX, r = it / TX - 1;
       double s[5][NV];
 #pragma unroll
@@ -318,27 +323,48 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
       cell_states<REC>(s, vp, vm);
 #pragma unroll
       for (int nv = 0; nv < NV; nv++) {
-        sSt[nv * NST + it] = vp[nv];
-        sSt[(5 + nv) * NST + it] = vm[nv];
+        sStY[nv * NST + it] = vp[nv];
+        sStY[(5 + nv) * NST + it] = vm[nv];
       }
-    }
-    __syncthreads();
-
-    // ---- P4: y-faces (fluxes overwrite the plane buffer) ---------------
-    for (int f = tid; f < NFY; f += NT) {
+    };
+    // y-faces (the fluxes overwrite the plane buffer, whose reads are done)
+    auto y_face = [&](int f) {
       int c = f % TX, fj = f / TX;  // face between rows fj-1 and fj
       int sl = fj * TX + c, sr = sl + TX;
       double vL[NV], vR[NV], F[7];
 #pragma unroll
       for (int nv = 0; nv < NV; nv++) {
-        vL[nv] = sSt[nv * NST + sl];
-        vR[nv] = sSt[(5 + nv) * NST + sr];
+        vL[nv] = sStY[nv * NST + sl];
+        vR[nv] = sStY[(5 + nv) * NST + sr];
       }
       riemann<SOLVER, 1, DIMS>(vL, vR, gamma, gmm1, F, F[5], F[6]);
 #pragma unroll
       for (int q = 0; q < 7; q++) sPlane[q * NFY + f] = F[q];
+    };
+
+    if constexpr (MERGE) {
+    // ---- P1: x- and y-states; P2: x- and y-faces (two barriers) -------
+    for (int it = tid; it < NST; it += NT) {
+      if (it < NSX) x_state(it);
+      if (it < NSY) y_state(it);
     }
     __syncthreads();
+    for (int f = tid; f < (NFX > NFY ? NFX : NFY); f += NT) {
+      if (f < NFX) x_face(f);
+      if (f < NFY) y_face(f);
+    }
+    __syncthreads();
+    } else {
+    // ---- P1: x-states; P2: x-faces; P3: y-states; P4: y-faces ----------
+    for (int it = tid; it < NSX; it += NT) x_state(it);
+    __syncthreads();
+    for (int f = tid; f < NFX; f += NT) x_face(f);
+    __syncthreads();
+    for (int it = tid; it < NSY; it += NT) y_state(it);
+    __syncthreads();
+    for (int f = tid; f < NFY; f += NT) y_face(f);
+    __syncthreads();
+    }
 
     // ---- P5: z-face, update, RK combination, cons->prim ---------------
     if (owner) {
