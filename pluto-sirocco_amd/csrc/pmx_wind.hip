@@ -12,6 +12,7 @@
 //   wind_end_kernel       time update and NextTimeStep (every second step
 //                         with cooling, as the reference).
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -159,7 +160,7 @@ struct VGrad {
   double vx1, vz1;
   // the cell's offset geometry (line_connect.c:562-607): centre (x, z),
   // the offset length maxds and the radial corner positions
-  double x, z, maxds, x11_0, dx0;
+  double x, z, maxds, x11_0, dx0, rdx0;   // rdx0 = 1 / dx0 for div_rc
 };
 
 __device__ __forceinline__ void vgrad_cell(const StageArgs& a, int j, int i, VGrad& g) {
@@ -200,11 +201,13 @@ __device__ __forceinline__ void vgrad_cell(const StageArgs& a, int j, int i, VGr
   g.maxds = maxds;
   g.x11_0 = x11_0;
   g.dx0 = x22_0 - x11_0;
+  g.rdx0 = 1.0 / g.dx0;
   g.x = T.x1[I] * sj * UL;
   g.z = T.x1[I] * cj * UL;
 }
 
 // q: the cell's line-force table entries of bin `ang`; sa: table stride.
+template <bool FAST = false>
 __device__ __forceinline__ double vgrad_dvds(const StageArgs& a, const VGrad& g,
                                              const double* q, long long sa, int ang) {
   const Const& k = a.k;
@@ -213,7 +216,7 @@ __device__ __forceinline__ double vgrad_dvds(const StageArgs& a, const VGrad& g,
   const double ddx1 = g.maxds * a.T.sta[ang], ddx2 = g.maxds * a.T.cta[ang];
   const double ds = sqrt(ddx1 * ddx1 + ddx2 * ddx2);
   const double roff = sqrt((g.x + ddx1) * (g.x + ddx1) + (g.z + ddx2) * (g.z + ddx2));
-  double g1 = (roff - g.x11_0) / g.dx0, t1, t2;
+  double g1 = qdiv<FAST>(roff - g.x11_0, g.dx0, g.rdx0), t1, t2;
   t1 = (1.0 - g1) * g.v11_0 + g1 * g.v21_0;
   t2 = (1.0 - g1) * g.v12_0 + g1 * g.v22_0;
   double b0 = (1.0 - g2) * t1 + g2 * t2;
@@ -402,6 +405,11 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       if (!tab) { A1 = force_factor(k, rho1, vth1); A2 = force_factor(k, rho2, vth2); }
       const long long sa = a.sv * a.na;
       const double* lf = T.lf + cell;
+      const double rc_c = 1.0 / C_c, r_ua = 1.0 / k.UA;
+      // the bin loop in two copies, chosen once per run (k.fast_div): its
+      // quotients through div_rc or through the division
+      auto bins = [&](auto fast) {
+        constexpr bool FAST = decltype(fast)::value;
 // two bins at a time give the scheduler two independent chains (C3: one
 // wave per slot, nothing else hides latency); 0.097 -> 0.093 ms per stage,
 // C5 14.2 -> 13.6 ms per step; four cost the feature kernels registers.
@@ -412,7 +420,7 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
 #pragma unroll PMX_ANG_UNROLL
       for (int ang = 0; ang < a.na; ang++) {
         const double* q = lf + (long long)ang * a.sv;
-        const double dvds = vgrad_dvds(a, vg, q, sa, ang);
+        const double dvds = vgrad_dvds<FAST>(a, vg, q, sa, ang);
         double fr = q[LF_FR * sa], ft = q[LF_FT * sa];
         double fm1, fm2;
         if (tab) {   // both calls read the fit of this cell (line_connect.c:858-862)
@@ -421,9 +429,13 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
         } else {
           force_mult_pair(k, A1, A2, dvds, fm1, fm2);
         }
-        g1r += ((fm1 * k.sigma_e * fr / C_c)) / k.UA;
-        g2t += ((fm2 * k.sigma_e * ft / C_c)) / k.UA;
+        // ((fm sigma_e f) / c) / UA, each quotient bit-identical
+        g1r += qdiv<FAST>(qdiv<FAST>(fm1 * k.sigma_e * fr, C_c, rc_c), k.UA, r_ua);
+        g2t += qdiv<FAST>(qdiv<FAST>(fm2 * k.sigma_e * ft, C_c, rc_c), k.UA, r_ua);
       }
+      };
+      if (k.fast_div) bins(std::true_type{});
+      else bins(std::false_type{});
     }
 
     // ---- x1 faces i-1/2 and i+1/2 --------------------------------------
@@ -1161,7 +1173,21 @@ Const make_const(const pmx_wind_desc* d) {
   k.roe = d->solver == PMX_SOLVER_ROE;
   k.first_dt = d->first_dt;
   k.fm_points = (d->krad == 999 && d->alpharad == 999) ? d->fm_points : 0;
+  k.fast_div = 0;   // set with the line-force tables
   return k;
+}
+
+// div_rc against the division on the device: the count of x whose two
+// quotients differ in any bit.
+__global__ void div_check_kernel(const double* __restrict__ x, long long n, double c, double rc,
+                                 unsigned long long* bad) {
+  unsigned long long mine = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double q = div_rc(x[i], c, rc), r = x[i] / c;
+    if (__double_as_longlong(q) != __double_as_longlong(r)) mine++;
+  }
+  if (mine) atomicAdd(bad, mine);
 }
 }  // namespace wind
 }  // namespace pmx
@@ -1284,6 +1310,20 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
     f2c[j] = (x2[J] - x11_1) / (x22_1 - x11_1);
   }
   const size_t sa = (size_t)na * w->sv;
+  // div_rc's ranges hold for every line-force quotient when the fluxes are
+  // 0 or of magnitude 2^-600..2^600 (the dividend is fm sigma_e f with fm
+  // in 1..4401), UA lies in 2^-100..2^100 and the radii x1 UL in
+  // 2^-200..2^200 (the radial weight's dividend and divisor are differences
+  // of such radii, 0 or at least an ulp of them)
+  auto div_rc_flux_ok = [](double f) {
+    const double af = fabs(f);
+    return af == 0.0 || (af >= 0x1p-600 && af <= 0x1p600);
+  };
+  bool fast_div = k.UA >= 0x1p-100 && k.UA <= 0x1p100;
+  for (int I = 0; I < nr + 2 * NG; I++) {
+    const double r = x1[I] * UL;
+    if (!(r >= 0x1p-200 && r <= 0x1p200)) fast_div = false;
+  }
   for (int a = 0; a < na; a++)
     for (int j = 0; j < nt; j++)
       for (int i = 0; i < nr; i++) {
@@ -1295,6 +1335,7 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
         if (maxds > (x1[I] * UL * fabs(x22_1 - x11_1))) maxds = x1[I] * UL * fabs(x22_1 - x11_1);
         maxds /= 2.0;
         double fr = d->flux_r[q], ft = d->flux_t[q], fp = d->flux_p[q];
+        if (!div_rc_flux_ok(fr) || !div_rc_flux_ok(ft)) fast_div = false;
         double mod_flux = sqrt(pow(fr, 2) + pow(ft, 2) + pow(fp, 2));
         lf[LF_FR * sa + q] = fr;
         lf[LF_FT * sa + q] = ft;
@@ -1312,6 +1353,7 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
         lf[LF_SIN * sa + q] = sin(toff);
         lf[LF_COS * sa + q] = cos(toff);
       }
+  w->k.fast_div = fast_div ? 1 : 0;
 
   // pack the geometry tables into one buffer
   std::vector<std::pair<const double**, const std::vector<double>*>> parts = {
@@ -2129,6 +2171,33 @@ int pmx_wind_batch_timing(pmx_wind_batch* b, int enable, int reset, double* stag
 int pmx_wind_batch_cells(pmx_wind_batch* b, long long* cells) {
   if (!b || !cells) { set_error("bad argument"); return -1; }
   *cells = b->cells;
+  return 0;
+}
+
+int pmx_check_div_rc(const double* x, long long n, double c, long long* mismatches) {
+  if (!x || n <= 0 || !(c > 0.0) || !mismatches) { set_error("bad argument"); return -1; }
+  double* d = nullptr;
+  unsigned long long* bad = nullptr;
+  PMX_HIP_CHECK(hipMalloc(&d, sizeof(double) * n));
+  if (hipMalloc(&bad, sizeof(unsigned long long)) != hipSuccess) {
+    hipFree(d);
+    set_error("hipMalloc");
+    return -1;
+  }
+  unsigned long long h = 0;
+  hipError_t e = hipMemcpy(d, x, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(bad, &h, sizeof(h), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const long long nb = std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(pmx::wind::div_check_kernel, dim3((unsigned)nb), dim3(256), 0, 0, d, n, c,
+                       1.0 / c, bad);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(&h, bad, sizeof(h), hipMemcpyDeviceToHost);
+  hipFree(d);
+  hipFree(bad);
+  if (e != hipSuccess) { set_error(hipGetErrorString(e)); return -1; }
+  *mismatches = (long long)h;
   return 0;
 }
 

@@ -54,6 +54,7 @@ struct Const {
   int ent;                                 // ENTROPY_SWITCH ALWAYS
   int hll;                                 // [Solver] hll instead of hllc
   int roe;                                 // [Solver] roe
+  int fast_div;                            // line-force quotients through div_rc
   double first_dt;                         // runtime->first_dt (dt-too-small stop)
   int fm_points;                           // > 0: tabulated force multiplier (krad = alpharad = 999)
 };
@@ -276,6 +277,29 @@ PMX_HD void plm_char_general(const double* vm1, const double* v0, const double* 
 // several times the cost.  The product differs from pow(t, alpha) by a few
 // 1e-15 relative in all; with krad = 0 both forms give M = 0 exactly.
 // ---------------------------------------------------------------------
+// x / c for c > 0, given rc = 1 / c correctly rounded, bit-identical to
+// the IEEE quotient when x = 0 or |x| and |x / c| lie in [2^-900, 2^900]:
+// q = x rc and two corrections q += (x - c q) rc, the residual exact in an
+// FMA.  By Markstein's theorem (rc correctly rounded, q within an ulp of
+// x / c, no underflow or overflow) the last correction gives the correctly
+// rounded quotient; the first brings q within an ulp in every case.  Five
+// FP64 instructions and no branch, where the division takes about eleven.
+PMX_HD double div_rc(double x, double c, double rc) {
+  double q = x * rc;
+  q = fma(fma(-q, c, x), rc, q);
+  q = fma(fma(-q, c, x), rc, q);
+  return copysign(q, x);   // +-0 keeps the sign of x, as x / c does for c > 0
+}
+
+// The line force's quotients: div_rc where the run's inputs keep every
+// operand inside its range (FAST, decided once per run on the host), the
+// division otherwise.
+template <bool FAST>
+PMX_HD double qdiv(double x, double c, double rc) {
+  if constexpr (FAST) return div_rc(x, c, rc);
+  else return x / c;
+}
+
 PMX_HD double force_factor(const Const& k, double rho_cgs, double v_th) {
   return pow(k.sigma_e * rho_cgs * v_th, k.alpharad);
 }

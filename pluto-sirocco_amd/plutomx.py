@@ -32,7 +32,7 @@ EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro
            "pmx_hydro_get_prim", "pmx_hydro_get_cons", "pmx_hydro_set_time",
            "pmx_hydro_get_time", "pmx_hydro_advance_step", "pmx_hydro_run",
            "pmx_hydro_sync", "pmx_hydro_run_graph", "pmx_hydro_stage_timing",
-           "pmx_hydro_device_state", "pmx_copy_probe",
+           "pmx_hydro_device_state", "pmx_copy_probe", "pmx_check_div_rc",
            "pmx_wind_create", "pmx_wind_destroy", "pmx_wind_set_prim", "pmx_wind_get_prim",
            "pmx_wind_get_cons", "pmx_wind_set_heatcool", "pmx_wind_set_time",
            "pmx_wind_get_time", "pmx_wind_set_dts", "pmx_wind_advance_step", "pmx_wind_split_source",
@@ -121,6 +121,8 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     lib.pmx_hydro_stage_timing.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ip]
     lib.pmx_hydro_device_state.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
     lib.pmx_copy_probe.argtypes = [ctypes.c_longlong, ctypes.c_int, dp]
+    lib.pmx_check_div_rc.argtypes = [dp, ctypes.c_longlong, ctypes.c_double,
+                                      ctypes.POINTER(ctypes.c_longlong)]
     lp = ctypes.POINTER(ctypes.c_long)
     lib.pmx_wind_create.argtypes = [ctypes.POINTER(WindDesc), ctypes.POINTER(vp)]
     lib.pmx_wind_destroy.argtypes = [vp]
@@ -210,6 +212,16 @@ def cell_widths(cfg: setups.HydroConfig, d: int) -> np.ndarray:
     if d >= cfg.dims:
         return np.ones(1)
     return np.full(n, (cfg.xend[d] - cfg.xbeg[d]) / float(n))
+
+
+def check_div_rc(x: np.ndarray, c: float) -> int:
+    """Number of x whose device quotient x / c from the reciprocal-and-FMA
+    evaluation differs in any bit from the device division."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    bad = ctypes.c_longlong()
+    _check(load_library().pmx_check_div_rc(_dptr(x), x.size, float(c), ctypes.byref(bad)),
+           "check_div_rc")
+    return bad.value
 
 
 def copy_probe(nbytes: int, reps: int = 10) -> float:

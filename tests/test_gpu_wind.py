@@ -92,6 +92,54 @@ def test_wind_bitwise_without_force_multiplier(pmx):
     assert np.array_equal(dts, o["dt_seq"]) and tm["dt"] == o["dt"]
 
 
+def test_reciprocal_division_is_bit_identical(pmx):
+    """The line force divides by c, by the run's unit acceleration and by
+    each cell's radial corner spacing through a precomputed reciprocal and
+    two FMA corrections (where the run's fluxes, units and radii keep every
+    operand in range).  On the device that evaluation must equal the IEEE
+    quotient bit for bit: x of both signs spread over 2^-300..2^300, signed
+    zeros, exact multiples of c and their neighbours, and values near the
+    ends of the range."""
+    rng = np.random.default_rng(7)
+    n = 1 << 21
+    m = rng.uniform(1.0, 2.0, n) * np.exp2(rng.integers(-300, 301, n).astype(float))
+    x = np.where(rng.random(n) < 0.5, -m, m)
+    x[:500] = 0.0
+    x[500:1000] = -0.0
+    cfg = setups.WindConfig(nx=(64, 32))
+    ua = cfg.unit_velocity ** 2 / cfg.unit_length
+    gr, _ = setups.wind_grid(cfg)
+    xl = gr["x"] * cfg.unit_length
+    dx0 = (xl[2:] - xl[:-2]) / 2.0              # corner spacing of each cell
+    consts = [2.99792458e10, ua, 1.0 / 3.0, 3.0, 0.1] + list(rng.choice(dx0, 4))
+    for c in consts:
+        assert 2.0 ** -45 < c < 2.0 ** 45
+        k = rng.integers(1, 1 << 40, 1 << 16).astype(float)
+        mult = k * c
+        e = rng.uniform(1.0, 2.0, 4096)
+        edges = np.concatenate([e * 2.0 ** 850, e * 2.0 ** -850, c * e * 2.0 ** 850,
+                                c * e * 2.0 ** -850])
+        edges = np.concatenate([edges, -edges])
+        xs = np.concatenate([x, mult, np.nextafter(mult, np.inf), np.nextafter(mult, -np.inf),
+                             edges])
+        assert pmx.check_div_rc(xs, c) == 0, c
+
+
+def test_reciprocal_division_is_chosen_per_run(pmx):
+    """A run whose fluxes hold a value outside 2^-600..2^600 (its line
+    force then takes the division) matches the oracle bit for bit, like the
+    runs that take the reciprocal."""
+    import dataclasses
+    cfg = dataclasses.replace(setups.WindConfig(nx=(32, 16), cooling=setups.COOL_NONE,
+                                                first_dt=1e-3), krad=0.0)
+    V0 = setups.wind_ic_mixed(cfg, rho0_cgs=1e-11)
+    fr, ft, fp = (np.array(f, copy=True) for f in setups.wind_fluxes(cfg))
+    fr[3, 0, 5, 7] = 1e-250
+    o = refrun.run_wind_oracle(cfg, V0, 4, fluxes=(fr, ft, fp))
+    V, U, dts, tm = gpu_run(pmx, cfg, V0, 4, fluxes=(fr, ft, fp))
+    assert np.array_equal(V, o["V"]) and np.array_equal(U, o["U"])
+
+
 @pytest.mark.parametrize("cooling", [setups.COOL_BLONDIN, setups.COOL_PLAIN])
 def test_wind_c3_grid_matches_oracle(pmx, cooling):
     """The 512 x 256 wind grid of the benchmark workload, 4 steps."""
