@@ -476,7 +476,7 @@ int pmx_copy_probe(long long nbytes, int reps, double *ms_per_copy);
 
 /* Arithmetic check (no counterpart in the reference): the line force's
  * divisions by per-cell and per-run constants (Src/LineDriven/
- * line_connect.c:762-766 and :898-899) are evaluated from a reciprocal with
+ * line_connect.c:753 and :899-900) are evaluated from a reciprocal with
  * FMA corrections where the run's inputs keep the operands in range; this
  * runs that evaluation and the plain division of each of the n values x
  * (0, or |x| and |x / c| within 2^-900..2^900) by c > 0 on the device and
