@@ -60,10 +60,25 @@ typedef struct {
                             reported as PMX_FLAG_DT_TOO_SMALL */
 } pmx_hydro_desc;
 
-/* Set in the flags that *_get_time return, and kept, once NextTimeStep has
- * produced a dt below 1e-9 first_dt: the point where the reference prints
- * "dt is too small" and quits.  The state after it is not meaningful. */
+/* Flags that *_get_time return.  The run-stopping ones (DT_TOO_SMALL,
+ * COOL_SUBSTEPS, TSTOP) are sticky and halt the main loop: once one is set,
+ * further steps of *_run / *_run_graph / pmx_wind_batch_run do nothing (t,
+ * dt and the state stay where the run stopped) until *_set_time is called.
+ * The single-step entries (*_advance_step, pmx_wind_split_source), which a
+ * driver calls with its own dt, are not halted. */
+/* NextTimeStep produced a dt below 1e-9 first_dt: the point where the
+ * reference prints "dt is too small" and quits (Src/main.c:672-683).  The
+ * state after it is not meaningful. */
 #define PMX_FLAG_DT_TOO_SMALL 4096
+/* The step that reached tstop has been taken: the reference's main loop
+ * cuts that step's dt to tstop - t (Src/main.c:226-229) and ends. */
+#define PMX_FLAG_TSTOP 8192
+/* A cell's Cash-Karp cooling integration exceeded 1000 sub-steps: the
+ * reference prints "! SolveODE_CK45: too many steps" and quits
+ * (Src/Cooling/cooling_ode_solver.c:190-192), also raised when a cell's
+ * integration does not end (the device caps every loop).  The run halts
+ * after the step in which it happened; that step's state is not meaningful. */
+#define PMX_FLAG_COOL_SUBSTEPS 256
 
 const char *pmx_last_error(void);
 
@@ -87,12 +102,21 @@ int pmx_hydro_get_cons(pmx_hydro *h, double *U, int pluto_layout);
 int pmx_hydro_set_time(pmx_hydro *h, double t, double dt);
 int pmx_hydro_get_time(pmx_hydro *h, double *t, double *dt, double *inv_dt_hyp,
                        int *c2p_flags);
+/* runtime.tstop (pluto.ini [Time] tstop; default: none).  The main loop
+ * (pmx_hydro_run / _run_graph) cuts the dt of the step that would reach
+ * tstop to tstop - t, as the reference does at the top of each step
+ * (Src/main.c:226-229), and halts after it with PMX_FLAG_TSTOP.  Also
+ * clears PMX_FLAG_TSTOP, so that a halted run continues to a later tstop.
+ * set_time applies the cut to its (t, dt) at once; get_time then reports
+ * the cut dt. */
+int pmx_hydro_set_tstop(pmx_hydro *h, double tstop);
 
 /* One AdvanceStep(d, Dts, grid) of the reference (Src/Time_Stepping/
  * rk_step.c:29, prototype Src/prototypes.h:5) with time step dt:
- * boundaries, all RK stages, cons->prim.  Returns the stage-1 inverse
- * hyperbolic time step Dts->invDt_hyp (update_stage.c:389-394).
- * Synchronises the stream. */
+ * boundaries, all RK stages, cons->prim.  *inv_dt_hyp is Dts->invDt_hyp:
+ * on entry the value the stage-1 maximum of C_dt folds into, on return
+ * MAX(entry, max C_dt) / DIMENSIONS (update_stage.c:389-394); NULL: 0 on
+ * entry, nothing returned.  Synchronises the stream. */
 int pmx_hydro_advance_step(pmx_hydro *h, double dt, double *inv_dt_hyp);
 
 /* nsteps iterations of the reference's main loop body for this path
@@ -145,7 +169,8 @@ typedef struct {
   int cooling;                  /* PMX_COOL_*                           */
   int bc[4];                    /* r-beg, r-end, theta-beg, theta-end   */
   double gamma, cfl, cfl_max_var;
-  double mu, l_star, f_x, t_x, krad, alpharad, cent_mass;
+  double mu, l_star, f_xray, t_x, krad, alpharad, cent_mass;   /* f_xray: f_x (a field
+                                   name apart from the problem's f_x parameter macro) */
   double unit_density, unit_length, unit_velocity;
   const double *x1, *x1l, *x1r, *dx1;
   const double *x2, *x2l, *x2r, *dx2;
@@ -190,7 +215,20 @@ typedef struct {
    * create.  Ignored for other krad / alpharad. */
   int fm_points;                /* 1 .. 4096 */
   const double *fm_t, *fm_m;
+  /* How the line force evaluates M = krad t^alpha per angle bin (ignored
+   * with the tabulated multiplier):
+   *   PMX_FORCE_POWER_REF (0): as the reference (line_connect.c:879-889),
+   *     t = sigma_e rho v_th / dvds and pow(t, alpha) for each of the two
+   *     LineForce calls of a cell; the device pow, like glibc's, is within
+   *     an ulp of the exact power.
+   *   PMX_FORCE_POWER_FACTORED (1): both calls share one power per bin,
+   *     (sigma_e rho v_th)^alpha once per cell and call times
+   *     exp2(-alpha log2 dvds) per bin: a few 1e-15 relative from the
+   *     reference's M, faster. */
+  int force_power;
 } pmx_wind_desc;
+#define PMX_FORCE_POWER_REF      0
+#define PMX_FORCE_POWER_FACTORED 1
 
 /* Create a context: uploads geometry, PLM weights and the line-force
  * tables (the one-time part of VGradCalc, line_connect.c:534-625). */
@@ -222,8 +260,12 @@ int pmx_wind_set_heatcool(pmx_wind *w, const double *xi, const double *t_r);
  * when a surrogate is attached (the network is trained on unit prefactors). */
 int pmx_wind_set_prefactors(pmx_wind *w, const double *pre);
 
-/* g_time, g_dt and the step counter g_stepNumber of the next step. */
+/* g_time, g_dt and the step counter g_stepNumber of the next step.  Also
+ * clears the run-stopping flags and cuts dt to tstop (pmx_wind_set_tstop). */
 int pmx_wind_set_time(pmx_wind *w, double t, double dt, long step);
+/* runtime.tstop (default: none): as pmx_hydro_set_tstop, for the wind's
+ * main loop (pmx_wind_run / _run_graph, and each grid of a batch). */
+int pmx_wind_set_tstop(pmx_wind *w, double tstop);
 /* The timeStep fields the step reduces into: Dts->invDt_hyp (max with the
  * stage-1 C_dt, then / DIMENSIONS) and Dts->dt_cool (min with the plain
  * integrator's limits).  The reference's Integrate() resets them
@@ -231,7 +273,9 @@ int pmx_wind_set_time(pmx_wind *w, double t, double dt, long step);
  * pmx_wind_split_source itself passes its values here first. */
 int pmx_wind_set_dts(pmx_wind *w, double inv_dt_hyp, double dt_cool);
 /* Also returns Dts->invDt_hyp, Dts->dt_cool and the OR of the cons->prim
- * flags since the last call (bit 8: a cooling sub-step limit was hit). */
+ * flags and the PMX_FLAG_* bits (PMX_FLAG_COOL_SUBSTEPS: a cooling
+ * integration hit its sub-step limit).  step: the steps the main loop has
+ * taken since set_time, added to its step; halted steps do not count. */
 int pmx_wind_get_time(pmx_wind *w, double *t, double *dt, long *step, double *inv_dt_hyp,
                       double *dt_cool, int *flags);
 

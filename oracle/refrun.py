@@ -38,10 +38,16 @@ def build(with_reference: bool | None = None) -> None:
     if with_reference:
         subprocess.run(["make", "-s", "-j8", "-C", str(HERE / "ref"), "all"],
                        check=True, stdout=subprocess.DEVNULL)
+        # the drop-in builds link libplutomx.so, built before this
+        if (PKG_DIR / "libplutomx.so").exists():
+            subprocess.run(["make", "-s", "-j8", "-C", str(HERE / "ref"), "glue"],
+                           check=True, stdout=subprocess.DEVNULL)
 
 
-def reference_binary(problem: str) -> pathlib.Path:
-    return REF_DIR / problem / "pluto"
+def reference_binary(problem: str, glue: bool = False) -> pathlib.Path:
+    """The reference build of a problem, or (glue) the same problem linked
+    with the drop-in (pluto-sirocco_amd/glue/pmx_glue.c + libplutomx.so)."""
+    return REF_DIR / (f"glue_{problem}" if glue else problem) / "pluto"
 
 
 def read_dump(path: pathlib.Path, shape) -> dict:
@@ -67,7 +73,7 @@ def read_dump(path: pathlib.Path, shape) -> dict:
 def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
                   timeout: float = 3600.0, fluxes=None, sirocco=None,
                   t0: float | None = None, files_dir=None, output: bool = False,
-                  restart=None) -> dict:
+                  restart=None, glue: bool = False) -> dict:
     """Run the reference for ``nsteps`` (>= 2) steps from ``V0``.
 
     PLUTO's ``-maxsteps N`` integrates N+1 steps (Src/main.c:233-240), so
@@ -88,7 +94,7 @@ def run_reference(cfg, V0: np.ndarray, nsteps: int, time_only: bool = False,
     dt and "dbl" (the files it wrote)."""
     if nsteps < 2:
         raise ValueError("the reference driver needs nsteps >= 2")
-    binary = reference_binary(cfg.reference_problem())
+    binary = reference_binary(cfg.reference_problem(), glue)
     if not binary.exists():
         raise FileNotFoundError(f"{binary} not built (make -C oracle/ref)")
     with tempfile.TemporaryDirectory(prefix="pmxref_") as tmp:
@@ -169,7 +175,12 @@ class _OrcCfg(ctypes.Structure):
                 ("solver", ctypes.c_int), ("rk", ctypes.c_int),
                 ("bc", ctypes.c_int * 6), ("gamma", ctypes.c_double),
                 ("cfl", ctypes.c_double), ("cfl_max_var", ctypes.c_double),
-                ("xbeg", ctypes.c_double * 3), ("xend", ctypes.c_double * 3)]
+                ("xbeg", ctypes.c_double * 3), ("xend", ctypes.c_double * 3),
+                ("tstop", ctypes.c_double), ("first_dt", ctypes.c_double)]
+
+
+# why an oracle run ended before its step count (oracle.h orc_last_quit)
+QUIT_NAMES = {0: None, 1: "dt_too_small", 2: "cool_substeps", 3: "tstop"}
 
 
 _lib = None
@@ -189,6 +200,7 @@ def lib():
         _lib.orc_cons_to_prim.argtypes = [ctypes.c_double, dp, dp]
         _lib.orc_cons_to_prim.restype = ctypes.c_int
         _lib.orc_prim_to_cons.argtypes = [ctypes.c_double, dp, dp]
+        _lib.orc_last_quit.restype = ctypes.c_int
     return _lib
 
 
@@ -207,6 +219,8 @@ def orc_cfg(cfg) -> _OrcCfg:
     for s in range(6):
         c.bc[s] = cfg.bc[s]
     c.gamma, c.cfl, c.cfl_max_var = cfg.gamma, cfg.cfl, cfg.cfl_max_var
+    c.tstop = getattr(cfg, "tstop", 1.e30)
+    c.first_dt = cfg.first_dt
     return c
 
 
@@ -220,10 +234,11 @@ def run_oracle(cfg, V0: np.ndarray, nsteps: int, dt0: float | None = None,
     c = orc_cfg(cfg)
     rc = lib().orc_run(ctypes.byref(c), _ptr(V), nsteps, _ptr(dt), _ptr(t),
                        _ptr(U), _ptr(hist))
-    if rc != 0:
+    if rc < 0:
         raise RuntimeError(f"orc_run failed ({rc})")
     return {"V": V, "U": U, "t": float(t[0]), "dt": float(dt[0]),
-            "dt_seq": hist[:nsteps].copy()}
+            "dt_seq": hist[:rc].copy(), "steps": rc,
+            "quit": QUIT_NAMES[lib().orc_last_quit()]}
 
 
 # ----------------------------------------------------------------------
@@ -250,7 +265,8 @@ class _OrcWindCfg(ctypes.Structure):
                 ("char_limiting", ctypes.c_int), ("entropy_switch", ctypes.c_int),
                 ("solver", ctypes.c_int),
                 ("xi", _dp), ("t_r", _dp), ("pre", _dp),
-                ("fm_points", ctypes.c_int), ("fm_t", _dp), ("fm_m", _dp)]
+                ("fm_points", ctypes.c_int), ("fm_t", _dp), ("fm_m", _dp),
+                ("tstop", ctypes.c_double), ("first_dt", ctypes.c_double)]
 
 
 def _wind_struct(cfg, fluxes=None, grid=None, sirocco=None):
@@ -298,6 +314,8 @@ def _wind_struct(cfg, fluxes=None, grid=None, sirocco=None):
     c.char_limiting = int(cfg.char_limiting)
     c.entropy_switch = int(cfg.entropy_switch)
     c.solver = int(cfg.solver)
+    c.tstop = getattr(cfg, "tstop", 1.e30)
+    c.first_dt = cfg.first_dt
     if sirocco is not None:
         if sirocco.xi is not None:
             xi, t_r = sirocco.heatcool_floored()
@@ -330,10 +348,11 @@ def run_wind_oracle(cfg, V0: np.ndarray, nsteps: int, dt0: float | None = None,
     uv = np.zeros((23, cfg.nx[1], cfg.nx[0])) if user_vars else None
     rc = lib().orc_wind_run_uv(ctypes.byref(c), _ptr(V), nsteps, _ptr(dt), _ptr(t), _ptr(U),
                                _ptr(hist), _ptr(uv) if uv is not None else None)
-    if rc != 0:
+    if rc < 0:
         raise RuntimeError(f"orc_wind_run failed ({rc})")
     out = {"V": V, "U": U, "t": float(t[0]), "dt": float(dt[0]),
-           "dt_seq": hist[:nsteps].copy()}
+           "dt_seq": hist[:rc].copy(), "steps": rc,
+           "quit": QUIT_NAMES[lib().orc_last_quit()]}
     if uv is not None:
         out["uv"] = uv
     return out
@@ -356,7 +375,7 @@ def run_wind_cool_oracle(cfg, V0: np.ndarray, dt: float, t: float = 0.0, xi=None
     rc = lib().orc_wind_cool(ctypes.byref(c), _ptr(V), dt, t, xi_p, t_p, _ptr(dtc))
     if rc != 0:
         raise RuntimeError(f"orc_wind_cool failed ({rc})")
-    return {"V": V, "dt_cool": float(dtc[0])}
+    return {"V": V, "dt_cool": float(dtc[0]), "quit": QUIT_NAMES[lib().orc_last_quit()]}
 
 
 WIND_COUNTERS = ("rkf12_ok", "ck45_calls", "ck45_sub", "ck45_reject", "bl_solve",

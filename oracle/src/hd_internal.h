@@ -16,4 +16,7 @@ void orc_roe(double gamma, int ndim, int n, const double *ql,
              const double *uL, const double *qr, const double *uR,
              double *flux, double *press, double *cmax, double *mach);
 
+/* records why a run ended before its step count (orc_last_quit) */
+void orc_set_quit(int why);
+
 #endif

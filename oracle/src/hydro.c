@@ -588,8 +588,14 @@ int orc_run(const orc_cfg *c, double *Vin, int nsteps, double *dt, double *t,
     s.V[VIDX(&s, nv, k + s.beg[2], j + s.beg[1], i + s.beg[0])] =
         Vin[(((size_t)nv*s.n[2] + k)*s.n[1] + j)*s.n[0] + i];
 
+  orc_set_quit(ORC_QUIT_NONE);
   for (step = 0; step < nsteps; step++) {            /* Src/main.c:215-337 */
     double dtn, dt_hyp;
+    int last = 0;
+    if ((*t + *dt) >= c->tstop*(1.0 - 1.e-8)) {     /* main.c:226-229 */
+      *dt = (c->tstop - *t);
+      last = 1;
+    }
     s.invDt_hyp = 0.0;                               /* main.c:406-415 */
     s.maxMach = 0.0;
     if (dt_hist) dt_hist[step] = *dt;
@@ -600,6 +606,16 @@ int orc_run(const orc_cfg *c, double *Vin, int nsteps, double *dt, double *t,
     dtn = dt_hyp;
     dtn = MIND(dtn, c->cfl_max_var*(*dt));           /* main.c:670 */
     if (c->cfl_max_var != 1.0) *dt = dtn;            /* main.c:695-696 */
+    if (dtn < c->first_dt*1.e-9) {                   /* main.c:672-683 */
+      orc_set_quit(ORC_QUIT_DT_TOO_SMALL);
+      step++;
+      break;
+    }
+    if (last) {
+      orc_set_quit(ORC_STOP_TSTOP);
+      step++;
+      break;
+    }
   }
 
   for (nv = 0; nv < NV; nv++)
@@ -612,5 +628,9 @@ int orc_run(const orc_cfg *c, double *Vin, int nsteps, double *dt, double *t,
   }
   for (d = 0; d < 3; d++) { free(s.dx[d]); free(s.inv_dx[d]); }
   free(s.V); free(s.U); free(s.U0); free(s.Cdt);
-  return 0;
+  return step;
 }
+
+static int g_quit = ORC_QUIT_NONE;
+void orc_set_quit(int why) { g_quit = why; }
+int orc_last_quit(void) { return g_quit; }

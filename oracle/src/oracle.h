@@ -36,15 +36,28 @@ typedef struct {
   int    bc[6];         /* x1beg, x1end, x2beg, x2end, x3beg, x3end       */
   double gamma, cfl, cfl_max_var;
   double xbeg[3], xend[3];
+  double tstop;         /* runtime.tstop (1e30 in the test configurations)  */
+  double first_dt;      /* runtime.first_dt (NextTimeStep's "too small")    */
 } orc_cfg;
 
+/* Why the last run stopped before nsteps (orc_last_quit): the reference's
+ * QUIT_PLUTO points on this path.  ORC_STOP_TSTOP: the step that reached
+ * tstop was taken (the main loop ends there). */
+enum { ORC_QUIT_NONE = 0, ORC_QUIT_DT_TOO_SMALL = 1, ORC_QUIT_COOL_SUBSTEPS = 2,
+       ORC_STOP_TSTOP = 3 };
+int orc_last_quit(void);
+
 /* Advance V (interior, in/out) by nsteps time steps exactly as PLUTO's
- * main loop does (Integrate + NextTimeStep, Src/main.c:215-337).
+ * main loop does (Integrate + NextTimeStep, Src/main.c:215-337), or until
+ * the loop ends earlier: the step that reaches tstop has its dt cut to
+ * tstop - t and is the last (main.c:226-229); NextTimeStep's "dt is too
+ * small" quits after the step (main.c:672-683).
  *   *dt   in: dt of the first step; out: dt of the next step
  *   *t    in/out: simulation time
  *   U_out optional: Uc after the last step (interior, [nv][k][j][i])
  *   dt_hist optional: nsteps entries, dt used at each step
- * Returns 0, or a negative value on bad configuration. */
+ * Returns the number of steps taken, or a negative value on bad
+ * configuration. */
 int orc_run(const orc_cfg *cfg, double *V, int nsteps, double *dt, double *t,
             double *U_out, double *dt_hist);
 
@@ -95,11 +108,16 @@ typedef struct {
   const double *fm_t;         /* [fm_points] t of the fits, as in the file   */
   const double *fm_m;         /* [fm_points][nx2][nx1] M per cell, as in the
                                  file (the reader keeps log10 of both)       */
+  double tstop;               /* runtime.tstop                                */
+  double first_dt;            /* runtime.first_dt                             */
 } orc_wind_cfg;
 
 /* Advance V ([6][nx2][nx1], in/out) by nsteps steps of the reference's
  * main loop with cooling (Src/main.c:215-337, :383-485, :521-697) starting
- * at step number 0.  dt/t in/out as in orc_run.  U_out optional. */
+ * at step number 0, or until it ends earlier as in orc_run; a Cash-Karp
+ * integration over 1000 sub-steps (cooling_ode_solver.c:190-192) also
+ * quits, after the step.  dt/t in/out as in orc_run.  U_out optional.
+ * Returns the number of steps taken. */
 int orc_wind_run(const orc_wind_cfg *cfg, double *V, int nsteps, double *dt,
                  double *t, double *U_out, double *dt_hist);
 

@@ -1158,7 +1158,10 @@ static double ck45(const Rad *R, double *v0, double *k1, double *v5th, double dt
       if (dt > (tend - t)) dt = tend - t;
       v0[RHOE] = v5th[RHOE];
       radiat(R, v0, k1);
-      if (ksub > 1000) { fprintf(stderr, "oracle ck45: too many substeps\n"); exit(1); }
+      if (ksub > 1000) {                  /* cooling_ode_solver.c:190-192: QUIT_PLUTO */
+        orc_set_quit(ORC_QUIT_COOL_SUBSTEPS);
+        break;
+      }
     } else {
       double dt_shrink = 0.9*dt*pow(err, -0.25);
       cnt[CNT_CK45_REJECT]++;
@@ -1449,10 +1452,16 @@ int orc_wind_run_uv(const orc_wind_cfg *c, double *Vin, int nsteps, double *dt, 
   w.t = *t;
   w.invDt_hyp = 0.0;
   w.dt_cool = 1.e38;
+  orc_set_quit(ORC_QUIT_NONE);
   for (step = 0; step < nsteps; step++) {       /* main.c:215-337 */
     /* with cooling, invDt / dt_cool are reset and dt recomputed only every
        second step (main.c:406-415, :329-333); without, every step */
     int cooling = c->cooling != ORC_COOL_NONE;
+    int last = 0;
+    if ((*t + *dt) >= c->tstop*(1.0 - 1.e-8)) {  /* main.c:226-229 */
+      *dt = (c->tstop - *t);
+      last = 1;
+    }
     if (dt_hist) dt_hist[step] = *dt;
     if (!cooling || (step - 1)%2 == 1) { w.invDt_hyp = 0.0; w.dt_cool = 1.e38; }
     if ((step%2) == 0) {
@@ -1474,13 +1483,20 @@ int orc_wind_run_uv(const orc_wind_cfg *c, double *Vin, int nsteps, double *dt, 
       else if (dtn == w.dt_cool) cnt[CNT_DT_COOL]++;
       else cnt[CNT_DT_CFL]++;
       if (c->cfl_max_var != 1.0) *dt = dtn;
+      if (dtn < c->first_dt*1.e-9)              /* main.c:672-683 */
+        orc_set_quit(ORC_QUIT_DT_TOO_SMALL);
     }
+    /* a quit inside the step (the cooling's sub-step limit) or in
+       NextTimeStep ends the run after it; so does the step that reached
+       tstop */
+    if (orc_last_quit() != ORC_QUIT_NONE) { step++; break; }
+    if (last) { orc_set_quit(ORC_STOP_TSTOP); step++; break; }
   }
 
   if (uv) user_vars(&w, uv);
   w_out(&w, Vin, U_out);
   w_free(&w);
-  return 0;
+  return step;
 }
 
 int orc_wind_cool(const orc_wind_cfg *c, double *V, double dt, double t, const double *xi,
@@ -1489,6 +1505,7 @@ int orc_wind_cool(const orc_wind_cfg *c, double *V, double dt, double t, const d
   W w;
   int i, j;
   memset(cnt, 0, sizeof(cnt));
+  orc_set_quit(ORC_QUIT_NONE);
   w_init(&w, c, V);
   if (xi)
     for (j = 0; j < w.n2; j++)

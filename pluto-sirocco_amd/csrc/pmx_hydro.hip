@@ -1,5 +1,6 @@
 // libplutomx: hydro context, step driver and C ABI (see include/plutomx.h).
 #include <hip/hip_runtime.h>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -32,13 +33,29 @@ struct DevScalars {
   double dt;                     // g_dt of the step being taken
   unsigned long long cdt_bits;   // max C_dt of stage 1
   double inv_dt_hyp;             // Dts->invDt_hyp of the last step
+  double tstop;                  // runtime.tstop
+  double dt_uncut;               // the dt a cut to tstop replaced
   int err;                       // cons->prim flags of the last step
-  int halt;                      // PMX_FLAG_DT_TOO_SMALL once set (sticky)
+  int halt;                      // PMX_FLAG_DT_TOO_SMALL / PMX_FLAG_TSTOP once set (sticky):
+                                 // later main-loop steps are no-ops
+  int last;                      // the next step is the last one (dt clamped to tstop - t)
 };
 
+// Src/main.c:226-229: the step that would reach tstop (to 1e-8 relative)
+// is the last one, and its dt is cut to land on tstop.
+__host__ __device__ inline void clamp_last_step(DevScalars& s) {
+  if ((s.t + s.dt) >= s.tstop * (1.0 - 1.e-8)) {
+    s.dt_uncut = s.dt;
+    s.dt = (s.tstop - s.t);
+    s.last = 1;
+  }
+}
+
 // dt/dx for every cell and direction, and reset of the step reductions.
+// A main-loop step (dt_override null) does nothing once the run has halted.
 __global__ void prep_kernel(DevScalars* s, const double* dx, double* dtdx, int ntot,
                             const double* dt_override) {
+  if (!dt_override && s->halt) return;
   double dt = dt_override ? *dt_override : s->dt;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   for (int q = t; q < ntot; q += gridDim.x * blockDim.x) dtdx[q] = dt / dx[q];
@@ -50,10 +67,15 @@ __global__ void prep_kernel(DevScalars* s, const double* dx, double* dtdx, int n
 }
 
 // Src/update_stage.c:389-394 then Src/main.c:596-697 (NextTimeStep, no
-// parabolic terms, no cooling) and the time update of main.c:296.
+// parabolic terms, no cooling) and the time update of main.c:296.  In the
+// main loop (loop != 0) a halted run does nothing; NextTimeStep's "dt is
+// too small" quit (main.c:672-683) halts it, and so does the end of the
+// step that reached tstop; otherwise the next step's dt is cut to land on
+// tstop when it would pass it (main.c:226-229).
 __global__ void finish_kernel(DevScalars* s, int ndim, double cfl, double cfl_max_var,
-                              double first_dt) {
-  double inv = 0.0;
+                              double first_dt, int loop, const double* inv_in) {
+  if (loop && s->halt) return;
+  double inv = inv_in ? *inv_in : 0.0;   // Dts->invDt_hyp the stage-1 maximum folds into
   double c = __longlong_as_double((long long)s->cdt_bits);
   inv = dmax(inv, c);
   inv /= (double)ndim;
@@ -64,8 +86,19 @@ __global__ void finish_kernel(DevScalars* s, int ndim, double cfl, double cfl_ma
   dt_hyp *= cfl;
   double dtnext = dt_hyp;
   dtnext = dmin(dtnext, cfl_max_var * dt);
-  if (dtnext < first_dt * 1.e-9) s->halt = PMX_FLAG_DT_TOO_SMALL;   // main.c:672-683
   if (cfl_max_var != 1.0) s->dt = dtnext;
+  if (!loop) return;
+  if (dtnext < first_dt * 1.e-9) s->halt |= PMX_FLAG_DT_TOO_SMALL;   // main.c:672-683
+  if (s->last) {
+    s->halt |= PMX_FLAG_TSTOP;
+    s->last = 0;
+  } else {
+    DevScalars q = *s;
+    clamp_last_step(q);
+    s->dt = q.dt;
+    s->dt_uncut = q.dt_uncut;
+    s->last = q.last;
+  }
 }
 
 // Plain streaming copy, used to calibrate the HBM byte counters and to
@@ -91,7 +124,7 @@ struct pmx_hydro {
   double* dtdx;
   double* inv_dl;
   DevScalars* s;
-  double* dt_in;    // device slot for an externally imposed dt
+  double* dt_in;    // device slots of an explicit step: dt, and Dts->invDt_hyp on entry
   hipStream_t stream;
   hipGraphExec_t graph;
   bool have_graph;
@@ -160,7 +193,7 @@ void dispatch_rec(pmx_hydro* h, const StageArgs& a, int nblocks, int mode) {
 }
 
 int stage(pmx_hydro* h, const double* Vin, const double* V0, double* Vout, int mode, double w0,
-          double wc, int write_u) {
+          double wc, int write_u, bool loop) {
   StageArgs a;
   a.Vin = Vin;
   a.V0 = V0;
@@ -181,6 +214,7 @@ int stage(pmx_hydro* h, const double* Vin, const double* V0, double* Vout, int m
   a.write_u = write_u;
   a.cdt_bits = &h->s->cdt_bits;
   a.err = &h->s->err;
+  a.halt = loop ? &h->s->halt : nullptr;
   int nblocks = a.tiles_x * a.tiles_y * a.tiles_z;
 
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -213,16 +247,18 @@ int advance(pmx_hydro* h, const double* dt_override) {
   hipLaunchKernelGGL(prep_kernel, dim3((ntot + 255) / 256), dim3(256), 0, h->stream, h->s,
                      h->dx, h->dtdx, ntot, dt_override);
   PMX_HIP_CHECK(hipGetLastError());
+  const bool loop = dt_override == nullptr;
   if (h->d.rk == 3) {
-    if (stage(h, h->V[0], h->V[0], h->V[1], MODE_FIRST, 0, 0, 1)) return -1;
-    if (stage(h, h->V[1], h->V[0], h->V[2], MODE_WEIGHTED, 0.75, 0.25, 1)) return -1;
-    if (stage(h, h->V[2], h->V[0], h->V[0], MODE_THIRD, 0, 0, 1)) return -1;
+    if (stage(h, h->V[0], h->V[0], h->V[1], MODE_FIRST, 0, 0, 1, loop)) return -1;
+    if (stage(h, h->V[1], h->V[0], h->V[2], MODE_WEIGHTED, 0.75, 0.25, 1, loop)) return -1;
+    if (stage(h, h->V[2], h->V[0], h->V[0], MODE_THIRD, 0, 0, 1, loop)) return -1;
   } else {
-    if (stage(h, h->V[0], h->V[0], h->V[1], MODE_FIRST, 0, 0, 1)) return -1;
-    if (stage(h, h->V[1], h->V[0], h->V[0], MODE_WEIGHTED, 0.5, 0.5, 1)) return -1;
+    if (stage(h, h->V[0], h->V[0], h->V[1], MODE_FIRST, 0, 0, 1, loop)) return -1;
+    if (stage(h, h->V[1], h->V[0], h->V[0], MODE_WEIGHTED, 0.5, 0.5, 1, loop)) return -1;
   }
   hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, h->stream, h->s, h->d.dims,
-                     h->d.cfl, h->d.cfl_max_var, h->d.first_dt);
+                     h->d.cfl, h->d.cfl_max_var, h->d.first_dt, loop ? 1 : 0,
+                     loop ? nullptr : dt_override + 1);
   PMX_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -316,7 +352,7 @@ int pmx_hydro_create(const pmx_hydro_desc* desc, pmx_hydro** out) {
   CK(hipMalloc(&h->dtdx, sizeof(double) * ntot));
   CK(hipMalloc(&h->inv_dl, sizeof(double) * ntot));
   CK(hipMalloc(&h->s, sizeof(DevScalars)));
-  CK(hipMalloc(&h->dt_in, sizeof(double)));
+  CK(hipMalloc(&h->dt_in, 2 * sizeof(double)));
   {
     std::vector<double> dx(ntot), inv(ntot);
     for (int i = 0; i < h->nx; i++) dx[i] = desc->dx1[i];
@@ -327,6 +363,7 @@ int pmx_hydro_create(const pmx_hydro_desc* desc, pmx_hydro** out) {
     CK(hipMemcpy(h->inv_dl, inv.data(), sizeof(double) * ntot, hipMemcpyHostToDevice));
     DevScalars s0;
     memset(&s0, 0, sizeof(s0));
+    s0.tstop = HUGE_VAL;
     CK(hipMemcpy(h->s, &s0, sizeof(s0), hipMemcpyHostToDevice));
   }
   for (int q = 0; q < 3; q++) CK(hipMemset(h->V[q], 0, vbytes));
@@ -430,6 +467,26 @@ int pmx_hydro_set_time(pmx_hydro* h, double t, double dt) {
   PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
   s0.t = t;
   s0.dt = dt;
+  s0.halt = 0;
+  s0.last = 0;
+  clamp_last_step(s0);
+  PMX_HIP_CHECK(hipMemcpyAsync(h->s, &s0, sizeof(s0), hipMemcpyHostToDevice, h->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
+  return 0;
+}
+
+int pmx_hydro_set_tstop(pmx_hydro* h, double tstop) {
+  if (!h || !(tstop > -HUGE_VAL)) { set_error("bad argument"); return -1; }
+  DevScalars s0;
+  PMX_HIP_CHECK(hipMemcpyAsync(&s0, h->s, sizeof(s0), hipMemcpyDeviceToHost, h->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
+  s0.tstop = tstop;
+  s0.halt &= ~PMX_FLAG_TSTOP;
+  if (s0.last) {           // undo a cut made for the previous tstop
+    s0.dt = s0.dt_uncut;
+    s0.last = 0;
+  }
+  clamp_last_step(s0);
   PMX_HIP_CHECK(hipMemcpyAsync(h->s, &s0, sizeof(s0), hipMemcpyHostToDevice, h->stream));
   PMX_HIP_CHECK(hipStreamSynchronize(h->stream));
   return 0;
@@ -449,7 +506,8 @@ int pmx_hydro_get_time(pmx_hydro* h, double* t, double* dt, double* inv_dt_hyp, 
 
 int pmx_hydro_advance_step(pmx_hydro* h, double dt, double* inv_dt_hyp) {
   if (!h) { set_error("null argument"); return -1; }
-  PMX_HIP_CHECK(hipMemcpyAsync(h->dt_in, &dt, sizeof(double), hipMemcpyHostToDevice, h->stream));
+  const double in[2] = {dt, inv_dt_hyp ? *inv_dt_hyp : 0.0};
+  PMX_HIP_CHECK(hipMemcpyAsync(h->dt_in, in, sizeof(in), hipMemcpyHostToDevice, h->stream));
   if (advance(h, h->dt_in)) return -1;
   if (h->timing) collect_timing(h);
   DevScalars s0;

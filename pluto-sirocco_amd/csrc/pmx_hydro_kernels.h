@@ -58,6 +58,8 @@ struct StageArgs {
   int write_u;              // store U (always needed except after the last stage)
   unsigned long long* cdt_bits;  // MODE_FIRST: max over cells of C_dt (bit pattern)
   int* err;                      // OR of cons->prim floor flags
+  const int* halt;               // main-loop steps: skip the stage once nonzero; null for a
+                                 // step the caller drives (explicit dt)
 };
 
 template <int REC> struct RecTraits {
@@ -162,6 +164,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
   double* const sStY = sSt + (MERGE ? NST * 10 : 0);
   __shared__ double sFx[NFX * 7];      // x-fluxes [flux(5) press cmax][face]
   __shared__ double sRed[NT / 64];
+
+  if (a.halt && *a.halt) return;       // the run has stopped (uniform over the launch)
 
   const int tid = threadIdx.x;
   const bool owner = tid < NC;         // thread owns tile cell (tx, ty)

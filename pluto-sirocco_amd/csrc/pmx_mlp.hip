@@ -400,7 +400,7 @@ __device__ __forceinline__ void reduce_dtcool(double cand, int bad, wind::Scalar
     double m = dmin(smin[0], smin[1]);
     if (m >= 0.0 && m < 1.0e300)
       atomicMin(&s->dtcool_bits, (unsigned long long)__double_as_longlong(m));
-    if (sbad[0] | sbad[1]) atomicOr(&s->err, 256);
+    if (sbad[0] | sbad[1]) atomicOr(&s->err, PMX_FLAG_COOL_SUBSTEPS);
   }
 }
 
@@ -431,12 +431,12 @@ __device__ __forceinline__ void reduce_dtcool_any(double cand, int bad, wind::Sc
     if ((threadIdx.x & 63) == 0 && s) {
       if (cand >= 0.0 && cand < 1.0e300)
         atomicMin(&s->dtcool_bits, (unsigned long long)__double_as_longlong(cand));
-      if (bad) atomicOr(&s->err, 256);
+      if (bad) atomicOr(&s->err, PMX_FLAG_COOL_SUBSTEPS);
     }
   } else if (s) {
     if (cand >= 0.0 && cand < 1.0e300)
       atomicMin(&s->dtcool_bits, (unsigned long long)__double_as_longlong(cand));
-    if (bad) atomicOr(&s->err, 256);
+    if (bad) atomicOr(&s->err, PMX_FLAG_COOL_SUBSTEPS);
   }
 }
 
@@ -473,7 +473,7 @@ __global__ __launch_bounds__(128) void surr_classify_kernel(CoolBatch b, SurrWor
     const CoolArgs& a = b.args[g];
     const long long cell = q - b.off[g];
     sg = a.s;
-    if (cell < a.sv) {
+    if (cell < a.sv && !(b.loop && a.s->halt)) {
       const Const& k = a.k;
       const double dt = a.dtc ? a.dtc[cell] : a.s->dt, t = a.s->t;
       RadCell R = cell_state(a, cell);
@@ -646,7 +646,7 @@ __global__ __launch_bounds__(128) void plain_cells_kernel(CoolBatch b, unsigned 
     const CoolArgs& a = b.args[g];
     const long long cell = q - b.off[g];
     sg = a.s;
-    if (cell < a.sv) {
+    if (cell < a.sv && !(b.loop && a.s->halt)) {
       RadCell R = cell_state(a, cell);
       const double p = a.V[PRS * a.sv + cell];
       a.V[PRS * a.sv + cell] =
@@ -952,6 +952,7 @@ int pmx_mlp_cool_cells(pmx_mlp* m, const pmx_wind_desc* d, long long n, const do
   A((void**)&doff, sizeof(long long) * 2);
   wind::Scalars s0;
   memset(&s0, 0, sizeof(s0));
+  s0.tstop = HUGE_VAL;
   s0.t = 5.0;    // g_time > 3: Radiat takes xi and T_r from the per-cell arrays
   s0.dt = 0.0;
   {
@@ -971,12 +972,12 @@ int pmx_mlp_cool_cells(pmx_mlp* m, const pmx_wind_desc* d, long long n, const do
     wind::CoolArgs a;
     a.V = dV; a.nr = 1; a.nt = (int)n; a.sv = n; a.k = k; a.x1 = dx1; a.xi_s = dxi; a.tr_s = dtr;
     a.pre = nullptr;
-    a.s = ds; a.dtc = ddt;
+    a.s = ds; a.dtc = ddt; a.loop = 0;
     const long long off[2] = {0, n};
     e = hipMemcpy(dargs, &a, sizeof(a), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(doff, off, sizeof(off), hipMemcpyHostToDevice);
     wind::CoolBatch b;
-    b.args = dargs; b.off = doff; b.G = 1; b.total = n;
+    b.args = dargs; b.off = doff; b.G = 1; b.total = n; b.loop = 0;
     sw.guess = mode == 2 ? 1 : 0;
     if (e == hipSuccess) e = hipEventRecord(e0, m->stream);
     if (e == hipSuccess) {

@@ -224,7 +224,7 @@ int pmx_sirocco_read_heatcool(const char* dir, const pmx_wind_desc* grid, const 
         const long long c = (long long)j * nr + i;
         const double r = grid->x1[i + NG] * grid->unit_length;
         const double nH = rho[c] * grid->unit_density / (1.43 * C_MP);
-        const double lx = grid->l_star * grid->f_x;
+        const double lx = grid->l_star * grid->f_xray;
         xi[c] = lx / nH / (r * r);
         t_r[c] = grid->t_x;
       }

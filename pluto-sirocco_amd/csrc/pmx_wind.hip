@@ -75,6 +75,7 @@ struct StageArgs {
   // ghost layers, [nt+4][nr+4]; null without flattening
   const unsigned char* shock;
   double* Ue;          // ENTROPY_SWITCH: Uc[ENTR] = rho s per zone; null without it
+  int loop;            // a main-loop step: nothing to do once s->halt is set
 };
 
 // The wind's userdef sides: r-beg maps like outflow (plus v_r <= 0), theta-
@@ -401,8 +402,11 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       double vth1 = sqrt((2.0 * C_kB * T1) / C_mp);
       double vth2 = sqrt((2.0 * C_kB * T2) / C_mp);
       const bool tab = (FEAT & FEAT_TAB) && k.fm_points > 0;
+      // factored power: A = (sigma_e rho v_th)^alpha per call; the
+      // reference's form: num = sigma_e rho v_th per call
       double A1 = 0.0, A2 = 0.0;
-      if (!tab) { A1 = force_factor(k, rho1, vth1); A2 = force_factor(k, rho2, vth2); }
+      if (!tab && k.fm_fast) { A1 = force_factor(k, rho1, vth1); A2 = force_factor(k, rho2, vth2); }
+      else if (!tab) { A1 = k.sigma_e * rho1 * vth1; A2 = k.sigma_e * rho2 * vth2; }
       const long long sa = a.sv * a.na;
       const double* lf = T.lf + cell;
       const double rc_c = 1.0 / C_c, r_ua = 1.0 / k.UA;
@@ -426,8 +430,10 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
         if (tab) {   // both calls read the fit of this cell (line_connect.c:858-862)
           fm1 = force_mult_tab(k, rho1, vth1, dvds, T.fm_lt, T.fm_lm + cell, a.sv);
           fm2 = force_mult_tab(k, rho2, vth2, dvds, T.fm_lt, T.fm_lm + cell, a.sv);
-        } else {
+        } else if (k.fm_fast) {
           force_mult_pair(k, A1, A2, dvds, fm1, fm2);
+        } else {
+          force_mult_ref(k, A1, A2, dvds, fm1, fm2);
         }
         // ((fm sigma_e f) / c) / UA, each quotient bit-identical
         g1r += qdiv<FAST>(qdiv<FAST>(fm1 * k.sigma_e * fr, C_c, rc_c), k.UA, r_ua);
@@ -652,6 +658,7 @@ __device__ __forceinline__ bool block_reduce_stage(Scalars* s, double cdt, int f
 __global__ __launch_bounds__(BX* BY) void wind_flag_kernel(StageArgs a, unsigned char* shock) {
   const int I = blockIdx.x * BX + threadIdx.x, J = blockIdx.y * BY + threadIdx.y;
   if (I >= a.nr + 4 || J >= a.nt + 4) return;
+  if (a.loop && a.s->halt) return;
   const int i = I - NG, j = J - NG;
   const Tables& T = a.T;
   double c[NVW], e[NVW], w[NVW], n[NVW], s[NVW];
@@ -682,6 +689,7 @@ __global__ __launch_bounds__(BX* BY) void wind_flag_kernel(StageArgs a, unsigned
 
 template <int STAGE, int FEAT>
 __device__ __forceinline__ void stage_grid(const StageArgs& a) {
+  if (a.loop && a.s->halt) return;     // the run has stopped (uniform over the launch)
   const int i = blockIdx.x * BX + threadIdx.x;
   const int j = blockIdx.y * BY + threadIdx.y;
   double cdt = 0.0;
@@ -717,6 +725,7 @@ void wind_stage_kernel_w(StageArgs a) {
 
 template <int MODE>
 __global__ __launch_bounds__(128) void wind_cool_kernel(CoolArgs a, unsigned long long* calls) {
+  if (a.loop && a.s->halt) return;     // uniform over the launch
   const long long cell = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   double cand = 1.0e300;
   int bad = 0;
@@ -760,7 +769,7 @@ __global__ __launch_bounds__(128) void wind_cool_kernel(CoolArgs a, unsigned lon
       double m = dmin(smin[0], smin[1]);
       if (m >= 0.0)
         atomicMin(&a.s->dtcool_bits, (unsigned long long)__double_as_longlong(m));
-      if (sbad[0] | sbad[1]) atomicOr(&a.s->err, 256);
+      if (sbad[0] | sbad[1]) atomicOr(&a.s->err, PMX_FLAG_COOL_SUBSTEPS);
     }
   }
 }
@@ -769,8 +778,11 @@ __global__ __launch_bounds__(128) void wind_cool_kernel(CoolArgs a, unsigned lon
 // the invDt / dt_cool reset that the next step performs first
 // (main.c:406-415).  With cooling both happen every second step
 // (main.c:329-333), without every step: `next` says whether this is one.
+// Then the run-stopping conditions and the cut of the next dt to tstop
+// (end_of_step); a halted run does nothing.
 __global__ void wind_end_kernel(Scalars* s, int next, double cfl, double cfl_max_var,
                                 double first_dt) {
+  if (s->halt) return;
   double dt = s->dt;
   s->t += dt;
   double dt_cool = __longlong_as_double((long long)s->dtcool_bits);
@@ -781,11 +793,12 @@ __global__ void wind_end_kernel(Scalars* s, int next, double cfl, double cfl_max
     double dtn = dt_hyp;
     dtn = dmin(dtn, dt_cool);
     dtn = dmin(dtn, cfl_max_var * dt);
-    if (dtn < first_dt * 1.e-9) s->halt = PMX_FLAG_DT_TOO_SMALL;   // main.c:672-683
+    if (dtn < first_dt * 1.e-9) s->halt |= PMX_FLAG_DT_TOO_SMALL;   // main.c:672-683
     if (cfl_max_var != 1.0) s->dt = dtn;
     s->inv_dt_hyp = 0.0;
     s->dtcool_bits = (unsigned long long)__double_as_longlong(1.e38);
   }
+  end_of_step(s);
 }
 
 // ComputeUserVar of the production problem (cv_idl/userdef_output.c:20-260)
@@ -1007,7 +1020,7 @@ int collect(pmx_wind* w) {
   return 0;
 }
 
-StageArgs stage_args(pmx_wind* w, const double* Vin, double* Vout) {
+StageArgs stage_args(pmx_wind* w, const double* Vin, double* Vout, bool loop = false) {
   StageArgs a;
   a.Vin = Vin;
   a.Vn = w->V[0];
@@ -1021,15 +1034,16 @@ StageArgs stage_args(pmx_wind* w, const double* Vin, double* Vout) {
   a.s = w->s;
   a.shock = w->shock;
   a.Ue = w->Ue;
+  a.loop = loop ? 1 : 0;
   return a;
 }
 
 // FlagShock of the step (SHOCK_FLATTENING MULTID) from V^n, before stage 1.
-int flag_shock(pmx_wind* w, hipStream_t st) {
+int flag_shock(pmx_wind* w, hipStream_t st, bool loop) {
   if (!w->shock) return 0;
   dim3 grid((w->nr + 4 + BX - 1) / BX, (w->nt + 4 + BY - 1) / BY), block(BX, BY);
-  hipLaunchKernelGGL(wind_flag_kernel, grid, block, 0, st, stage_args(w, w->V[0], w->V[1]),
-                     w->shock);
+  hipLaunchKernelGGL(wind_flag_kernel, grid, block, 0, st,
+                     stage_args(w, w->V[0], w->V[1], loop), w->shock);
   PMX_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -1069,17 +1083,17 @@ void launch_stage(pmx_wind* w, dim3 grid, dim3 block, const StageArgs& a) {
   }
 }
 
-int advance(pmx_wind* w) {
+int advance(pmx_wind* w, bool loop) {
   dim3 grid((w->nr + BX - 1) / BX, (w->nt + BY - 1) / BY), block(BX, BY);
   w->stepped = true;
   hipEvent_t e[2];
-  if (flag_shock(w, w->stream)) return -1;
+  if (flag_shock(w, w->stream, loop)) return -1;
   if (timed_begin(w, e)) return -1;
-  launch_stage<1>(w, grid, block, stage_args(w, w->V[0], w->V[1]));
+  launch_stage<1>(w, grid, block, stage_args(w, w->V[0], w->V[1], loop));
   PMX_HIP_CHECK(hipGetLastError());
   if (timed_end(w, e, false)) return -1;
   if (timed_begin(w, e)) return -1;
-  launch_stage<2>(w, grid, block, stage_args(w, w->V[1], w->V[0]));
+  launch_stage<2>(w, grid, block, stage_args(w, w->V[1], w->V[0], loop));
   PMX_HIP_CHECK(hipGetLastError());
   return timed_end(w, e, false);
 }
@@ -1096,18 +1110,21 @@ CoolArgs cool_args(pmx_wind* w) {
   a.pre = w->pre;
   a.s = w->s;
   a.dtc = nullptr;
+  a.loop = 0;
   return a;
 }
 
-int cool(pmx_wind* w) {
+int cool(pmx_wind* w, bool loop) {
   if (w->cooling == PMX_COOL_NONE) return 0;
   CoolArgs a = cool_args(w);
+  a.loop = loop ? 1 : 0;
   dim3 grid((unsigned)((w->sv + 127) / 128)), block(128);
   hipEvent_t e[2];
   if (timed_begin(w, e)) return -1;
   if (w->mlp) {
     CoolBatch b;
     b.args = w->cargs; b.off = w->coff; b.G = 1; b.total = w->sv;
+    b.loop = loop ? 1 : 0;
     if (mlp::surrogate_cool(w->mlp, b, w->sw, w->stream)) return -1;
   } else if (w->cooling == PMX_COOL_BLONDIN)
     hipLaunchKernelGGL(wind_cool_kernel<PMX_COOL_BLONDIN>, grid, block, 0, w->stream, a,
@@ -1161,7 +1178,7 @@ Const make_const(const pmx_wind_desc* d) {
   k.gm_code = C_G * d->cent_mass / (k.UL * k.UV * k.UV);
   k.sigma_e = C_sigmaT / C_amu / 1.18;
   k.krad = d->krad; k.alpharad = d->alpharad;
-  k.lx = d->l_star * d->f_x;
+  k.lx = d->l_star * d->f_xray;
   k.t_x = d->t_x;
   k.E_cost = k.UL / k.UD / pow(k.UV, 3.0);
   // cv_idl/init.c:184-188: dfloor = DFLOOR / UNIT_DENSITY, pfloor with T = 500 K
@@ -1174,6 +1191,7 @@ Const make_const(const pmx_wind_desc* d) {
   k.first_dt = d->first_dt;
   k.fm_points = (d->krad == 999 && d->alpharad == 999) ? d->fm_points : 0;
   k.fast_div = 0;   // set with the line-force tables
+  k.fm_fast = d->force_power == PMX_FORCE_POWER_FACTORED;
   return k;
 }
 
@@ -1427,6 +1445,7 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
   }
   Scalars s0;
   memset(&s0, 0, sizeof(s0));
+  s0.tstop = HUGE_VAL;
   s0.dt_cool = 1.e38;
   s0.dtcool_bits = (unsigned long long)0;
   memcpy(&s0.dtcool_bits, &s0.dt_cool, sizeof(double));
@@ -1564,9 +1583,31 @@ int pmx_wind_set_time(pmx_wind* w, double t, double dt, long step) {
   PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
   s0.t = t;
   s0.dt = dt;
+  s0.step = step;
+  s0.halt = 0;
+  s0.last = 0;
+  clamp_last_step(s0);
   PMX_HIP_CHECK(hipMemcpyAsync(w->s, &s0, sizeof(Scalars), hipMemcpyHostToDevice, w->stream));
   PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
   w->step = step;
+  return 0;
+}
+
+int pmx_wind_set_tstop(pmx_wind* w, double tstop) {
+  if (!w || !(tstop > -HUGE_VAL)) { set_error("bad argument"); return -1; }
+  Scalars s0;
+  PMX_HIP_CHECK(hipMemcpyAsync(&s0, w->s, sizeof(Scalars), hipMemcpyDeviceToHost, w->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  s0.tstop = tstop;
+  s0.halt &= ~PMX_FLAG_TSTOP;
+  if (s0.last) {           // undo a cut made for the previous tstop
+    s0.dt = s0.dt_uncut;
+    s0.last = 0;
+  }
+  clamp_last_step(s0);
+  PMX_HIP_CHECK(hipMemcpyAsync(w->s, &s0, sizeof(Scalars), hipMemcpyHostToDevice, w->stream));
+  PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
+  w->step = (long)s0.step;   // steps the halted run skipped do not count
   return 0;
 }
 
@@ -1591,7 +1632,7 @@ int pmx_wind_get_time(pmx_wind* w, double* t, double* dt, long* step, double* in
   PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
   if (t) *t = s0.t;
   if (dt) *dt = s0.dt;
-  if (step) *step = w->step;
+  if (step) *step = (long)s0.step;
   if (inv_dt_hyp) *inv_dt_hyp = s0.inv_dt_hyp;
   if (dt_cool) {
     double c;
@@ -1604,21 +1645,21 @@ int pmx_wind_get_time(pmx_wind* w, double* t, double* dt, long* step, double* in
 
 int pmx_wind_advance_step(pmx_wind* w) {
   if (!w) { set_error("null argument"); return -1; }
-  return advance(w);
+  return advance(w, false);
 }
 
 int pmx_wind_split_source(pmx_wind* w) {
   if (!w) { set_error("null argument"); return -1; }
-  return cool(w);
+  return cool(w, false);
 }
 
 // One iteration of the main loop for step number st (Strang order by
 // parity, NextTimeStep every second step with cooling).
 int wind_step(pmx_wind* w, long st) {
   if (st % 2 == 0) {
-    if (advance(w) || cool(w)) return -1;
+    if (advance(w, true) || cool(w, true)) return -1;
   } else {
-    if (cool(w) || advance(w)) return -1;
+    if (cool(w, true) || advance(w, true)) return -1;
   }
   const int next = (w->cooling == PMX_COOL_NONE) || (st % 2 == 1);
   hipLaunchKernelGGL(wind_end_kernel, dim3(1), dim3(1), 0, w->stream, w->s, next, w->cfl,
@@ -1757,6 +1798,7 @@ void wind_batch_stage_kernel(const StageArgs* __restrict__ args, const int* __re
     // Mcell-updates/s)
     const int g = __builtin_amdgcn_readfirstlane(find_item(tile_off, G, item));
     const StageArgs& a = args[g];
+    if (a.s->halt) continue;           // a halted grid (uniform over the workgroup)
     const int tx_n = (a.nr + TBX - 1) / TBX;
     const int local = item - tile_off[g];
     const int ty = local / tx_n, tx = local - ty * tx_n;
@@ -1775,6 +1817,7 @@ __global__ void wind_batch_end_kernel(const StageArgs* args, int G, int next, co
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= G) return;
   Scalars* s = args[g].s;
+  if (s->halt) return;
   double inv = s->inv_dt_hyp;
   inv = dmax(inv, __longlong_as_double((long long)s->cdt_bits));
   inv /= (double)2;
@@ -1790,11 +1833,12 @@ __global__ void wind_batch_end_kernel(const StageArgs* args, int G, int next, co
     double dtn = dt_hyp;
     dtn = dmin(dtn, dt_cool);
     dtn = dmin(dtn, cfl_max_var[g] * dt);
-    if (dtn < args[g].k.first_dt * 1.e-9) s->halt = PMX_FLAG_DT_TOO_SMALL;
+    if (dtn < args[g].k.first_dt * 1.e-9) s->halt |= PMX_FLAG_DT_TOO_SMALL;
     if (cfl_max_var[g] != 1.0) s->dt = dtn;
     s->inv_dt_hyp = 0.0;
     s->dtcool_bits = (unsigned long long)__double_as_longlong(1.e38);
   }
+  end_of_step(s);
 }
 
 }  // namespace wind
@@ -1904,7 +1948,7 @@ int batch_timed(pmx_wind_batch* b, hipEvent_t* e, bool begin, bool cool) {
 int batch_advance(pmx_wind_batch* b) {
   hipEvent_t e[2];
   for (pmx_wind* w : b->grids)
-    if (flag_shock(w, b->stream)) return -1;
+    if (flag_shock(w, b->stream, true)) return -1;
   if (batch_timed(b, e, true, false)) return -1;
   launch_batch_stage<1>(b);
   PMX_HIP_CHECK(hipGetLastError());
@@ -1919,6 +1963,7 @@ int batch_cool(pmx_wind_batch* b) {
   if (b->cooling == PMX_COOL_NONE) return 0;
   CoolBatch cb;
   cb.args = b->cargs; cb.off = b->cell_off; cb.G = b->G; cb.total = b->total_cells;
+  cb.loop = 1;
   hipEvent_t e[2];
   if (batch_timed(b, e, true, true)) return -1;
   if (b->mlp) {
@@ -1995,8 +2040,8 @@ int pmx_wind_batch_create(const pmx_wind_desc* descs, int ngrids, pmx_wind_batch
   b->cells = 0;
   for (int g = 0; g < G; g++) {
     pmx_wind* w = b->grids[g];
-    a1[g] = stage_args(w, w->V[0], w->V[1]);
-    a2[g] = stage_args(w, w->V[1], w->V[0]);
+    a1[g] = stage_args(w, w->V[0], w->V[1], true);
+    a2[g] = stage_args(w, w->V[1], w->V[0], true);
     ca[g] = cool_args(w);
     cfl[g] = w->cfl;
     cmv[g] = w->cfl_max_var;

@@ -13,9 +13,45 @@ struct Scalars {
   double t, dt, inv_dt_hyp, dt_cool;
   unsigned long long cdt_bits;     // max C_dt of the current step's stage 1
   unsigned long long dtcool_bits;  // running min of the cooling dt candidates
+  double tstop;                    // runtime.tstop
+  double dt_uncut;                 // the dt a cut to tstop replaced
+  long long step;                  // g_stepNumber of the next step
   int err;
-  int halt;                        // PMX_FLAG_DT_TOO_SMALL once set (sticky)
+  int halt;                        // PMX_FLAG_DT_TOO_SMALL / _COOL_SUBSTEPS / _TSTOP once set
+                                   // (sticky): later main-loop steps are no-ops
+  int last;                        // the next step is the last one (dt cut to tstop - t)
 };
+
+// Src/main.c:226-229: the step that would reach tstop (to 1e-8 relative)
+// is the last one, and its dt is cut to land on tstop.
+__host__ __device__ inline void clamp_last_step(Scalars& s) {
+  if ((s.t + s.dt) >= s.tstop * (1.0 - 1.e-8)) {
+    s.dt_uncut = s.dt;
+    s.dt = (s.tstop - s.t);
+    s.last = 1;
+  }
+}
+
+// End of a main-loop step, after NextTimeStep: the run-stopping conditions
+// (a cooling integration over its sub-step limit, the step that reached
+// tstop) halt the run; otherwise the next step's dt is cut to tstop.
+__device__ inline void end_of_step(Scalars* s) {
+  s->step++;
+  if (s->err & PMX_FLAG_COOL_SUBSTEPS) {
+    s->halt |= PMX_FLAG_COOL_SUBSTEPS;
+    s->err &= ~PMX_FLAG_COOL_SUBSTEPS;
+  }
+  if (s->last) {
+    s->halt |= PMX_FLAG_TSTOP;
+    s->last = 0;
+    return;
+  }
+  Scalars q = *s;
+  clamp_last_step(q);
+  s->dt = q.dt;
+  s->dt_uncut = q.dt_uncut;
+  s->last = q.last;
+}
 
 // Arguments of the cooling kernels.
 struct CoolArgs {
@@ -29,6 +65,7 @@ struct CoolArgs {
   const double* pre;     // [5][sv] cooling prefactors, or null: all 1.0
   Scalars* s;
   const double* dtc;     // per-cell dt (independent cells), or null: s->dt
+  int loop;              // a main-loop step: nothing to do once s->halt is set
 };
 
 // Several grids' cooling arguments at once: grid g owns the global cell
@@ -39,6 +76,7 @@ struct CoolBatch {
   const long long* off;   // device array, G + 1 entries
   int G;
   long long total;
+  int loop;               // a main-loop step: cells of halted grids are left alone
 };
 
 __device__ __forceinline__ int find_grid(const long long* off, int G, long long q) {

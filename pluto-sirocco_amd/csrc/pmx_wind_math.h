@@ -55,6 +55,7 @@ struct Const {
   int hll;                                 // [Solver] hll instead of hllc
   int roe;                                 // [Solver] roe
   int fast_div;                            // line-force quotients through div_rc
+  int fm_fast;                             // factored force-multiplier power (see force_mult_pair)
   double first_dt;                         // runtime->first_dt (dt-too-small stop)
   int fm_points;                           // > 0: tabulated force multiplier (krad = alpharad = 999)
 };
@@ -340,6 +341,26 @@ PMX_HD double force_mult_tab(const Const& k, double rho_cgs, double v_th, double
     if (M > 4400.) M = 4400.;
   }
   return 1.0 + M;
+}
+
+// The reference's form of both calls' multipliers for one bin
+// (line_connect.c:879-893): t = (sigma_e rho v_th) / dvds and M = krad
+// pow(t, alpha) per call, capped at 4400; num = sigma_e rho v_th of the
+// call, in the reference's operation order.  The device pow and glibc's
+// are both within an ulp of the exact power, so M agrees with the
+// reference's to an ulp (and bit for bit on most inputs).
+PMX_HD void force_mult_ref(const Const& k, double num1, double num2, double dvds, double& f1,
+                           double& f2) {
+  const double M_max = 4400.;
+  double M1 = 0.0, M2 = 0.0;
+  if (dvds > 0.0) {
+    M1 = k.krad * pow(num1 / dvds, k.alpharad);
+    M2 = k.krad * pow(num2 / dvds, k.alpharad);
+    if (M1 > M_max) M1 = M_max;
+    if (M2 > M_max) M2 = M_max;
+  }
+  f1 = 1.0 + M1;
+  f2 = 1.0 + M2;
 }
 
 PMX_HD void force_mult_pair(const Const& k, double A1, double A2, double dvds,

@@ -49,14 +49,20 @@ EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro
            "pmx_sirocco_fm_points", "pmx_sirocco_read_force_multiplier",
            "pmx_sirocco_read_heatcool", "pmx_sirocco_read_prefactors",
            "pmx_wind_snapshot", "pmx_wind_snapshot_wait", "pmx_write_dbl", "pmx_write_restart",
-           "pmx_read_restart", "pmx_read_dbl", "pmx_wind_snapshot_host", "pmx_wind_run_graph"]
+           "pmx_read_restart", "pmx_read_dbl", "pmx_wind_snapshot_host", "pmx_wind_run_graph",
+           "pmx_hydro_set_tstop", "pmx_wind_set_tstop"]
 
 
 class PmxError(RuntimeError):
     pass
 
 
+# Run-stopping flags (sticky; the main loop's steps are no-ops once one is
+# set, until set_time), as in include/plutomx.h.
+PMX_FLAG_COOL_SUBSTEPS = 256   # a Cash-Karp cooling integration exceeded 1000 sub-steps
 PMX_FLAG_DT_TOO_SMALL = 4096   # NextTimeStep stopped: dt < 1e-9 first_dt
+PMX_FLAG_TSTOP = 8192          # the step that reached tstop has been taken
+PMX_FLAGS_HALT = PMX_FLAG_COOL_SUBSTEPS | PMX_FLAG_DT_TOO_SMALL | PMX_FLAG_TSTOP
 
 
 class HydroDesc(ctypes.Structure):
@@ -76,7 +82,7 @@ class WindDesc(ctypes.Structure):
                 ("bc", ctypes.c_int * 4),
                 ("gamma", ctypes.c_double), ("cfl", ctypes.c_double),
                 ("cfl_max_var", ctypes.c_double),
-                ("mu", ctypes.c_double), ("l_star", ctypes.c_double), ("f_x", ctypes.c_double),
+                ("mu", ctypes.c_double), ("l_star", ctypes.c_double), ("f_xray", ctypes.c_double),
                 ("t_x", ctypes.c_double), ("krad", ctypes.c_double),
                 ("alpharad", ctypes.c_double), ("cent_mass", ctypes.c_double),
                 ("unit_density", ctypes.c_double), ("unit_length", ctypes.c_double),
@@ -88,7 +94,11 @@ class WindDesc(ctypes.Structure):
                 ("dfloor", ctypes.c_double), ("shock_flattening", ctypes.c_int),
                 ("char_limiting", ctypes.c_int), ("entropy_switch", ctypes.c_int),
                 ("solver", ctypes.c_int), ("first_dt", ctypes.c_double),
-                ("fm_points", ctypes.c_int), ("fm_t", _dp), ("fm_m", _dp)]
+                ("fm_points", ctypes.c_int), ("fm_t", _dp), ("fm_m", _dp),
+                ("force_power", ctypes.c_int)]
+
+PMX_FORCE_POWER_REF = 0        # the reference's t and pow(t, alpha) per LineForce call
+PMX_FORCE_POWER_FACTORED = 1   # one exp2(-alpha log2 dvds) per bin for both calls
 
 
 _lib = None
@@ -113,6 +123,7 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     lib.pmx_hydro_get_prim.argtypes = [vp, dp, ctypes.c_int]
     lib.pmx_hydro_get_cons.argtypes = [vp, dp, ctypes.c_int]
     lib.pmx_hydro_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_double]
+    lib.pmx_hydro_set_tstop.argtypes = [vp, ctypes.c_double]
     lib.pmx_hydro_get_time.argtypes = [vp, dp, dp, dp, ip]
     lib.pmx_hydro_advance_step.argtypes = [vp, ctypes.c_double, dp]
     lib.pmx_hydro_run.argtypes = [vp, ctypes.c_int]
@@ -131,6 +142,7 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     lib.pmx_wind_get_cons.argtypes = [vp, dp]
     lib.pmx_wind_set_heatcool.argtypes = [vp, dp, dp]
     lib.pmx_wind_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_long]
+    lib.pmx_wind_set_tstop.argtypes = [vp, ctypes.c_double]
     lib.pmx_wind_get_time.argtypes = [vp, dp, dp, lp, dp, dp, ip]
     lib.pmx_wind_set_dts.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     lib.pmx_wind_advance_step.argtypes = [vp]
@@ -253,6 +265,7 @@ class Hydro:
         _check(lib.pmx_hydro_create(ctypes.byref(d), ctypes.byref(h)), "pmx_hydro_create")
         self._h = h
         self._lib = lib
+        self.set_tstop(getattr(cfg, "tstop", 1.e30))
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -303,6 +316,11 @@ class Hydro:
     def set_time(self, t: float, dt: float) -> None:
         _check(self._lib.pmx_hydro_set_time(self._h, t, dt), "set_time")
 
+    def set_tstop(self, tstop: float) -> None:
+        """runtime.tstop: run() cuts the step that reaches it and halts
+        after it (PMX_FLAG_TSTOP)."""
+        _check(self._lib.pmx_hydro_set_tstop(self._h, tstop), "set_tstop")
+
     def get_time(self):
         t, dt, inv = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         fl = ctypes.c_int()
@@ -341,9 +359,10 @@ def wind_desc_scalars(cfg: setups.WindConfig) -> WindDesc:
     d.nx1, d.nx2, d.cooling = cfg.nx[0], cfg.nx[1], cfg.cooling
     for q in range(4):
         d.bc[q] = cfg.bc[q]
-    for name in ("gamma", "cfl", "cfl_max_var", "mu", "l_star", "f_x", "t_x", "krad",
+    for name in ("gamma", "cfl", "cfl_max_var", "mu", "l_star", "t_x", "krad",
                  "alpharad", "cent_mass", "unit_density", "unit_length", "unit_velocity"):
         setattr(d, name, getattr(cfg, name))
+    d.f_xray = cfg.f_x
     d.nangles = cfg.nangles
     d.dfloor = cfg.dfloor
     d.shock_flattening = int(cfg.shock_flattening)
@@ -351,6 +370,7 @@ def wind_desc_scalars(cfg: setups.WindConfig) -> WindDesc:
     d.entropy_switch = int(cfg.entropy_switch)
     d.solver = int(cfg.solver)
     d.first_dt = cfg.first_dt
+    d.force_power = int(getattr(cfg, "force_power", PMX_FORCE_POWER_REF))
     return d
 
 
@@ -598,6 +618,7 @@ class Wind:
         """View of a context owned elsewhere (a batch's grid)."""
         w = cls.__new__(cls)
         w.cfg, w._h, w._lib, w._owned = cfg, handle, load_library(), False
+        w.set_tstop(getattr(cfg, "tstop", 1.e30))
         return w
 
     def __init__(self, cfg: setups.WindConfig, fluxes=None, grid=None, sirocco=None):
@@ -611,6 +632,7 @@ class Wind:
         self._keep = []
         self._h = h
         self._lib = lib
+        self.set_tstop(getattr(cfg, "tstop", 1.e30))
 
     def close(self) -> None:
         if getattr(self, "_h", None):
@@ -706,6 +728,11 @@ class Wind:
 
     def set_time(self, t: float, dt: float, step: int = 0) -> None:
         _check(self._lib.pmx_wind_set_time(self._h, t, dt, step), "set_time")
+
+    def set_tstop(self, tstop: float) -> None:
+        """runtime.tstop: run() cuts the step that reaches it and halts
+        after it (PMX_FLAG_TSTOP)."""
+        _check(self._lib.pmx_wind_set_tstop(self._h, tstop), "set_tstop")
 
     def get_time(self) -> dict:
         t, dt, inv, dtc = (ctypes.c_double() for _ in range(4))

@@ -48,6 +48,7 @@ class HydroConfig:
     first_dt: float = 1.e-4
     xbeg: tuple = (-0.5, -0.5, -0.5)
     xend: tuple = (0.5, 0.5, 0.5)
+    tstop: float = 1.e30          # pluto.ini [Time] tstop
 
     @property
     def nghost(self) -> int:
@@ -82,7 +83,7 @@ class HydroConfig:
         lines += ["", "[Time]", "",
                   f"CFL              {self.cfl!r}",
                   f"CFL_max_var      {self.cfl_max_var!r}",
-                  "tstop            1.e30",
+                  f"tstop            {float(self.tstop)!r}",
                   f"first_dt         {self.first_dt!r}",
                   "", "[Solver]", "",
                   f"Solver         {_SOLVER_NAMES[self.solver]}",
@@ -249,6 +250,10 @@ class WindConfig:
     unit_density: float = 1e-14
     unit_length: float = 1e9
     unit_velocity: float = 1e9
+    tstop: float = 1.e30                     # pluto.ini [Time] tstop
+    # line force's power per angle bin: 0 the reference's form, 1 factored
+    # (one exp2/log2 per bin for both LineForce calls; a few 1e-15 off)
+    force_power: int = 0
 
     dims = 2
     nghost = 2
@@ -278,7 +283,8 @@ class WindConfig:
         if self.shock_flattening or self.char_limiting or self.entropy_switch:
             raise ValueError("shock flattening / characteristic limiting are built with the "
                              "cv_idl boundaries only")
-        return {COOL_BLONDIN: "wind_blondin", COOL_PLAIN: "wind_plain"}[self.cooling]
+        return {COOL_NONE: "wind_nocool", COOL_BLONDIN: "wind_blondin",
+                COOL_PLAIN: "wind_plain"}[self.cooling]
 
     def pluto_ini(self, output: bool = False) -> str:
         """pluto.ini of the reference build; ``output``: the production
@@ -291,7 +297,7 @@ class WindConfig:
                  "", "[Time]", "",
                  f"CFL              {self.cfl!r}",
                  f"CFL_max_var      {self.cfl_max_var!r}",
-                 "tstop            1.e30",
+                 f"tstop            {float(self.tstop)!r}",
                  f"first_dt         {self.first_dt!r}",
                  "", "[Solver]", "", f"Solver         {_SOLVER_NAMES[self.solver]}",
                  "", "[Boundary]", ""]

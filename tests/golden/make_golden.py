@@ -178,8 +178,173 @@ def wind_inputs(cfg):
             "flux_r": fr, "flux_t": ft, "flux_p": fp}
 
 
-def main(which=("hydro", "wind", "cvidl", "roe", "restart", "output")):
+def state_digest(V, U, t, dt, dt_seq) -> dict:
+    """A size-independent record of a full-size state: SHA-256 of the
+    little-endian float64 bytes of V and U (component-major [nv][k][j][i]),
+    per-variable sums, the time, the next dt and the dt sequence (exact
+    float64 values as hex strings)."""
+    import hashlib
+    V = np.ascontiguousarray(V, dtype="<f8")
+    U = np.ascontiguousarray(U, dtype="<f8")
+    return {"sha256_V": hashlib.sha256(V.tobytes()).hexdigest(),
+            "sha256_U": hashlib.sha256(U.tobytes()).hexdigest(),
+            "sum_V": [float(x).hex() for x in V.reshape(V.shape[0], -1).sum(axis=1)],
+            "sum_U": [float(x).hex() for x in U.reshape(U.shape[0], -1).sum(axis=1)],
+            "t": float(t).hex(), "dt": float(dt).hex(),
+            "dt_seq": [float(x).hex() for x in dt_seq]}
+
+
+def digest_cases():
+    """The Sedov workload at its full size and stated step count: 256^3,
+    PPM + HLLC + RK3, CFL 0.3, outflow, seeded 1% lognormal density noise,
+    50 steps (about 8 minutes of one core)."""
+    cfg = setups.c2_sedov(256)
+    yield "sedov_ppm_hllc_rk3_256_50steps", cfg, 1234, 50
+
+
+def tstop_cases():
+    """Runs that end at a finite tstop (pluto.ini [Time] tstop) before their
+    step limit: the step that would pass tstop is cut to land on it
+    (Src/main.c:226-229) and is the last.  tstop is placed from the
+    oracle's own times (bit-identical to the reference before the cut): half
+    way into step 7 (an odd count), 0.3 of the way into step 8 (even), and
+    5e-9 relative beyond the end of step 6 (inside the 1e-8 relative
+    tolerance, so step 6 is stretched by a hair and is the last).  Yields (name,
+    cfg, V0, nsteps limit)."""
+    kh = dataclasses.replace(setups.c1_kh(32), first_dt=1.e-3)
+    V0 = setups.kelvin_helmholtz(kh, seed=7)
+    r = refrun.run_oracle(kh, V0, 10)
+    t_after = np.cumsum(r["dt_seq"])
+    dts = r["dt_seq"]
+    for tag, tstop in (("odd7", t_after[5] + 0.5 * dts[6]),
+                       ("even8", t_after[6] + 0.3 * dts[7]),
+                       ("tol6", t_after[5] * (1.0 + 5.e-9))):
+        yield f"kh_plm_vl_hllc_rk2_32_tstop_{tag}", dataclasses.replace(kh, tstop=float(tstop)), V0, 30
+    wc = setups.WindConfig(nx=(24, 12), cooling=setups.COOL_PLAIN, first_dt=1.e-2)
+    W0 = setups.wind_ic_mixed(wc, rho0_cgs=1.e-9)
+    r = refrun.run_wind_oracle(wc, W0, 10)
+    t_after = np.cumsum(r["dt_seq"])
+    dts = r["dt_seq"]
+    for tag, tstop in (("odd5", t_after[3] + 0.5 * dts[4]),
+                       ("even6", t_after[4] + 0.3 * dts[5])):
+        yield f"wind_plain_mixed_24x12_tstop_{tag}", dataclasses.replace(wc, tstop=float(tstop)), W0, 30
+
+
+SAMPLE_STRIDE = 11   # every 11th cell (flat [j][i] index) of a full-size wind state
+
+
+def fullsize_wind_cases():
+    """The line-driven wind at its full size and stated step count: 512 x
+    256, PLM + HLLC + RK2, line force, plain cooling, 200 steps from the
+    seeded start state (about 20 minutes of one core).  Too large to store
+    whole: the fixture keeps every SAMPLE_STRIDE-th cell of V and U, the
+    per-variable sums over all cells and the dt sequence."""
+    cfg = setups.c3_wind(512, 256, setups.COOL_PLAIN)
+    yield "wind_plain_512x256_200steps_sample", cfg, setups.wind_ic(cfg), 200
+
+
+TRAJ_STEPS = (2, 4, 6, 8, 10, 14, 20, 30, 40, 60)
+TRAJ_STRIDE = 31
+
+
+def trajectory_cases():
+    """How the full-size wind (512 x 256, plain cooling) amplifies last-bit
+    differences: its trajectory at TRAJ_STEPS, every TRAJ_STRIDE-th cell,
+    and, per step and variable, how far the same run started from densities
+    one ulp larger (numpy.nextafter) has moved from it (max |dV| over max |V|
+    over all cells).  Written by the oracle, run in even chunks (the step
+    parity of the Strang order is kept); its state after 200 steps must be
+    the compiled reference's (wind_plain_512x256_200steps_sample) bit for
+    bit, which main() checks before writing."""
+    cfg = setups.c3_wind(512, 256, setups.COOL_PLAIN)
+    yield "wind_plain_512x256_trajectory_sample", cfg, setups.wind_ic(cfg)
+
+
+def libm_free_cases():
+    """The full-size wind without the force multiplier (KRAD = 0) and without
+    cooling: no pow / exp / log10 anywhere in the step, so the device must
+    reproduce the reference's 200 steps bit for bit (digest as for Sedov)."""
+    cfg = dataclasses.replace(setups.c3_wind(512, 256, setups.COOL_NONE), krad=0.0)
+    yield "wind_nocool_k0_512x256_200steps", cfg, setups.wind_ic(cfg), 200
+
+
+def _oracle_chunks(cfg, V, steps):
+    t, dt, done, out = 0.0, cfg.first_dt, 0, []
+    for s in steps:
+        r = refrun.run_wind_oracle(cfg, V, s - done, dt0=dt, t0=t)
+        V, t, dt, done = r["V"], r["t"], r["dt"], s
+        out.append((s, V, t, dt))
+    return out
+
+
+def main(which=("hydro", "wind", "cvidl", "roe", "restart", "output", "tstop")):
     refrun.build(with_reference=True)
+    for name, cfg, V0, n in (libm_free_cases() if "libmfree" in which else ()):
+        inp = wind_inputs(cfg)
+        r = refrun.run_reference(cfg, V0, n, fluxes=(inp["flux_r"], inp["flux_t"], inp["flux_p"]),
+                                 timeout=4 * 3600)
+        d = state_digest(r["V"], r["U"], r["t"], r["dt"], r["dt_seq"])
+        d.update(config=dataclasses.asdict(cfg), ic="setups.wind_ic (seed 1234)", nsteps=n)
+        (OUT / f"{name}.json").write_text(json.dumps(d, indent=1) + "\n")
+        print(f"{name}: {n} steps, t={r['t']:.6e}, next dt={r['dt']:.6e}")
+    for name, cfg, V0 in (trajectory_cases() if "trajectory" in which else ()):
+        steps = TRAJ_STEPS + (80, 100, 140, 200)
+        base = _oracle_chunks(cfg, V0, steps)
+        ref200 = np.load(OUT / "wind_plain_512x256_200steps_sample.npz")
+        nv = V0.shape[0]
+        assert np.array_equal(base[-1][1].reshape(nv, -1)[:, ::int(ref200["stride"])],
+                              ref200["V_sample"]), "oracle left the reference's trajectory"
+        Vp = V0.copy()
+        Vp[0] = np.nextafter(V0[0], np.inf)
+        pert = _oracle_chunks(cfg, Vp, steps)
+        dev = np.array([[np.abs(a[1][q] - b[1][q]).max() / np.abs(b[1][q]).max()
+                         for q in range(nv)] for a, b in zip(pert, base)])
+        keep = [i for i, s in enumerate(steps) if s in TRAJ_STEPS]
+        meta = dataclasses.asdict(cfg)
+        np.savez_compressed(
+            OUT / f"{name}.npz", steps=np.array(steps), sample_steps=np.array(TRAJ_STEPS),
+            stride=np.array(TRAJ_STRIDE),
+            V_sample=np.stack([base[i][1].reshape(nv, -1)[:, ::TRAJ_STRIDE] for i in keep]),
+            t=np.array([b[2] for b in base]), dt=np.array([b[3] for b in base]),
+            perturbed_dev=dev, config=np.array(json.dumps(meta)))
+        print(f"{name}: 1-ulp density perturbation after " +
+              ", ".join(f"{s}: {d.max():.1e}" for s, d in zip(steps, dev)))
+    for name, cfg, V0, n in (fullsize_wind_cases() if "fullwind" in which else ()):
+        inp = wind_inputs(cfg)
+        r = refrun.run_reference(cfg, V0, n, fluxes=(inp["flux_r"], inp["flux_t"], inp["flux_p"]),
+                                 timeout=4 * 3600)
+        nv = r["V"].shape[0]
+        Vf, Uf = r["V"].reshape(nv, -1), r["U"].reshape(nv, -1)
+        meta = dataclasses.asdict(cfg)
+        np.savez_compressed(OUT / f"{name}.npz", V_sample=Vf[:, ::SAMPLE_STRIDE],
+                            U_sample=Uf[:, ::SAMPLE_STRIDE], stride=np.array(SAMPLE_STRIDE),
+                            sum_V=Vf.sum(axis=1), sum_U=Uf.sum(axis=1), dt_seq=r["dt_seq"],
+                            t=np.array(r["t"]), dt=np.array(r["dt"]), nsteps=np.array(n),
+                            config=np.array(json.dumps(meta)))
+        print(f"{name}: {n} steps, t={r['t']:.6e}, next dt={r['dt']:.6e}")
+    for name, cfg, V0, n in (tstop_cases() if "tstop" in which else ()):
+        kw = {}
+        if hasattr(cfg, "nangles"):
+            inp = wind_inputs(cfg)
+            kw = {"fluxes": (inp["flux_r"], inp["flux_t"], inp["flux_p"])}
+        else:
+            inp = {}
+        r = refrun.run_reference(cfg, V0, n, **kw)
+        steps = len(r["dt_seq"])
+        assert steps < n, name
+        meta = dataclasses.asdict(cfg)
+        np.savez_compressed(OUT / f"{name}.npz", V0=V0, V=r["V"], U=r["U"],
+                            dt_seq=r["dt_seq"], t=np.array(r["t"]), dt=np.array(r["dt"]),
+                            nsteps=np.array(steps), tstop=np.array(cfg.tstop),
+                            config=np.array(json.dumps(meta)), **inp)
+        print(f"{name}: {steps} steps to tstop {cfg.tstop!r}, t={r['t']!r}, next dt={r['dt']:.6e}")
+    for name, cfg, seed, n in (digest_cases() if "digest" in which else ()):
+        V0 = setups.sedov(cfg, seed=seed)
+        r = refrun.run_reference(cfg, V0, n, timeout=4 * 3600)
+        d = state_digest(r["V"], r["U"], r["t"], r["dt"], r["dt_seq"])
+        d.update(config=dataclasses.asdict(cfg), seed=seed, nsteps=n)
+        (OUT / f"{name}.json").write_text(json.dumps(d, indent=1) + "\n")
+        print(f"{name}: {n} steps, t={r['t']:.6e}, next dt={r['dt']:.6e}")
     wind = list(wind_cases()) if "wind" in which else []
     wind += list(cvidl_cases()) if "cvidl" in which else []
     wind += list(roe_cases()) if "roe" in which else []
@@ -236,4 +401,4 @@ def main(which=("hydro", "wind", "cvidl", "roe", "restart", "output")):
 
 
 if __name__ == "__main__":
-    main(tuple(sys.argv[1:]) or ("hydro", "wind", "cvidl", "roe", "restart", "output"))
+    main(tuple(sys.argv[1:]) or ("hydro", "wind", "cvidl", "roe", "restart", "output", "tstop"))

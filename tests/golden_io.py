@@ -10,8 +10,9 @@ GOLDEN = pathlib.Path(__file__).resolve().parent / "golden"
 
 
 def names(kind: str = "hydro"):
-    """Fixture names: kind "hydro" (Cartesian step) or "wind"."""
-    all_ = sorted(p.stem for p in GOLDEN.glob("*.npz"))
+    """Fixture names: kind "hydro" (Cartesian step) or "wind".  States too
+    large to store whole (*_sample: a subsample of cells) are not listed."""
+    all_ = sorted(p.stem for p in GOLDEN.glob("*.npz") if not p.stem.endswith("_sample"))
     wind = [n for n in all_ if n.startswith("wind_")]
     return wind if kind == "wind" else [n for n in all_ if n not in wind]
 

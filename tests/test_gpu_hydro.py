@@ -206,7 +206,8 @@ def test_full_size_periodic_conservation(sedov256):
 
 def test_dt_too_small_stop_is_reported():
     """NextTimeStep's stop (dt < 1e-9 first_dt, Src/main.c:672-683) sets
-    PMX_FLAG_DT_TOO_SMALL, which stays set; a normal run leaves it clear."""
+    PMX_FLAG_DT_TOO_SMALL, which stays set and halts the main loop after
+    the step, where the oracle quits too; a normal run leaves it clear."""
     import dataclasses
     base = setups.c1_kh(32)
     V0 = setups.kelvin_helmholtz(base, seed=3)
@@ -218,10 +219,61 @@ def test_dt_too_small_stop_is_reported():
         h.set_time(0.0, base.first_dt)
         h.run(2)
         h.sync()
-        assert h.get_time()[3] & plutomx.PMX_FLAG_DT_TOO_SMALL
-        h.run(1)
+        t, dt, _, flags = h.get_time()
+        assert flags & plutomx.PMX_FLAG_DT_TOO_SMALL
+        V = h.get_prim()
+        # the reference quits here: further steps of the loop do nothing
+        h.run(3)
         h.sync()
-        assert h.get_time()[3] & plutomx.PMX_FLAG_DT_TOO_SMALL
+        t1, dt1, _, flags = h.get_time()
+        assert flags & plutomx.PMX_FLAG_DT_TOO_SMALL and (t1, dt1) == (t, dt)
+        assert np.array_equal(h.get_prim(), V)
+        # set_time clears the stop
+        h.set_time(t, dt)
+        assert not h.get_time()[3] & plutomx.PMX_FLAG_DT_TOO_SMALL
+    ref = refrun.run_oracle(cfg, V0, 5, dt0=base.first_dt)
+    assert ref["quit"] == "dt_too_small" and ref["steps"] == 1
+    assert_bitwise(V, ref["V"], "V at the stop")
+    assert (t, dt) == (ref["t"], ref["dt"])
+
+
+TSTOP = [n for n in golden_io.names() if "_tstop_" in n]
+
+
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("name", TSTOP)
+def test_tstop_cut_and_halt(name, graph):
+    """pluto.ini tstop (Src/main.c:226-229): the step that would pass tstop
+    (or reach it to 1e-8 relative) is cut to land on it and is the last.
+    Reference fixtures end at an odd and an even step count and inside the
+    tolerance.  Steps asked for beyond it are no-ops (the run halts with
+    PMX_FLAG_TSTOP), eagerly and under graph replay, also when the run is
+    split; set_tstop then continues the run to a later tstop as a restart
+    from the halted state would (oracle)."""
+    cfg, g = golden_io.load(name)
+    n = int(g["nsteps"])
+    say(f"[gpu] tstop {name} graph={graph}")
+    with plutomx.Hydro(cfg) as h:
+        h.set_prim(g["V0"])
+        h.set_time(0.0, cfg.first_dt)
+        h.run(3, graph=graph)
+        h.run(n + 4, graph=graph)
+        h.sync()
+        t, dt, _, flags = h.get_time()
+        assert flags & plutomx.PMX_FLAG_TSTOP
+        assert_bitwise(h.get_prim(), g["V"], "V")
+        assert_bitwise(h.get_cons(), g["U"], "U")
+        assert t == float(g["t"]) == float(g["tstop"]) and dt == float(g["dt"])
+        t2 = t + 2.5 * dt
+        h.set_tstop(t2)
+        assert not h.get_time()[3] & plutomx.PMX_FLAG_TSTOP
+        h.run(10, graph=graph)
+        h.sync()
+        ref = refrun.run_oracle(dataclasses.replace(cfg, tstop=t2), g["V"], 10, dt0=dt, t0=t)
+        assert ref["steps"] == 3 and ref["quit"] == "tstop"
+        assert_bitwise(h.get_prim(), ref["V"], "V after the second tstop")
+        t3, dt3, _, flags = h.get_time()
+        assert t3 == ref["t"] == t2 and dt3 == ref["dt"] and flags & plutomx.PMX_FLAG_TSTOP
 
 
 def test_full_size_sedov_self_similar_radius():

@@ -252,7 +252,35 @@ def test_wind_dt_too_small_stop_is_reported(pmx):
             w.set_time(0.0, base.first_dt, 0)
             w.run(4)
             w.sync()
-            assert bool(w.get_time()["flags"] & pmx.PMX_FLAG_DT_TOO_SMALL) == expect
+            tm = w.get_time()
+            assert bool(tm["flags"] & pmx.PMX_FLAG_DT_TOO_SMALL) == expect
+            # the reference quits at the first NextTimeStep (after step 1,
+            # with cooling): the loop halts there
+            assert tm["step"] == (2 if expect else 4)
+
+
+@pytest.mark.parametrize("name", [n for n in golden_io.names("wind") if "_tstop_" in n])
+def test_wind_tstop_cut_and_halt(pmx, name):
+    """pluto.ini tstop for the wind's main loop (Src/main.c:226-229): the
+    reference fixtures end at an odd and an even step (the Strang order and
+    the every-second-step NextTimeStep depend on the parity); asked for more
+    steps, the eager loop and the graph replay both stop there with
+    PMX_FLAG_TSTOP and report the steps actually taken."""
+    cfg, g = golden_io.load(name)
+    fluxes, grid = fixture_inputs(g)
+    n = int(g["nsteps"])
+    for graph in (False, True):
+        with pmx.Wind(cfg, fluxes=fluxes, grid=grid) as w:
+            w.set_prim(g["V0"])
+            w.set_time(0.0, cfg.first_dt, 0)
+            w.run(n + 5, graph=graph)
+            w.sync()
+            tm = w.get_time()
+            assert tm["flags"] & pmx.PMX_FLAG_TSTOP and tm["step"] == n, (graph, tm)
+            assert_close(w.get_prim(), g["V"], f"{name} V graph={graph}")
+            assert_close(w.get_cons(), g["U"], f"{name} U graph={graph}")
+            assert tm["t"] == float(g["tstop"]) or abs(tm["t"] - float(g["t"])) <= RTOL * tm["t"]
+            assert abs(tm["dt"] - float(g["dt"])) <= RTOL * float(g["dt"])
 
 
 @pytest.mark.parametrize("cooling", [setups.COOL_BLONDIN, setups.COOL_PLAIN])
