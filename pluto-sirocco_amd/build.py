@@ -32,9 +32,15 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=
          "-I", str(HERE.parent / "include"), "-I", str(CSRC)]
 
 
+# Per translation unit: the Cartesian stage kernel runs 1.7% faster with
+# LLVM's iterative ILP scheduling (Sedov 256^3: 1.284 vs 1.307 ms per stage,
+# bit-identical); the other kernels keep the default scheduler.
+TU_FLAGS = {"pmx_hydro": ["-mllvm", "-amdgpu-sched-strategy=iterative-ilp"]}
+
+
 def _compile(src: pathlib.Path, extra=(), build_dir=BUILD) -> pathlib.Path:
     obj = build_dir / (src.stem + ".o")
-    cmd = [HIPCC, *FLAGS, *extra, "-c", str(src), "-o", str(obj)]
+    cmd = [HIPCC, *FLAGS, *TU_FLAGS.get(src.stem, []), *extra, "-c", str(src), "-o", str(obj)]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{proc.stderr[-6000:]}")

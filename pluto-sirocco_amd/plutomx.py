@@ -243,24 +243,33 @@ def copy_probe(nbytes: int, reps: int = 10) -> float:
     return ms.value
 
 
+def hydro_desc(cfg: setups.HydroConfig, keep: list) -> HydroDesc:
+    """The pmx_hydro_desc of a configuration; the cell-width arrays it
+    points to are appended to `keep` (they must outlive its use)."""
+    d = HydroDesc()
+    d.dims = cfg.dims
+    d.nx1, d.nx2 = cfg.nx[0], cfg.nx[1]
+    d.nx3 = cfg.nx[2] if cfg.dims == 3 else 1
+    d.recon = PMX_REC_PPM if cfg.recon == setups.PPM else REC_CODE[(cfg.recon, cfg.limiter)]
+    d.solver, d.rk = cfg.solver, cfg.rk
+    for s in range(6):
+        d.bc[s] = cfg.bc[s]
+    d.gamma, d.cfl, d.cfl_max_var = cfg.gamma, cfg.cfl, cfg.cfl_max_var
+    d.first_dt = cfg.first_dt
+    dx = [cell_widths(cfg, q) for q in range(3)]
+    keep.extend(dx)
+    d.dx1, d.dx2, d.dx3 = (_dptr(x) for x in dx)
+    return d
+
+
 class Hydro:
     """One grid advanced on the GPU by libplutomx."""
 
     def __init__(self, cfg: setups.HydroConfig):
         self.cfg = cfg
         lib = load_library()
-        d = HydroDesc()
-        d.dims = cfg.dims
-        d.nx1, d.nx2 = cfg.nx[0], cfg.nx[1]
-        d.nx3 = cfg.nx[2] if cfg.dims == 3 else 1
-        d.recon = PMX_REC_PPM if cfg.recon == setups.PPM else REC_CODE[(cfg.recon, cfg.limiter)]
-        d.solver, d.rk = cfg.solver, cfg.rk
-        for s in range(6):
-            d.bc[s] = cfg.bc[s]
-        d.gamma, d.cfl, d.cfl_max_var = cfg.gamma, cfg.cfl, cfg.cfl_max_var
-        d.first_dt = cfg.first_dt
-        self._dx = [cell_widths(cfg, q) for q in range(3)]
-        d.dx1, d.dx2, d.dx3 = (_dptr(x) for x in self._dx)
+        self._dx = []
+        d = hydro_desc(cfg, self._dx)
         h = ctypes.c_void_p()
         _check(lib.pmx_hydro_create(ctypes.byref(d), ctypes.byref(h)), "pmx_hydro_create")
         self._h = h
@@ -340,6 +349,13 @@ class Hydro:
 
     def sync(self) -> None:
         _check(self._lib.pmx_hydro_sync(self._h), "sync")
+
+    def device_state(self):
+        """Device addresses of the current V and U ([nv][k][j][i])."""
+        V, U = ctypes.c_void_p(), ctypes.c_void_p()
+        _check(self._lib.pmx_hydro_device_state(self._h, ctypes.byref(V), ctypes.byref(U)),
+               "device_state")
+        return V, U
 
     def stage_timing(self, enable: bool, reset: bool = True):
         ms, n = ctypes.c_double(), ctypes.c_int()

@@ -7,6 +7,7 @@ import re
 import pytest
 
 import plutomx
+import setups
 
 ROOT = pathlib.Path(__file__).resolve().parents[1]
 
@@ -93,3 +94,4 @@ def test_wind_descriptor_validation_without_gpu():
     for cfg, msg in zip(bad, msgs):
         with pytest.raises(plutomx.PmxError, match=msg):
             plutomx.Wind(cfg)
+
