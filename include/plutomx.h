@@ -119,6 +119,22 @@ int pmx_hydro_set_tstop(pmx_hydro *h, double tstop);
  * entry, nothing returned.  Synchronises the stream. */
 int pmx_hydro_advance_step(pmx_hydro *h, double dt, double *inv_dt_hyp);
 
+/* No counterpart in the reference: pmx_hydro_advance_step for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  No context: desc as for
+ * pmx_hydro_create; V (in: V^n, out: V^{n+1}) and U (out: Uc after the last
+ * stage) are the caller's DEVICE arrays [nv][k][j][i] of the interior; dt
+ * and *inv_dt_hyp as in pmx_hydro_advance_step.  The intermediate stage
+ * states, the geometry tables and the time scalars live in one fixed
+ * device array of the library (40 MiB on each GPU it runs on, held until
+ * the process ends), so the grid may hold at most 524288 cells with
+ * nx1 + nx2 + nx3 <= 8192 (larger grids are refused with an error).  The
+ * call allocates nothing, runs on the null stream and waits for it.  Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (threads, or streams other than the null stream) race. */
+int pmx_hydro_advance_step_noalloc(const pmx_hydro_desc *desc, double *V, double *U, double dt,
+                                   double *inv_dt_hyp);
+
 /* nsteps iterations of the reference's main loop body for this path
  * (Integrate + time update + NextTimeStep, Src/main.c:215-337,
  * :383-485, :521-697) entirely on the device: dt and t stay in device

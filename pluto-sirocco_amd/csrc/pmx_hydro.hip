@@ -581,6 +581,106 @@ int pmx_hydro_stage_timing(pmx_hydro* h, int enable, int reset, double* total_ms
   return 0;
 }
 
+// ---------------------------------------------------------------------
+// The step without allocation: pmx_hydro_advance_step for callers that
+// cannot allocate device memory (code under HIP graph capture, tight
+// serving loops, harnesses that forbid hipMalloc).  The caller passes the
+// state; everything else the step needs (the intermediate stage states,
+// the dt/dx tables, the time scalars) lives in `scratch`, a fixed array
+// in device memory that every GPU holds for the life of the process.
+// ---------------------------------------------------------------------
+}  // extern "C"
+
+namespace {
+
+constexpr long long NOALLOC_MAX_CELLS = 1ll << 19;   // 2 stage states of 5 doubles
+constexpr int NOALLOC_MAX_LINE = 8192;               // nx + ny + nz
+// [V1 | V2] stage states, then dx, dt/dx, 1/dx tables, the dt slots and
+// the time scalars: 40 MiB + 192 KiB
+constexpr size_t NOALLOC_DOUBLES = 2 * (size_t)NV * NOALLOC_MAX_CELLS +
+                                   3 * (size_t)NOALLOC_MAX_LINE + 2 + 16;
+static_assert(sizeof(DevScalars) <= 16 * sizeof(double), "scalar slots");
+
+__device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+double* g_scratch[NOALLOC_MAX_DEVICES];   // scratch's address on each device, once looked up
+
+// scratch's device address on the current device (host code never passes
+// the array's name itself to a kernel: on the host it is a stand-in).
+int noalloc_scratch(double** out) {
+  int dev = 0;
+  PMX_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) { set_error("device index out of range"); return -1; }
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    PMX_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch[dev];
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pmx_hydro_advance_step_noalloc(const pmx_hydro_desc* desc, double* V, double* U, double dt,
+                                   double* inv_dt_hyp) {
+  if (!desc || !V || !U) { set_error("null argument"); return -1; }
+  if (check_desc(desc)) return -1;
+  const int nx = desc->nx1, ny = desc->nx2, nz = desc->dims == 3 ? desc->nx3 : 1;
+  const long long ncell = (long long)nx * ny * nz;
+  if (ncell > NOALLOC_MAX_CELLS || nx + ny + nz > NOALLOC_MAX_LINE) {
+    set_error("pmx_hydro_advance_step_noalloc: the grid (" + std::to_string(ncell) +
+              " cells, nx+ny+nz = " + std::to_string(nx + ny + nz) + ") exceeds the limit of " +
+              std::to_string(NOALLOC_MAX_CELLS) + " cells and nx+ny+nz <= " +
+              std::to_string(NOALLOC_MAX_LINE) + " (use pmx_hydro_create)");
+    return -1;
+  }
+  double* base = nullptr;
+  if (noalloc_scratch(&base)) return -1;
+  // a context on the stack whose device buffers are the caller's state and
+  // slices of scratch; the step runs on the null stream and is waited for
+  pmx_hydro h;
+  h.d = *desc;
+  h.nx = nx; h.ny = ny; h.nz = nz;
+  h.ncell = ncell;
+  h.cz = z_chunk(nz);
+  h.V[0] = V;
+  h.V[1] = base;
+  h.V[2] = base + (size_t)NV * ncell;
+  h.U = U;
+  double* tab = base + 2 * (size_t)NV * NOALLOC_MAX_CELLS;
+  const int ntot = nx + ny + nz;
+  h.dx = tab;
+  h.dtdx = tab + ntot;
+  h.inv_dl = tab + 2 * ntot;
+  h.dt_in = tab + 3 * NOALLOC_MAX_LINE;
+  h.s = reinterpret_cast<DevScalars*>(h.dt_in + 2);
+  h.stream = nullptr;
+  h.have_graph = false;
+  h.timing = false;
+  h.timed_ms = 0.0;
+  h.timed_launches = 0;
+  std::vector<double> dx, inv;
+  host_tables(desc, nx, ny, nz, dx, inv);
+  DevScalars s0;
+  memset(&s0, 0, sizeof(s0));       // what the last call left is cleared
+  s0.tstop = HUGE_VAL;
+  const double in[2] = {dt, inv_dt_hyp ? *inv_dt_hyp : 0.0};
+  PMX_HIP_CHECK(hipMemcpyAsync(h.dx, dx.data(), sizeof(double) * ntot, hipMemcpyHostToDevice, 0));
+  PMX_HIP_CHECK(hipMemcpyAsync(h.inv_dl, inv.data(), sizeof(double) * ntot,
+                               hipMemcpyHostToDevice, 0));
+  PMX_HIP_CHECK(hipMemcpyAsync(h.s, &s0, sizeof(s0), hipMemcpyHostToDevice, 0));
+  PMX_HIP_CHECK(hipMemcpyAsync(h.dt_in, in, sizeof(in), hipMemcpyHostToDevice, 0));
+  if (advance(&h, h.dt_in)) return -1;
+  PMX_HIP_CHECK(hipMemcpyAsync(&s0, h.s, sizeof(s0), hipMemcpyDeviceToHost, 0));
+  PMX_HIP_CHECK(hipStreamSynchronize(0));
+  if (inv_dt_hyp) *inv_dt_hyp = s0.inv_dt_hyp;
+  return 0;
+}
+
 int pmx_copy_probe(long long nbytes, int reps, double* ms_per_copy) {
   if (nbytes <= 0 || nbytes % 16 != 0 || reps <= 0) { set_error("bad argument"); return -1; }
   double2 *a = nullptr, *b = nullptr;

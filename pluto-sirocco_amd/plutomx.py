@@ -50,7 +50,7 @@ EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro
            "pmx_sirocco_read_heatcool", "pmx_sirocco_read_prefactors",
            "pmx_wind_snapshot", "pmx_wind_snapshot_wait", "pmx_write_dbl", "pmx_write_restart",
            "pmx_read_restart", "pmx_read_dbl", "pmx_wind_snapshot_host", "pmx_wind_run_graph",
-           "pmx_hydro_set_tstop", "pmx_wind_set_tstop"]
+           "pmx_hydro_set_tstop", "pmx_wind_set_tstop", "pmx_hydro_advance_step_noalloc"]
 
 
 class PmxError(RuntimeError):
@@ -124,6 +124,8 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     lib.pmx_hydro_get_cons.argtypes = [vp, dp, ctypes.c_int]
     lib.pmx_hydro_set_time.argtypes = [vp, ctypes.c_double, ctypes.c_double]
     lib.pmx_hydro_set_tstop.argtypes = [vp, ctypes.c_double]
+    lib.pmx_hydro_advance_step_noalloc.argtypes = [ctypes.POINTER(HydroDesc), vp, vp,
+                                                   ctypes.c_double, dp]
     lib.pmx_hydro_get_time.argtypes = [vp, dp, dp, dp, ip]
     lib.pmx_hydro_advance_step.argtypes = [vp, ctypes.c_double, dp]
     lib.pmx_hydro_run.argtypes = [vp, ctypes.c_int]

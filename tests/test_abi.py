@@ -95,3 +95,16 @@ def test_wind_descriptor_validation_without_gpu():
         with pytest.raises(plutomx.PmxError, match=msg):
             plutomx.Wind(cfg)
 
+
+def test_noalloc_step_refuses_grids_over_its_limit():
+    """pmx_hydro_advance_step_noalloc checks the grid against the size of
+    its fixed device array before any device work."""
+    lib = plutomx.load_library()
+    cfg = setups.c2_sedov(128)            # 2.1 M cells, over the 524288-cell limit
+    keep = []
+    d = plutomx.hydro_desc(cfg, keep)
+    inv = ctypes.c_double(0.0)
+    rc = lib.pmx_hydro_advance_step_noalloc(ctypes.byref(d), ctypes.c_void_p(16),
+                                            ctypes.c_void_p(16), 1.e-3, ctypes.byref(inv))
+    assert rc == -1
+    assert b"exceeds the limit of 524288 cells" in lib.pmx_last_error()
