@@ -41,6 +41,8 @@ HBM_PEAK_GBS = 8000.0
 # size reads on this device), mean over the three RK3 stage launches; the
 # commands and per-launch values are in profiles/stage_kernel_sedov256.md.
 TRAFFIC_BYTES_PER_LAUNCH_256 = 3.8061e9
+TRAFFIC_SOURCE = ("profiles/stage_kernel_sedov256.md: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
+                  "of this kernel (a profile value, not measured in this run)")
 
 
 def algorithmic_bytes_per_cell(rk: int) -> list:
@@ -67,6 +69,67 @@ def cpu_baseline(n_sample: int = 128, steps: int = 3) -> dict | None:
     return {"value": round(value, 4), "unit": "Mcell-updates/s", "cores": 1, "kind": kind,
             "sample": f"Sedov {n_sample}^3 PPM+HLLC+RK3, {nst} steps, 1 process on "
                       f"{os.cpu_count()} host cores ({wall:.2f} s)"}
+
+
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo, as lscpu reports it)."""
+    try:
+        for line in pathlib.Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline_all_cores(n_sample: int = 128, steps: int = 3) -> dict | None:
+    """The reference CPU build on every host core this run may use: one
+    independent serial process per core on the same bounded sample (there
+    is no MPI here), aggregate cell-updates over the wall time of the
+    slowest (the throughput analogue of a sweep of independent runs)."""
+    import concurrent.futures as cf
+    from oracle import refrun
+    cfg = setups.c2_sedov(n_sample)
+    if not refrun.reference_binary(cfg.reference_problem()).exists():
+        return None
+    V0 = setups.sedov(cfg, seed=1234)
+    cores = min(os.cpu_count() or 1, 16)   # the GPU's share of a shared host
+    with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+        runs = list(ex.map(lambda _: refrun.run_reference(cfg, V0, steps, time_only=True),
+                           range(cores)))
+    wall = max(r["wall"] for r in runs)
+    nst = runs[0]["steps_timed"]
+    return {"value": round(cfg.ncells * nst * cores / wall / 1e6, 4), "unit": "Mcell-updates/s",
+            "cores": cores, "kind": "reference", "cpu": cpu_model(),
+            "sample": f"Sedov {n_sample}^3 PPM+HLLC+RK3, {nst} steps, {cores} concurrent "
+                      f"processes (slowest {wall:.2f} s)"}
+
+
+def kh_leg(n: int = 256, steps: int = 100, warmup: int = 10) -> dict:
+    """The 2-D Kelvin-Helmholtz workload (256^2, PLM van Leer + HLLC + RK2,
+    CFL 0.4, periodic) for its stated 100 steps, graph-replayed: a 65,536-
+    cell grid is launch-latency bound (23.6 MB of algorithmic traffic per
+    step, 3 us at the HBM peak), so the replay is what a driver would use."""
+    import plutomx
+    cfg = setups.c1_kh(n)
+    V0 = setups.kelvin_helmholtz(cfg, seed=1234)
+    with plutomx.Hydro(cfg) as h:
+        h.set_prim(V0)
+        h.set_time(0.0, cfg.first_dt)
+        h.run(warmup, graph=True)
+        h.sync()
+        out = {}
+        for graph in (True, False):
+            t0 = time.perf_counter()
+            h.run(steps, graph=graph)
+            h.sync()
+            el = time.perf_counter() - t0
+            out["graph" if graph else "eager"] = {
+                "Mcell_updates_per_s": round(cfg.ncells * steps / el / 1e6, 2),
+                "ms_per_step": round(1e3 * el / steps, 4)}
+    out["workload"] = (f"2-D KH {n}^2, PLM(van Leer)+HLLC+RK2, CFL 0.4, periodic, "
+                       f"{steps} steps (the metric's CPU-runnable configuration)")
+    return out
 
 
 def wind_cpu_baseline(nr: int = 256, nt: int = 128, steps: int = 4) -> dict | None:
@@ -313,6 +376,7 @@ def main() -> int:
     traffic = TRAFFIC_BYTES_PER_LAUNCH_256 if args.n == 256 else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "traffic_source": TRAFFIC_SOURCE if traffic else None,
                 "kernel": "stage_kernel (fused PPM+HLLC stage, all directions)",
                 "mean_launch_ms": round(mean_launch_ms, 4),
                 "algorithmic_bytes_per_launch": bytes_per_launch,
@@ -330,7 +394,9 @@ def main() -> int:
                               "Mcell_updates_per_s": round(cfg.ncells / host_ms / 1e3, 2),
                               "what": "set_prim + 1 step + get_prim of the full state from "
                                       "pageable host memory (not the headline value)"},
-           "cpu_baseline": None if args.no_cpu else cpu_baseline()}
+           "cpu_baseline": None if args.no_cpu else cpu_baseline(),
+           "cpu_baseline_all_cores": None if args.no_cpu else cpu_baseline_all_cores(),
+           "kh_256": kh_leg()}
     if not args.no_wind:
         out["wind"] = wind_legs(args.wind_steps, cpu=not args.no_cpu)
     print(json.dumps(out), flush=True)

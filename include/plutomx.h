@@ -242,6 +242,15 @@ typedef struct {
    *     exp2(-alpha log2 dvds) per bin: a few 1e-15 relative from the
    *     reference's M, faster. */
   int force_power;
+  /* The cv_idl disk (parameters RHO_0 [g/cm^3], RHO_ALPHA, DISK_MDOT [g/s]):
+   * with dfloor > 0 and shock_flattening (the production set-up, whose
+   * MULTID runs the reference with 3 ghost zones), the internal-boundary
+   * pass of cv_idl/init.c:276-311 resets the last interior polar row at
+   * every stage, after the floors, when that row's centre and volume
+   * centroid agree to 1e-6 (from about 48 polar zones on): the disk
+   * density rho_0 (r / r_in)^-alpha with the polar momentum kept, v_r = 0,
+   * the Keplerian v_phi, the pressure of the disk temperature, tracer 1. */
+  double rho_0, rho_alpha, disk_mdot;
 } pmx_wind_desc;
 #define PMX_FORCE_POWER_REF      0
 #define PMX_FORCE_POWER_FACTORED 1

@@ -266,7 +266,9 @@ class _OrcWindCfg(ctypes.Structure):
                 ("solver", ctypes.c_int),
                 ("xi", _dp), ("t_r", _dp), ("pre", _dp),
                 ("fm_points", ctypes.c_int), ("fm_t", _dp), ("fm_m", _dp),
-                ("tstop", ctypes.c_double), ("first_dt", ctypes.c_double)]
+                ("tstop", ctypes.c_double), ("first_dt", ctypes.c_double),
+                ("rho_0", ctypes.c_double), ("rho_alpha", ctypes.c_double),
+                ("disk_mdot", ctypes.c_double)]
 
 
 def _wind_struct(cfg, fluxes=None, grid=None, sirocco=None):
@@ -287,6 +289,9 @@ def _wind_struct(cfg, fluxes=None, grid=None, sirocco=None):
         L.orc_wind_cool.restype = ctypes.c_int
         L.orc_wind_cool_time.argtypes = [ctypes.POINTER(_OrcWindCfg)] + [ctypes.c_double] * 6
         L.orc_wind_cool_time.restype = ctypes.c_double
+        L.orc_wind_cool_time_pre.argtypes = ([ctypes.POINTER(_OrcWindCfg)] +
+                                             [ctypes.c_double] * 6 + [dp])
+        L.orc_wind_cool_time_pre.restype = ctypes.c_double
         L._wind_bound = True
     gr, gt = grid if grid is not None else setups.wind_grid(cfg)
     fr, ft, fp = fluxes if fluxes is not None else setups.wind_fluxes(cfg)
@@ -316,6 +321,7 @@ def _wind_struct(cfg, fluxes=None, grid=None, sirocco=None):
     c.solver = int(cfg.solver)
     c.tstop = getattr(cfg, "tstop", 1.e30)
     c.first_dt = cfg.first_dt
+    c.rho_0, c.rho_alpha, c.disk_mdot = cfg.rho_0, cfg.rho_alpha, cfg.disk_mdot
     if sirocco is not None:
         if sirocco.xi is not None:
             xi, t_r = sirocco.heatcool_floored()
@@ -380,7 +386,7 @@ def run_wind_cool_oracle(cfg, V0: np.ndarray, dt: float, t: float = 0.0, xi=None
 
 WIND_COUNTERS = ("rkf12_ok", "ck45_calls", "ck45_sub", "ck45_reject", "bl_solve",
                  "bl_widen", "bl_second", "below_minT", "dt_cfl", "dt_cool", "dt_maxvar",
-                 "c2p_fail", "floor_T", "floor_cells", "shock_zones")
+                 "c2p_fail", "floor_T", "floor_cells", "shock_zones", "midplane_cells")
 
 
 def wind_counters() -> dict:
@@ -390,8 +396,13 @@ def wind_counters() -> dict:
     return dict(zip(WIND_COUNTERS, list(out)))
 
 
-def wind_cool_time(cfg, rho, p0, xi, p1, tol):
+def wind_cool_time(cfg, rho, p0, xi, p1, tol, pre=None):
     """Exact time for p0 -> p1 under the cooling ODE (oracle
-    orc_wind_cool_time); NaN if unreachable."""
+    orc_wind_cool_time); NaN if unreachable.  ``pre``: the five cooling
+    prefactors (comp_h, comp_c, xray_h, brem_c, line_c), default 1."""
     c, keep = _wind_struct(cfg)
-    return lib().orc_wind_cool_time(ctypes.byref(c), rho, p0, xi, cfg.t_x, p1, tol)
+    if pre is None:
+        return lib().orc_wind_cool_time(ctypes.byref(c), rho, p0, xi, cfg.t_x, p1, tol)
+    pre = np.ascontiguousarray(pre, dtype=np.float64)
+    return lib().orc_wind_cool_time_pre(ctypes.byref(c), rho, p0, xi, cfg.t_x, p1, tol,
+                                        _ptr(pre))

@@ -110,6 +110,9 @@ typedef struct {
                                  file (the reader keeps log10 of both)       */
   double tstop;               /* runtime.tstop                                */
   double first_dt;            /* runtime.first_dt                             */
+  /* the cv_idl disk of the midplane row (parameters RHO_0 [g/cm^3],
+   * RHO_ALPHA, DISK_MDOT [g/s]; cv_idl/init.c:276-311) */
+  double rho_0, rho_alpha, disk_mdot;
 } orc_wind_cfg;
 
 /* Advance V ([6][nx2][nx1], in/out) by nsteps steps of the reference's
@@ -144,12 +147,17 @@ int orc_wind_cool(const orc_wind_cfg *cfg, double *V, double dt, double t, const
  * way (p1 cannot be reached). */
 double orc_wind_cool_time(const orc_wind_cfg *cfg, double rho, double p0, double xi, double t_r,
                           double p1, double tol);
+/* The same with the cell's five cooling prefactors pre[5] (comp_h, comp_c,
+ * xray_h, brem_c, line_c: cv_idl/radiat.c:118-123) in place of 1. */
+double orc_wind_cool_time_pre(const orc_wind_cfg *cfg, double rho, double p0, double xi,
+                              double t_r, double p1, double tol, const double *pre);
 
 /* Branch counters of the last orc_wind_run / orc_wind_cool: rkf12 accepted, ck45 calls,
  * ck45 substeps, ck45 rejections, BLONDIN solves, bracket widenings,
  * second (heat-cool root) solves, cells below the 1e4 K cut, dt limited by
  * CFL / cooling / CFL_max_var, cons->prim failures, temperature floors,
- * cells changed by the internal-boundary floors, zones flagged as shocked.
+ * cells changed by the internal-boundary floors, zones flagged as shocked,
+ * zones reset by the cv_idl midplane row.
  * Copies min(n, count) entries; returns the count. */
 int orc_wind_counters(long *out, int n);
 

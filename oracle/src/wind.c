@@ -45,7 +45,8 @@
 /* path counters, for tests that check which branches a case exercises */
 enum { CNT_RKF12_OK, CNT_CK45_CALLS, CNT_CK45_SUB, CNT_CK45_REJECT, CNT_BL_SOLVE,
        CNT_BL_WIDEN, CNT_BL_SECOND, CNT_BELOW_MINT, CNT_DT_CFL, CNT_DT_COOL,
-       CNT_DT_MAXVAR, CNT_C2P_FAIL, CNT_FLOOR_T, CNT_FLOOR_D, CNT_SHOCK, CNT_N };
+       CNT_DT_MAXVAR, CNT_C2P_FAIL, CNT_FLOOR_T, CNT_FLOOR_D, CNT_SHOCK, CNT_MIDPLANE,
+       CNT_N };
 static long cnt[CNT_N];
 
 int orc_wind_counters(long *out, int n)
@@ -68,6 +69,7 @@ int orc_wind_counters(long *out, int n)
 #define C_kB 1.3806505e-16
 #define C_mp 1.67262171e-24
 #define C_PI 3.14159265358979
+#define C_sigma 5.67051e-5
 #define C_sigmaT 6.6524e-25
 
 typedef struct {
@@ -97,6 +99,7 @@ typedef struct {
   /* tabulated force multiplier: log10 t [fm_points], log10 M [p][j][i]
      (interior layout), as read_sirocco_fluxes keeps them */
   double *fm_lt, *fm_lm;
+  int mid_row;                        /* the cv_idl midplane row, or -1     */
 } W;
 
 enum { FLAG_MINMOD = 1, FLAG_HLL = 4 };   /* Src/pluto.h:212-214 */
@@ -221,9 +224,7 @@ static void geometry(W *w)
 /* init.c:196-273: density, pressure and temperature floors on every    */
 /* cell (TOT_LOOP), momentum kept by scaling the velocities, the sound   */
 /* speed kept by the new pressure; PrimToCons of the changed cells.     */
-/* The midplane assignment of the same pass (:276-311) writes the first */
-/* ghost row beyond theta_max, which the X2-end reflective fill that    */
-/* follows overwrites, so it is not restated.                           */
+/* Then the pass's midplane row (:276-311): see midplane_cell.          */
 /* ------------------------------------------------------------------ */
 static int floor_cell(const W *w, double *v)
 {
@@ -256,6 +257,45 @@ static int floor_cell(const W *w, double *v)
   return changed;
 }
 
+/* The midplane row of the same pass (init.c:276-311).  The pass runs
+ * over every zone (TOT_LOOP) and resets those of row j == np_int + 2 when
+ * that row's centre x_glob and its volume centroid xgc agree to 1e-6
+ * (:280-281), which they do from about 48 polar zones on.  With 2 ghost
+ * zones that row is the first ghost row beyond theta_max, and the X2-end
+ * fill that follows overwrites it; with SHOCK_FLATTENING MULTID the
+ * reference runs with 3 (Src/get_nghost.c:48-57) and it is the LAST
+ * INTERIOR row: its density is set to the disk's rho_0 (r / r_in)^-alpha
+ * with the polar momentum kept, v_r to 0, v_phi to the Keplerian speed,
+ * the pressure to the Shakura-Sunyaev disk temperature's, the tracer to 1,
+ * after the floors and without a new Uc (PrimToCons ran before it).  r and
+ * theta are the zone's volume centroids (xgc). */
+static int midplane_row(const W *w)
+{
+  const orc_wind_cfg *c = w->c;
+  int j = NG + w->n2 - 1;
+  if (c->dfloor <= 0.0 || !c->shock_flattening) return -1;
+  return (fabs(w->x2[j] - w->xgc2[j])/w->x2[j]) < 1e-6 ? j : -1;
+}
+
+static void midplane_cell(const W *w, int i, double *v)
+{
+  const orc_wind_cfg *c = w->c;
+  const double rho_0 = c->rho_0/w->UD, rho_alpha = c->rho_alpha, disk_mdot = c->disk_mdot;
+  const double r_WD = w->x1l[NG];                  /* g_domBeg[IDIR] */
+  const double gm_cgs = C_G*c->cent_mass, gm_code = gm_cgs/(w->UL*w->UV*w->UV);
+  double r = w->xgc1[i], theta = w->xgc2[w->mid_row], rcyl = r*sin(theta);
+  double rho_mid = rho_0*pow((r/r_WD), -1.0*rho_alpha), teff, temp;
+  v[VX2] = (v[RHO]*v[VX2])/rho_mid;
+  v[RHO] = rho_mid;
+  v[VX1] = 0.0;
+  v[VX3] = sqrt(gm_code/r)*sin(theta);
+  teff = pow(3.0*gm_cgs*disk_mdot/(8.0*C_PI*C_sigma), 0.25);
+  teff *= pow(r_WD*w->UL, -0.75);
+  temp = teff*pow(r_WD/rcyl, 0.75)*pow(1.0 - sqrt(r_WD/rcyl), 0.25);
+  v[PRS] = rho_mid*temp/(w->KELVIN*c->mu);
+  v[TRC] = 1.0;
+}
+
 static void internal_floors(W *w)
 {
   int i, j, nv;
@@ -271,6 +311,11 @@ static void internal_floors(W *w)
          * Vc[ENTR], from the entropy the last cons->prim left */
         if (w->c->entropy_switch) w->Ue[CI(w, j, i)] = v[RHO]*w->S[CI(w, j, i)];
         cnt[CNT_FLOOR_D]++;
+      }
+      if (j == w->mid_row) {
+        midplane_cell(w, i, v);
+        for (nv = 0; nv < NVW; nv++) w->V[VI(w, nv, j, i)] = v[nv];
+        cnt[CNT_MIDPLANE]++;
       }
     }
 }
@@ -1277,6 +1322,7 @@ static void w_init(W *pw, const orc_wind_cfg *c, const double *Vin)
   w.U0e = calloc(nc, sizeof(double));
 
   geometry(&w);
+  w.mid_row = midplane_row(&w);
   for (a = 0; a < w.na; a++)
     for (j = 0; j < w.n2; j++)
       for (i = 0; i < w.n1; i++) {
@@ -1558,6 +1604,13 @@ static double ct_simpson(const Rad *R, double a, double b, double fa, double fm,
 double orc_wind_cool_time(const orc_wind_cfg *c, double rho, double p0, double xi, double t_r,
                           double p1, double tol)
 {
+  static const double ones[5] = {1.0, 1.0, 1.0, 1.0, 1.0};
+  return orc_wind_cool_time_pre(c, rho, p0, xi, t_r, p1, tol, ones);
+}
+
+double orc_wind_cool_time_pre(const orc_wind_cfg *c, double rho, double p0, double xi,
+                              double t_r, double p1, double tol, const double *pre)
+{
   W w;
   Rad R;
   double rv[1];
@@ -1571,7 +1624,7 @@ double orc_wind_cool_time(const orc_wind_cfg *c, double rho, double p0, double x
   rv[0] = rho;
   w.V = rv;
   R.w = &w; R.x1 = 1.0; R.xi_s = xi; R.tr_s = t_r;
-  R.pre[0] = R.pre[1] = R.pre[2] = R.pre[3] = R.pre[4] = 1.0;
+  R.pre[0] = pre[0]; R.pre[1] = pre[1]; R.pre[2] = pre[2]; R.pre[3] = pre[3]; R.pre[4] = pre[4];
   a = log(p0/(c->gamma - 1.0));
   b = log(p1/(c->gamma - 1.0));
   if (a == b) return 0.0;

@@ -55,6 +55,9 @@ struct Tables {
   const double *f1c, *f2c;     // [nr], [nt]: centre bilinear weights
   const double *sta, *cta;     // [na]: sin / cos of the bin angle
   double x1l0, sin_x2l0;       // x1 and |sin x2| of the lower edge of the first ghost zone
+  // the cv_idl midplane row (k.midplane): per interior column, the disk's
+  // density, Keplerian v_phi and pressure, [3][nr]
+  const double* mid;
   // tabulated force multiplier (k.fm_points > 0): log10 t [fm_points] and
   // log10 M [fm_points][nt][nr] of the fits, as read_sirocco_fluxes keeps them
   const double *fm_lt, *fm_lm;
@@ -88,6 +91,20 @@ __device__ __forceinline__ int map_bc_wind(int g, int n, int bcl, int bcr, bool&
   return map_bc(g, n, bcl == BC_WIND_INNER ? BC_OUTFLOW : bcl, bcr, flip);
 }
 
+// The midplane row of the cv_idl side-0 pass (cv_idl/init.c:276-311) on
+// zone (last polar row, column im), after its floors: the disk density
+// with the polar momentum kept, v_r = 0, the Keplerian v_phi, the disk
+// pressure, tracer 1 (the per-column values come from the host set-up).
+__device__ __forceinline__ void midplane_cell(const StageArgs& a, int im, double* v) {
+  const double rho_mid = a.T.mid[im];
+  v[VX2] = (v[RHO] * v[VX2]) / rho_mid;
+  v[RHO] = rho_mid;
+  v[VX1] = 0.0;
+  v[VX3] = a.T.mid[a.nr + im];
+  v[PRS] = a.T.mid[2 * a.nr + im];
+  v[TRC] = 1.0;
+}
+
 // The state of cell (jj, ii) as the reference's Boundary() leaves it at
 // the start of a stage (Src/boundary.c:56-480): ghost cells are never
 // stored; an out-of-domain index maps to the interior cell the boundary
@@ -109,6 +126,7 @@ __device__ __forceinline__ int ld6(const StageArgs& a, const double* __restrict_
   for (int nv = 0; nv < NVW; nv++) v[nv] = p[nv * a.sv];
   int changed = 0;
   if (a.k.dfloor > 0.0) changed = floor_cell(a.k, v);
+  if (a.k.midplane && jm == a.nt - 1) midplane_cell(a, im, v);
   if (fr) v[VX1] = -v[VX1];
   if (ii < 0 && a.bc[0] == BC_WIND_INNER) v[VX1] = dmin(v[VX1], 0.0);
   if (ft) v[VX2] = -v[VX2];
@@ -144,6 +162,11 @@ __device__ __forceinline__ void ld_vel(const StageArgs& a, int jj, int ii, doubl
       const double dfact = rho / a.k.dfloor;
       v1 = dfact * v1;
       v2 = dfact * v2;
+      rho = a.k.dfloor;
+    }
+    if (a.k.midplane && jm == a.nt - 1) {   // midplane_cell's velocities
+      v2 = (rho * v2) / a.T.mid[im];
+      v1 = 0.0;
     }
   }
   if (fr) v1 = -v1;
@@ -565,10 +588,19 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
 
     // ---- update, RK2 combination, cons->prim ---------------------------
     double u[NVW];
+    // a zone of the midplane row that the floors changed: its Uc is
+    // PrimToCons of the floored state, before the midplane reset
+    double vfl[NVW];
+    const bool mid_row = k.midplane && j == a.nt - 1;
+    if (STAGE == 2 && floored && mid_row) {
+#pragma unroll
+      for (int nv = 0; nv < NVW; nv++) vfl[nv] = a.Vin[nv * a.sv + cell];
+      floor_cell(k, vfl);
+    }
     if (STAGE == 1 || floored) {
       // stage 1: U0 = PrimToCons of the bounded state (rk_step.c:124-125);
       // stage 2: a cell the floors changed gets PrimToCons of its new state
-      p2c6(vc, u, k.gmm1);
+      p2c6((STAGE == 2 && mid_row) ? vfl : vc, u, k.gmm1);
     } else {
 #pragma unroll
       for (int nv = 0; nv < NVW; nv++) u[nv] = a.U[nv * a.sv + cell];
@@ -582,6 +614,7 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
 #pragma unroll
       for (int nv = 0; nv < NVW; nv++) vn[nv] = a.Vn[nv * a.sv + cell];
       if (k.dfloor > 0.0) floor_cell(k, vn);   // U0 was formed after the floors
+      if (mid_row) midplane_cell(a, i, vn);    // and the midplane row
       p2c6(vn, u0, k.gmm1);
 #pragma unroll
       for (int nv = 0; nv < NVW; nv++) u[nv] = 0.5 * u0[nv] + 0.5 * u[nv];
@@ -593,7 +626,8 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
       // entropy the stage-1 cons->prim left (Ue / rho of the stage-1 U)
       double ue;
       if (STAGE == 1) ue = vc[RHO] * sc;
-      else if (floored) ue = vc[RHO] * (a.Ue[cell] * (1.0 / a.U[RHO * a.sv + cell]));
+      else if (floored) ue = (mid_row ? vfl[RHO] : vc[RHO]) *
+                             (a.Ue[cell] * (1.0 / a.U[RHO * a.sv + cell]));
       else ue = a.Ue[cell];
       ue += rhs1e;
       ue += rhs2e;
@@ -602,6 +636,7 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
 #pragma unroll
         for (int nv = 0; nv < NVW; nv++) vn[nv] = a.Vn[nv * a.sv + cell];
         if (k.dfloor > 0.0) floor_cell(k, vn);
+        if (mid_row) midplane_cell(a, i, vn);
         const double u0e = vn[RHO] * entropy_of(vn, k.gamma);
         ue = 0.5 * u0e + 0.5 * ue;
       }
@@ -981,6 +1016,7 @@ namespace {
 
 constexpr double C_AH = 1.008, C_AHe = 4.004, C_AZ = 30.0;   // Src/pluto.h:296-301
 constexpr double C_G = 6.6726e-8, C_PI = 3.14159265358979;
+constexpr double C_sigma = 5.67051e-5;                          // Src/pluto.h:320
 
 int timed_begin(pmx_wind* w, hipEvent_t* e) {
   if (!w->timing) return 0;
@@ -1192,6 +1228,7 @@ Const make_const(const pmx_wind_desc* d) {
   k.fm_points = (d->krad == 999 && d->alpharad == 999) ? d->fm_points : 0;
   k.fast_div = 0;   // set with the line-force tables
   k.fm_fast = d->force_power == PMX_FORCE_POWER_FACTORED;
+  k.midplane = 0;   // set with the geometry
   return k;
 }
 
@@ -1307,6 +1344,42 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
   plm_coeffs(x1.data(), x1r.data(), dx1.data(), xgc1.data(), tr, pr.data());
   plm_coeffs(x2.data(), x2r.data(), dx2.data(), xgc2.data(), tt, pt.data());
 
+  // The cv_idl side-0 pass's midplane row (cv_idl/init.c:276-311): with
+  // SHOCK_FLATTENING MULTID the reference runs with 3 ghost zones, and the
+  // pass's row j == np_int + 2 is the last interior polar row; it is reset
+  // when that row's centre and volume centroid agree to 1e-6 (:280-281).
+  // Per column: the disk density rho_0 (r / r_in)^-alpha, the Keplerian
+  // v_phi and the pressure of the disk temperature at the zone's centroid,
+  // with the reference's expressions and the host libm.
+  std::vector<double> mid;
+  if (d->dfloor > 0.0 && d->shock_flattening) {
+    const int J = nt - 1 + NG;
+    if ((fabs(x2[J] - xgc2[J]) / x2[J]) < 1e-6) {
+      if (!(d->rho_0 > 0.0) || !(d->disk_mdot >= 0.0)) {
+        set_error("the cv_idl midplane row needs rho_0 > 0 and disk_mdot >= 0 (RHO_0, "
+                  "DISK_MDOT)");
+        delete w;
+        return -1;
+      }
+      mid.resize(3 * (size_t)nr);
+      const double rho_0 = d->rho_0 / k.UD, rho_alpha = d->rho_alpha, disk_mdot = d->disk_mdot;
+      const double r_WD = x1l[NG];                                   // g_domBeg[IDIR]
+      const double gm_cgs = C_G * d->cent_mass, gm_code = gm_cgs / (k.UL * k.UV * k.UV);
+      const double theta = xgc2[J];
+      for (int i = 0; i < nr; i++) {
+        const double r = xgc1[i + NG], rcyl = r * sin(theta);
+        const double rho_mid = rho_0 * pow((r / r_WD), -1.0 * rho_alpha);
+        double teff = pow(3.0 * gm_cgs * disk_mdot / (8.0 * C_PI * C_sigma), 0.25);
+        teff *= pow(r_WD * k.UL, -0.75);
+        const double temp = teff * pow(r_WD / rcyl, 0.75) * pow(1.0 - sqrt(r_WD / rcyl), 0.25);
+        mid[i] = rho_mid;
+        mid[nr + i] = sqrt(gm_code / r) * sin(theta);
+        mid[2 * nr + i] = rho_mid * temp / (k.KELVIN * d->mu);
+      }
+      w->k.midplane = 1;
+    }
+  }
+
   // VGradCalc set-up (line_connect.c:534-625): offset point per (bin,
   // cell), turned into bilinear weights and its trigonometric factors.
   const double UL = k.UL;
@@ -1380,7 +1453,7 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
       {&w->T.x2, &x2}, {&w->T.dx2, &dx2}, {&w->T.s, &s}, {&w->T.sp, &sp}, {&w->T.dmu, &dmu},
       {&w->T.inv_dx2, &inv_dx2}, {&w->T.ct, &ct}, {&w->T.sin2, &sin2}, {&w->T.cos2, &cos2},
       {&w->T.pt, &pt}, {&w->T.f1c, &f1c}, {&w->T.f2c, &f2c}, {&w->T.sta, &sta},
-      {&w->T.cta, &cta}};
+      {&w->T.cta, &cta}, {&w->T.mid, &mid}};
   size_t ntab = 0;
   for (auto& p : parts) ntab += p.second->size();
   std::vector<double> packed;

@@ -58,6 +58,7 @@ struct Const {
   int fm_fast;                             // factored force-multiplier power (see force_mult_pair)
   double first_dt;                         // runtime->first_dt (dt-too-small stop)
   int fm_points;                           // > 0: tabulated force multiplier (krad = alpharad = 999)
+  int midplane;                            // the cv_idl side-0 pass resets the last polar row
 };
 
 // ---------------------------------------------------------------------

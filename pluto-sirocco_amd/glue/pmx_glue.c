@@ -238,6 +238,9 @@ static pmx_wind *wind_context(Data *d, Grid *grid)
   desc.x2r = grid->xr[JDIR] + g2; desc.dx2 = grid->dx[JDIR] + g2;
 #if INTERNAL_BOUNDARY == YES
   desc.dfloor = g_inputParam[DFLOOR];
+  desc.rho_0 = g_inputParam[RHO_0];
+  desc.rho_alpha = g_inputParam[RHO_ALPHA];
+  desc.disk_mdot = g_inputParam[DISK_MDOT];
 #endif
 #if SHOCK_FLATTENING == MULTID
   desc.shock_flattening = 1;

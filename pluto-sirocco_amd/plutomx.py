@@ -95,7 +95,9 @@ class WindDesc(ctypes.Structure):
                 ("char_limiting", ctypes.c_int), ("entropy_switch", ctypes.c_int),
                 ("solver", ctypes.c_int), ("first_dt", ctypes.c_double),
                 ("fm_points", ctypes.c_int), ("fm_t", _dp), ("fm_m", _dp),
-                ("force_power", ctypes.c_int)]
+                ("force_power", ctypes.c_int),
+                ("rho_0", ctypes.c_double), ("rho_alpha", ctypes.c_double),
+                ("disk_mdot", ctypes.c_double)]
 
 PMX_FORCE_POWER_REF = 0        # the reference's t and pow(t, alpha) per LineForce call
 PMX_FORCE_POWER_FACTORED = 1   # one exp2(-alpha log2 dvds) per bin for both calls
@@ -389,6 +391,7 @@ def wind_desc_scalars(cfg: setups.WindConfig) -> WindDesc:
     d.solver = int(cfg.solver)
     d.first_dt = cfg.first_dt
     d.force_power = int(getattr(cfg, "force_power", PMX_FORCE_POWER_REF))
+    d.rho_0, d.rho_alpha, d.disk_mdot = cfg.rho_0, cfg.rho_alpha, cfg.disk_mdot
     return d
 
 

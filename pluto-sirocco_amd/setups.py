@@ -254,6 +254,11 @@ class WindConfig:
     # line force's power per angle bin: 0 the reference's form, 1 factored
     # (one exp2/log2 per bin for both LineForce calls; a few 1e-15 off)
     force_power: int = 0
+    # the cv_idl disk (its midplane row): RHO_0 [g/cm^3], R_0, RHO_ALPHA, DISK_MDOT [g/s]
+    rho_0: float = 1.e-9
+    r_0: float = 1.0
+    rho_alpha: float = 2.0
+    disk_mdot: float = 6.3e17
 
     dims = 2
     nghost = 2
@@ -314,12 +319,12 @@ class WindConfig:
                   "log        1000000", "analysis  -1.0  1",
                   "", "[Parameters]", ""]
         if self.dfloor > 0.0:
-            # the cv_idl parameter list, in its definitions.h order; RHO_0,
-            # R_0, RHO_ALPHA and DISK_MDOT only enter the midplane ghost row
-            # that the X2-end reflective fill overwrites
-            lines += [f"MU             {self.mu!r}", "RHO_0          1e-09", "R_0            1.0",
-                      "RHO_ALPHA      2.0", f"CENT_MASS      {self.cent_mass!r}",
-                      "DISK_MDOT      6.3e17", f"T_ISO          {self.t_iso!r}",
+            # the cv_idl parameter list, in its definitions.h order (RHO_0,
+            # RHO_ALPHA and DISK_MDOT enter its midplane row)
+            lines += [f"MU             {self.mu!r}", f"RHO_0          {self.rho_0!r}",
+                      f"R_0            {self.r_0!r}", f"RHO_ALPHA      {self.rho_alpha!r}",
+                      f"CENT_MASS      {self.cent_mass!r}",
+                      f"DISK_MDOT      {self.disk_mdot!r}", f"T_ISO          {self.t_iso!r}",
                       f"L_star         {self.l_star!r}", f"f_x            {self.f_x!r}",
                       f"f_uv           {self.f_uv!r}", f"T_x            {self.t_x!r}",
                       f"KRAD           {self.krad!r}", f"ALPHARAD       {self.alpharad!r}",

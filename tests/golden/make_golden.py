@@ -100,6 +100,22 @@ def cvidl_cases():
            setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7)), 8)
 
 
+def midplane_cases():
+    """The production set-up (SHOCK_FLATTENING MULTID, so 3 ghost zones) at
+    64 polar zones, where the cv_idl side-0 pass's midplane block
+    (cv_idl/init.c:276-311) resets the last interior theta row every stage
+    (below ~48 polar zones the row's centre and centroid differ by more than
+    its 1e-6 test and it does not): with MULTID alone and with the full
+    production reconstruction and HLL."""
+    cfg = dataclasses.replace(setups.cvidl_wind(32, 64, dfloor=1.e-15), first_dt=1.e-3,
+                              shock_flattening=True)
+    yield ("wind_cvidl_flat_midplane_32x64", cfg,
+           setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7)), 8)
+    cfg = dataclasses.replace(cfg, char_limiting=True, entropy_switch=True, solver=setups.HLL)
+    yield ("wind_cvidl_prod_midplane_32x64", cfg,
+           setups.wind_ic_mixed(cfg, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7)), 8)
+
+
 def roe_cases():
     """The wind with [Solver] roe (Src/HD/roe.c: its 2-D strong-shock switch
     to HLL), plain and on the production set-up (Roe's own HLL branch next
@@ -348,6 +364,7 @@ def main(which=("hydro", "wind", "cvidl", "roe", "restart", "output", "tstop")):
     wind = list(wind_cases()) if "wind" in which else []
     wind += list(cvidl_cases()) if "cvidl" in which else []
     wind += list(roe_cases()) if "roe" in which else []
+    wind += list(midplane_cases()) if "midplane" in which else []
     for name, cfg, V0, n in wind:
         inp = wind_inputs(cfg)
         fl = (inp["flux_r"], inp["flux_t"], inp["flux_p"])

@@ -111,3 +111,51 @@ def test_blondin_step_is_the_trapezoid_root(dt):
     # Brent stops within 1 K of the root
     dT = np.abs(r["V"][setups.PRS, 0] / p_an - 1.0) * Tf
     assert dT.max() <= 1.0, dT.max()
+
+
+def test_exact_cooling_time_closed_form():
+    """The oracle's exact cooling time (the integral of dE / f(E) that judges
+    the surrogate) against the bremsstrahlung law: with only brem_c left,
+    dE/dt = -B sqrt(E), so the time from E0 to E1 is 2 (sqrt(E0) - sqrt(E1))
+    / B; a transposed prefactor (line_c for brem_c) misses it."""
+    cfg, V, pre, kelvin = brem_case(setups.COOL_PLAIN)
+    g = cfg.gamma
+    E_cost = cfg.unit_length / cfg.unit_density / cfg.unit_velocity ** 3
+    pre5 = pre[:5, 0, 0]
+    for q in range(3):
+        rho = float(V[setups.RHO, 0].ravel()[q])
+        p0 = float(V[setups.PRS, 0].ravel()[q])
+        nH = rho * cfg.unit_density / (1.43 * MP)
+        a = (g - 1.0) * kelvin * cfg.mu / rho                 # T = a E
+        B = 1.21 * nH * nH * 3.3e-27 * E_cost * np.sqrt(a)
+        E0 = p0 / (g - 1.0)
+        for frac in (0.9, 0.5, 0.1):
+            p1 = frac * p0                                   # T stays above 1e5 K
+            assert p1 / (g - 1.0) * a > 1.e5
+            E1 = p1 / (g - 1.0)
+            t_an = 2.0 * (np.sqrt(E0) - np.sqrt(E1)) / B
+            t = refrun.wind_cool_time(cfg, rho, p0, 1.0, p1, 1e-12 * t_an, pre=pre5)
+            assert abs(t / t_an - 1.0) < 1e-9, (q, frac, t, t_an)
+        swapped = pre5.copy()
+        swapped[[3, 4]] = swapped[[4, 3]]
+        t_sw = refrun.wind_cool_time(cfg, rho, p0, 1.0, 0.5 * p0, 1e-9, pre=swapped)
+        assert not abs(t_sw / (2.0 * (np.sqrt(E0) - np.sqrt(0.5 * E0)) / B) - 1.0) < 1e-3
+
+
+def test_exact_cooling_time_is_additive():
+    """With every rate on (unit prefactors), the time from p0 to p2 is the
+    time to an intermediate p1 plus the time from there (the integral's own
+    consistency, which a wrong sign, bound or quadrature rule breaks), and
+    it is 0 from a state to itself."""
+    cfg = setups.WindConfig(nx=(8, 4), cooling=setups.COOL_PLAIN)
+    kelvin = cfg.unit_velocity ** 2 * setups.CONST["amu"] / KB
+    for rho_cgs, T0, xi in ((1e-12, 3e6, 10.0), (3e-11, 1e7, 300.0), (1e-13, 8e5, 1e4)):
+        rho = rho_cgs / cfg.unit_density
+        p0 = rho * T0 / (kelvin * cfg.mu)
+        t_all = refrun.wind_cool_time(cfg, rho, p0, xi, 0.3 * p0, 1e-14)
+        if not np.isfinite(t_all):
+            continue
+        t_a = refrun.wind_cool_time(cfg, rho, p0, xi, 0.6 * p0, 1e-14)
+        t_b = refrun.wind_cool_time(cfg, rho, 0.6 * p0, xi, 0.3 * p0, 1e-14)
+        assert t_all > 0.0 and abs((t_a + t_b) / t_all - 1.0) < 1e-9, (rho_cgs, t_a, t_b, t_all)
+        assert refrun.wind_cool_time(cfg, rho, p0, xi, p0, 1e-14) == 0.0

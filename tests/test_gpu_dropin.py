@@ -81,6 +81,9 @@ def wind_cases():
                                shock_flattening=True, char_limiting=True, entropy_switch=True,
                                solver=setups.HLL)
     yield "cvidl_production", prod, _mixed(prod), 8, {}
+    # at 64 polar zones the cv_idl boundary pass also resets the last polar row
+    prod64 = dataclasses.replace(prod, nx=(32, 64))
+    yield "cvidl_production_midplane", prod64, _mixed(prod64), 8, {}
     tab = dataclasses.replace(prod, krad=999.0, alpharad=999.0)
     V0 = _mixed(tab)
     sir = setups.sirocco_inputs(tab, V0, heatcool=True, prefactors=True, fm_points=12,

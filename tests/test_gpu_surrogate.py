@@ -103,11 +103,18 @@ def test_disagreements_are_errors_of_the_plain_integrator(runs, cells):
 
 
 def test_network_gain_over_unguided_refinement(runs):
-    """The same refinement started without the network (explicit step
-    limited to one e-fold) fails more often and has more cells beyond 1e-3."""
+    """What the network buys: the same refinement started without it (the
+    explicit step limited to one e-fold) sends strictly more cells to the
+    plain integrator or takes strictly longer (by 10%), and never does
+    better on fallbacks or on the cells beyond 1e-3.  (Measured: 0 vs 7
+    fallbacks of 10,622 stiff cells, 102 vs 133 ms; the Radiat() calls are
+    within 0.3% of each other, the quadratures of the certified solve
+    dominating them either way.)"""
     pp = runs[0][0]
-    (ps, sts, _), (pa, sta, _) = runs[1], runs[2]
+    (ps, sts, mss), (pa, sta, msa) = runs[1], runs[2]
+    print(f"network: {sts}, {mss:.3f} ms; without: {sta}, {msa:.3f} ms", flush=True)
     assert sts["fallback"] <= sta["fallback"], (sts, sta)
+    assert sts["fallback"] < sta["fallback"] or mss < 0.9 * msa, (sts, sta, mss, msa)
     assert np.sum(rel_err(ps, pp) > 1e-3) <= np.sum(rel_err(pa, pp) > 1e-3)
 
 

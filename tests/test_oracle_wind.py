@@ -63,6 +63,8 @@ COVERAGE = {
     "wind_blondin_restart_32x16": ["bl_solve", "below_minT"],
     "wind_plain_tabulated_32x16": ["rkf12_ok", "ck45_calls"],
     "wind_cvidl_prod_restart_32x16": ["floor_cells", "shock_zones", "ck45_calls"],
+    "wind_cvidl_flat_midplane_32x64": ["floor_cells", "shock_zones", "midplane_cells"],
+    "wind_cvidl_prod_midplane_32x64": ["floor_cells", "shock_zones", "midplane_cells"],
 }
 
 
@@ -289,3 +291,29 @@ def test_roe_fixtures_pin_the_solver():
                                        int(g["nsteps"]), fluxes=fl, grid=grid)
             assert not np.array_equal(r["V"], g["V"]), (name, other)
 
+
+
+@pytest.mark.parametrize("name", ["wind_cvidl_flat_midplane_32x64",
+                                  "wind_cvidl_prod_midplane_32x64"])
+def test_midplane_fixtures_pin_the_midplane_row(name):
+    """At 64 polar zones the cv_idl side-0 pass resets the last interior
+    row (MULTID runs with 3 ghost zones): each of the disk's parameters
+    changes the fixture's result, so the reset is pinned; at 16 polar zones
+    the row's centre and centroid differ by 2.6e-5 relative, above the
+    pass's 1e-6 test, and the same parameters change nothing."""
+    import dataclasses
+    cfg, g = golden_io.load(name)
+    grid = ({"x": g["x1"], "xl": g["x1l"], "xr": g["x1r"], "dx": g["dx1"]},
+            {"x": g["x2"], "xl": g["x2l"], "xr": g["x2r"], "dx": g["dx2"]})
+    fl = (g["flux_r"], g["flux_t"], g["flux_p"])
+    for variant in (dataclasses.replace(cfg, rho_0=2.e-9), dataclasses.replace(cfg, rho_alpha=1.5),
+                    dataclasses.replace(cfg, disk_mdot=1.e18)):
+        r = refrun.run_wind_oracle(variant, g["V0"], int(g["nsteps"]), fluxes=fl, grid=grid)
+        assert not np.array_equal(r["V"], g["V"])
+    cfg16, g16 = golden_io.load("wind_cvidl_flat_mixed_32x16")
+    grid16 = ({"x": g16["x1"], "xl": g16["x1l"], "xr": g16["x1r"], "dx": g16["dx1"]},
+              {"x": g16["x2"], "xl": g16["x2l"], "xr": g16["x2r"], "dx": g16["dx2"]})
+    r = refrun.run_wind_oracle(dataclasses.replace(cfg16, rho_0=2.e-9), g16["V0"],
+                               int(g16["nsteps"]), grid=grid16,
+                               fluxes=(g16["flux_r"], g16["flux_t"], g16["flux_p"]))
+    assert np.array_equal(r["V"], g16["V"]) and refrun.wind_counters()["midplane_cells"] == 0
