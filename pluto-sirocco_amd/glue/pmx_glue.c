@@ -46,6 +46,16 @@ static void pmx_glue_fail(const char *what)
 
 #define PMX_CALL(x) do { if ((x) != 0) pmx_glue_fail(#x); } while (0)
 
+/* FailSafe (Src/failsafe.c:5-158) retries a step whose cons->prim failed
+ * from the saved state, with HLL / flat / minmod flags around the failed
+ * zones; it needs AdvanceStep to return at the first failing stage
+ * (rk_step.c:186-188, :262-264, :323-325) and honour per-zone flags the
+ * device step does not read.  Refused at compile time rather than run
+ * without the retry. */
+#if FAILSAFE == YES
+ #error "libplutomx: FAILSAFE YES is not supported (the device step has no FailSafe retry)"
+#endif
+
 /* ------------------------------------------------------------------ */
 #if GEOMETRY == CARTESIAN && !(defined LINE_DRIVEN_WIND && LINE_DRIVEN_WIND != NO)
 /* ------------------------------------------------------------------ */

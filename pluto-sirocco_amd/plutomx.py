@@ -12,6 +12,7 @@ from __future__ import annotations
 import ctypes
 import os
 import pathlib
+import weakref
 
 import numpy as np
 
@@ -627,6 +628,30 @@ def read_sirocco_files(cfg: setups.WindConfig, directory, V: np.ndarray | None =
     return out
 
 
+class _PinnedView:
+    """Owner (numpy base) of a zero-copy, read-only view of a context's
+    pinned snapshot buffer; the context tracks these through weak
+    references."""
+
+    def __init__(self, ptr, shape):
+        self.__array_interface__ = {"data": (ctypes.cast(ptr, ctypes.c_void_p).value, True),
+                                    "shape": shape, "typestr": "<f8", "version": 3}
+
+
+class _DeferredDestroy:
+    """Destroys a wind context once every listed snapshot view is gone."""
+
+    def __init__(self, lib, handle, views):
+        self._lib, self._h, self._left = lib, handle, len(views)
+        for v in views:
+            weakref.finalize(v, self._dropped)
+
+    def _dropped(self):
+        self._left -= 1
+        if self._left == 0:
+            self._lib.pmx_wind_destroy(self._h)
+
+
 class Wind:
     """The line-driven disk wind (2-D spherical) advanced on the GPU.
 
@@ -656,9 +681,17 @@ class Wind:
         self.set_tstop(getattr(cfg, "tstop", 1.e30))
 
     def close(self) -> None:
+        """Destroy the context.  While a zero-copy snapshot view
+        (snapshot_wait(copy=False)) is still referenced, the context (and
+        the pinned buffer the view reads) is destroyed when the last such
+        view is dropped."""
         if getattr(self, "_h", None):
             if self._owned:
-                self._lib.pmx_wind_destroy(self._h)
+                views = list(getattr(self, "_views", ()))
+                if views:
+                    _DeferredDestroy(self._lib, self._h, views)
+                else:
+                    self._lib.pmx_wind_destroy(self._h)
             self._h = None
 
     def __del__(self):
@@ -720,13 +753,18 @@ class Wind:
     def snapshot(self, user_vars: bool = True) -> None:
         """Queue an output snapshot (V and the production user variables);
         later steps may be queued before snapshot_wait()."""
+        if any(True for _ in getattr(self, "_views", ())):
+            raise PmxError("snapshot(): a zero-copy view of the previous snapshot "
+                           "(snapshot_wait(copy=False)) is still referenced; the new "
+                           "snapshot would overwrite it while it is read")
         self._snap_uv = bool(user_vars)
         _check(self._lib.pmx_wind_snapshot(self._h, int(user_vars)), "snapshot")
 
     def snapshot_wait(self, copy: bool = True) -> dict:
         """The last snapshot: V [6][1][nt][nr], uv [23][1][nt][nr] (or None),
-        t, dt, step.  copy=False: read-only views of the pinned host copy,
-        valid until the next snapshot of this context."""
+        t, dt, step.  copy=False: read-only views of the pinned host copy;
+        while any of them is referenced, snapshot() refuses (it would
+        overwrite the buffer) and close() leaves the buffer to the views."""
         nt, nr = self.cfg.nx[1], self.cfg.nx[0]
         nv = 6 + (23 if self._snap_uv else 0)
         out = np.empty((nv, 1, nt, nr)) if copy else None
@@ -737,8 +775,11 @@ class Wind:
         if not copy:
             ptr = ctypes.POINTER(ctypes.c_double)()
             _check(self._lib.pmx_wind_snapshot_host(self._h, ctypes.byref(ptr)), "snapshot_host")
-            out = np.ctypeslib.as_array(ptr, shape=(nv, 1, nt, nr))
-            out.flags.writeable = False
+            holder = _PinnedView(ptr, (nv, 1, nt, nr))
+            if getattr(self, "_views", None) is None:
+                self._views = weakref.WeakSet()
+            self._views.add(holder)
+            out = np.asarray(holder)
         return {"V": out[:6], "uv": out[6:] if self._snap_uv else None, "t": t.value,
                 "dt": dt.value, "step": st.value}
 

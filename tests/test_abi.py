@@ -108,3 +108,36 @@ def test_noalloc_step_refuses_grids_over_its_limit():
                                             ctypes.c_void_p(16), 1.e-3, ctypes.byref(inv))
     assert rc == -1
     assert b"exceeds the limit of 524288 cells" in lib.pmx_last_error()
+
+
+def test_zero_copy_snapshot_view_lifetime():
+    """A zero-copy snapshot view is read-only, is tracked weakly by its
+    context, and a context closed while a view is referenced is destroyed
+    when the last view goes (host logic of Wind.snapshot_wait(copy=False)
+    and Wind.close, with a stand-in for the library)."""
+    import gc
+    import weakref
+    import numpy as np
+    buf = (ctypes.c_double * 12)(*range(12))
+    holder = plutomx._PinnedView(ctypes.cast(buf, ctypes.POINTER(ctypes.c_double)), (3, 4))
+    views = weakref.WeakSet([holder])
+    a = np.asarray(holder)
+    del holder
+    assert not a.flags.writeable and a[2, 3] == 11.0
+    part = a[1:]
+    del a
+    assert len(views) == 1          # a slice keeps the view's owner alive
+
+    class Lib:
+        destroyed = []
+
+        def pmx_wind_destroy(self, h):
+            self.destroyed.append(h)
+
+    lib = Lib()
+    plutomx._DeferredDestroy(lib, "ctx", list(views))
+    gc.collect()
+    assert lib.destroyed == []
+    del part
+    gc.collect()
+    assert lib.destroyed == ["ctx"] and len(views) == 0

@@ -82,6 +82,21 @@ def test_snapshot_is_taken_when_queued(pmx):
         s2 = w.snapshot_wait(copy=False)
         assert np.array_equal(s2["V"], V7) and s2["uv"] is None
         assert not s2["V"].flags.writeable
+        # the zero-copy view holds the buffer: no new snapshot over it ...
+        with pytest.raises(pmx.PmxError, match="still referenced"):
+            w.snapshot(user_vars=False)
+        V = s2["V"]
+        del s2
+        with pytest.raises(pmx.PmxError, match="still referenced"):
+            w.snapshot(user_vars=False)
+        del V
+        w.snapshot(user_vars=False)
+        w.run(1)
+        s3 = w.snapshot_wait(copy=False)
+        V8 = w.get_prim()
+    # ... and closing the context leaves the buffer to the view
+    assert np.array_equal(s3["V"], V8)
+    del s3
 
 
 def test_user_vars_need_a_step(pmx):
