@@ -105,6 +105,13 @@ def cpu_baseline_all_cores(n_sample: int = 128, steps: int = 3) -> dict | None:
                       f"processes (slowest {wall:.2f} s)"}
 
 
+# 4 -> 64 -> 64 -> 64 -> 1: forward 2 (4*64 + 64*64 + 64*64 + 64) = 17,024
+# FLOP per sample; backward adds the weight and input gradients (the first
+# layer's weights only): 2 (4*64) + 4 (64*64) * 2 + 4 * 64 = 33,536
+MLP_FWD_FLOP_PER_SAMPLE = 2 * (4 * 64 + 64 * 64 + 64 * 64 + 64)
+MLP_TRAIN_FLOP_PER_SAMPLE = MLP_FWD_FLOP_PER_SAMPLE + 2 * 4 * 64 + 4 * 64 * 64 * 2 + 4 * 64
+
+
 def kh_leg(n: int = 256, steps: int = 100, warmup: int = 10) -> dict:
     """The 2-D Kelvin-Helmholtz workload (256^2, PLM van Leer + HLLC + RK2,
     CFL 0.4, periodic) for its stated 100 steps, graph-replayed: a 65,536-
@@ -165,22 +172,27 @@ def wind_legs(steps: int = 200, cpu: bool = True) -> dict:
     V0 = setups.wind_ic(cfg)
 
     def run_wind(mlp=None):
+        # the main loop replayed as a captured HIP graph (what a driver
+        # runs), then 20 eager steps with per-launch events for the split
+        # into stage and cooling time
         with plutomx.Wind(cfg) as w:
             w.set_prim(V0)
             w.set_time(0.0, cfg.first_dt, 0)
             if mlp is not None:
                 w.set_surrogate(mlp)
-            w.run(2)
+            w.run(2, graph=True)
             w.sync()
-            w.timing(True, reset=True)
             w.surrogate_stats(reset=True)
             t0 = time.perf_counter()
-            w.run(steps)
+            w.run(steps, graph=True)
             w.sync()
             el = time.perf_counter() - t0
-            tm = w.timing(False)
             st = w.surrogate_stats()
             T = w.get_prim()
+            w.timing(True, reset=True)
+            w.run(20)
+            w.sync()
+            tm = w.timing(False)
         return {"Mcell_updates_per_s": round(cfg.ncells * steps / el / 1e6, 2),
                 "ms_per_step": round(1e3 * el / steps, 4),
                 "stage_ms_per_launch": round(tm["stage_ms"] / max(tm["stage_launches"], 1), 4),
@@ -189,7 +201,7 @@ def wind_legs(steps: int = 200, cpu: bool = True) -> dict:
 
     plain, Tp = run_wind()
     out["c3_plain"] = dict(plain, workload=f"wind 512x256, PLM+HLLC+RK2, line force, plain "
-                                            f"cooling, {steps} steps")
+                                            f"cooling, {steps} steps, graph replay")
     if cpu:
         out["c3_plain"]["cpu_baseline"] = wind_cpu_baseline()
     # the same wind with the production problem's set-up (cv_idl boundaries
@@ -261,7 +273,14 @@ def wind_legs(steps: int = 200, cpu: bool = True) -> dict:
                                training={"samples": 2_000_000, "sample_s": round(t1 - t0, 2),
                                          "steps": 20000, "batch": 8192,
                                          "train_s": round(t2 - t1, 2),
-                                         "final_loss": float(np.mean(hist[-50:]))},
+                                         "final_loss": float(np.mean(hist[-50:])),
+                                         # forward + backward FLOPs of the 4-64-64-64-1
+                                         # network per sample (dense layers only, the first
+                                         # layer unpadded), over the wall time of the steps
+                                         # (training kernel, Adam update and launches)
+                                         "TFLOP_per_s_wall": round(
+                                             MLP_TRAIN_FLOP_PER_SAMPLE * 8192 * 20000
+                                             / (t2 - t1) / 1e12, 3)},
                                speedup_vs_plain=round(plain["ms_per_step"] / surr["ms_per_step"], 3))
     # cooling alone over the training hull (dt 1e-2..1e2 t_cool)
     m.sample(cfg, 131072, seed=99)

@@ -39,6 +39,15 @@ __device__ __forceinline__ int map_bc(int g, int n, int bcl, int bcr, bool& flip
   return g < 0 ? 0 : (g >= n ? n - 1 : g);
 }
 
+// Spread reductions.  Device-scope atomics that every workgroup of a
+// launch aims at one word serialise (on MI355X about 10 ns each: some
+// 20 us at the end of a 1024-workgroup launch for two such reductions);
+// the workgroups of a launch therefore fold their results into RED_SLOTS
+// words RED_STRIDE words (128 B, a cache line) apart, chosen by workgroup,
+// and the single consumer of the reduction folds the slots.
+constexpr int RED_SLOTS = 32, RED_STRIDE = 16;
+__host__ __device__ inline int red_index(unsigned wg) { return (int)(wg % RED_SLOTS) * RED_STRIDE; }
+
 // Block id remap so that the tiles one XCD executes are contiguous in the
 // logical tile order (block b runs on XCD b % 8 on MI355X): neighbouring
 // tiles then share that XCD's L2 for their halo reads.

@@ -31,7 +31,6 @@ constexpr int NTHREADS = 256;
 struct DevScalars {
   double t;                      // g_time
   double dt;                     // g_dt of the step being taken
-  unsigned long long cdt_bits;   // max C_dt of stage 1
   double inv_dt_hyp;             // Dts->invDt_hyp of the last step
   double tstop;                  // runtime.tstop
   double dt_uncut;               // the dt a cut to tstop replaced
@@ -39,6 +38,8 @@ struct DevScalars {
   int halt;                      // PMX_FLAG_DT_TOO_SMALL / PMX_FLAG_TSTOP once set (sticky):
                                  // later main-loop steps are no-ops
   int last;                      // the next step is the last one (dt clamped to tstop - t)
+  unsigned long long cdt_red[RED_SLOTS * RED_STRIDE];   // stage-1 C_dt maxima, spread
+                                                        // (words red_index(k), 0 = none)
 };
 
 // Src/main.c:226-229: the step that would reach tstop (to 1e-8 relative)
@@ -59,9 +60,9 @@ __global__ void prep_kernel(DevScalars* s, const double* dx, double* dtdx, int n
   double dt = dt_override ? *dt_override : s->dt;
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   for (int q = t; q < ntot; q += gridDim.x * blockDim.x) dtdx[q] = dt / dx[q];
+  if (t < RED_SLOTS) s->cdt_red[red_index(t)] = 0ull;
   if (t == 0) {
     if (dt_override) s->dt = dt;
-    s->cdt_bits = 0ull;
     s->err = 0;
   }
 }
@@ -76,7 +77,13 @@ __global__ void finish_kernel(DevScalars* s, int ndim, double cfl, double cfl_ma
                               double first_dt, int loop, const double* inv_in) {
   if (loop && s->halt) return;
   double inv = inv_in ? *inv_in : 0.0;   // Dts->invDt_hyp the stage-1 maximum folds into
-  double c = __longlong_as_double((long long)s->cdt_bits);
+  unsigned long long m = 0ull;   // non-negative doubles order like their bit patterns
+#pragma unroll
+  for (int k = 0; k < RED_SLOTS; k++) {
+    const unsigned long long v = s->cdt_red[red_index(k)];
+    m = v > m ? v : m;
+  }
+  double c = __longlong_as_double((long long)m);
   inv = dmax(inv, c);
   inv /= (double)ndim;
   s->inv_dt_hyp = inv;
@@ -212,7 +219,7 @@ int stage(pmx_hydro* h, const double* Vin, const double* V0, double* Vout, int m
   a.tiles_y = (h->ny + TILE_Y - 1) / TILE_Y;
   a.tiles_z = (h->d.dims == 3) ? (h->nz + h->cz - 1) / h->cz : 1;
   a.write_u = write_u;
-  a.cdt_bits = &h->s->cdt_bits;
+  a.cdt_bits = h->s->cdt_red;
   a.err = &h->s->err;
   a.halt = loop ? &h->s->halt : nullptr;
   int nblocks = a.tiles_x * a.tiles_y * a.tiles_z;
@@ -595,11 +602,12 @@ namespace {
 
 constexpr long long NOALLOC_MAX_CELLS = 1ll << 19;   // 2 stage states of 5 doubles
 constexpr int NOALLOC_MAX_LINE = 8192;               // nx + ny + nz
+constexpr size_t NOALLOC_SCALARS = 16 + RED_SLOTS * RED_STRIDE;   // doubles for DevScalars
 // [V1 | V2] stage states, then dx, dt/dx, 1/dx tables, the dt slots and
-// the time scalars: 40 MiB + 192 KiB
+// the time scalars (with their spread reduction words): 40 MiB + 196 KiB
 constexpr size_t NOALLOC_DOUBLES = 2 * (size_t)NV * NOALLOC_MAX_CELLS +
-                                   3 * (size_t)NOALLOC_MAX_LINE + 2 + 16;
-static_assert(sizeof(DevScalars) <= 16 * sizeof(double), "scalar slots");
+                                   3 * (size_t)NOALLOC_MAX_LINE + 2 + NOALLOC_SCALARS;
+static_assert(sizeof(DevScalars) <= NOALLOC_SCALARS * sizeof(double), "scalar slots");
 
 __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
 

@@ -37,6 +37,9 @@ enum { MODE_FIRST = 0, MODE_WEIGHTED = 1, MODE_THIRD = 2 };
 #ifndef PMX_MERGE_XY
 #define PMX_MERGE_XY 1     // 3-D: x and y sweeps share phases, 4 barriers per plane instead of 6
 #endif
+#ifndef PMX_ZUNROLL
+#define PMX_ZUNROLL 1   // unroll of the z-plane loop
+#endif
 #ifndef PMX_PREFETCH_U
 #define PMX_PREFETCH_U 0   // load U and V0 of the cell at the top of the plane loop
 #endif
@@ -56,7 +59,8 @@ struct StageArgs {
   int cz;                   // z planes per workgroup
   int tiles_x, tiles_y, tiles_z;
   int write_u;              // store U (always needed except after the last stage)
-  unsigned long long* cdt_bits;  // MODE_FIRST: max over cells of C_dt (bit pattern)
+  unsigned long long* cdt_bits;  // MODE_FIRST: max over cells of C_dt (bit pattern), spread
+                                 // over the words red_index(k) by workgroup
   int* err;                      // OR of cons->prim floor flags
   const int* halt;               // main-loop steps: skip the stage once nonzero; null for a
                                  // step the caller drives (explicit dt)
@@ -240,6 +244,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
     }
   }
 
+#pragma unroll PMX_ZUNROLL
   for (int k = k0; k < k1; k++) {
     double vc[NV];  // own cell value at plane k
     if (owner) {
@@ -481,7 +486,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
       double m = sRed[0];
       for (int q = 1; q < NT / 64; q++) m = dmax(m, sRed[q]);
       // non-negative doubles order like their bit patterns
-      atomicMax(a.cdt_bits, (unsigned long long)__double_as_longlong(m));
+      atomicMax(a.cdt_bits + red_index(blockIdx.x), (unsigned long long)__double_as_longlong(m));
     }
   }
   if (fail) atomicOr(a.err, fail);

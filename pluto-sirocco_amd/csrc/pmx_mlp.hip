@@ -374,34 +374,22 @@ using wind::SurrWork;
 using wind::CoolBatch;
 using wind::find_grid;
 
-// Appends the lanes with `pred` to a list: one atomic per wavefront.
-// Every lane of the wavefront must call it.
-__device__ __forceinline__ long long wave_append(unsigned long long* ctr, bool pred) {
+// Appends the lanes with `pred` to a list whose tail is *ctr, and adds
+// their number to the running total *total (if given): one atomic each per
+// wavefront.  Every lane of the wavefront must call it.
+__device__ __forceinline__ long long wave_append(unsigned long long* ctr, bool pred,
+                                                 unsigned long long* total = nullptr) {
   const unsigned long long mask = __ballot(pred);
   if (mask == 0ull) return -1;
   const int lane = threadIdx.x & 63;
   const int leader = __ffsll((long long)mask) - 1;
   unsigned long long base = 0ull;
-  if (lane == leader) base = atomicAdd(ctr, (unsigned long long)__popcll(mask));
+  if (lane == leader) {
+    base = atomicAdd(ctr, (unsigned long long)__popcll(mask));
+    if (total) atomicAdd(total, (unsigned long long)__popcll(mask));
+  }
   base = __shfl(base, leader);
   return pred ? (long long)(base + __popcll(mask & ((1ull << lane) - 1ull))) : -1;
-}
-
-__device__ __forceinline__ void reduce_dtcool(double cand, int bad, wind::Scalars* s) {
-  __shared__ double smin[2];
-  __shared__ int sbad[2];
-  for (int off = 32; off > 0; off >>= 1) {
-    cand = dmin(cand, __shfl_xor(cand, off));
-    bad |= __shfl_xor(bad, off);
-  }
-  if ((threadIdx.x & 63) == 0) { smin[threadIdx.x >> 6] = cand; sbad[threadIdx.x >> 6] = bad; }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double m = dmin(smin[0], smin[1]);
-    if (m >= 0.0 && m < 1.0e300)
-      atomicMin(&s->dtcool_bits, (unsigned long long)__double_as_longlong(m));
-    if (sbad[0] | sbad[1]) atomicOr(&s->err, PMX_FLAG_COOL_SUBSTEPS);
-  }
 }
 
 __device__ __forceinline__ RadCell cell_state(const CoolArgs& a, long long cell) {
@@ -430,29 +418,37 @@ __device__ __forceinline__ void reduce_dtcool_any(double cand, int bad, wind::Sc
     }
     if ((threadIdx.x & 63) == 0 && s) {
       if (cand >= 0.0 && cand < 1.0e300)
-        atomicMin(&s->dtcool_bits, (unsigned long long)__double_as_longlong(cand));
+        atomicMin(&s->dtcool_red[red_index(blockIdx.x)],
+                  (unsigned long long)__double_as_longlong(cand));
       if (bad) atomicOr(&s->err, PMX_FLAG_COOL_SUBSTEPS);
     }
   } else if (s) {
     if (cand >= 0.0 && cand < 1.0e300)
-      atomicMin(&s->dtcool_bits, (unsigned long long)__double_as_longlong(cand));
+      atomicMin(&s->dtcool_red[red_index(blockIdx.x)],
+                (unsigned long long)__double_as_longlong(cand));
     if (bad) atomicOr(&s->err, PMX_FLAG_COOL_SUBSTEPS);
   }
 }
 
-__global__ void surr_reset_kernel(unsigned long long* cnt) {
-  if (threadIdx.x < 4) cnt[threadIdx.x] = 0ull;
+// Ends a call of the surrogate path: clears the list tails cnt[0] and
+// cnt[1] for the next call (one workgroup, after the call's last kernel;
+// a last-workgroup ticket in that kernel costs more than this launch).
+__global__ void surr_end_call_kernel(unsigned long long* cnt) {
+  if (threadIdx.x < 2) cnt[threadIdx.x] = 0ull;
 }
 
-__global__ void surr_totals_kernel(unsigned long long* cnt) {
-  if (threadIdx.x < 4) cnt[4 + threadIdx.x] += cnt[threadIdx.x];
-}
-
-// Adds the lanes' Radiat call counts to a counter: one atomic per wavefront.
+// Adds the workgroup's Radiat call counts to its spread slot of the
+// counter slots at ctr: one device atomic per workgroup.  Every thread of
+// the workgroup must call it.
 __device__ __forceinline__ void count_calls(unsigned long long* ctr, int n) {
-  long long v = n;
+  __shared__ unsigned long long sc;
+  unsigned long long v = (unsigned long long)n;
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  if ((threadIdx.x & 63) == 0 && v) atomicAdd(ctr, (unsigned long long)v);
+  if (threadIdx.x == 0) sc = 0ull;
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0 && v) atomicAdd(&sc, v);
+  __syncthreads();
+  if (threadIdx.x == 0 && sc) atomicAdd(ctr + red_index(blockIdx.x), sc);
 }
 
 // Non-stiff cells (|dE/dt| <= 0.5 E/dt) take the reference integrator here
@@ -481,7 +477,9 @@ __global__ __launch_bounds__(128) void surr_classify_kernel(CoolBatch b, SurrWor
       double E0 = p / (k.gamma - 1.0);
       k1 = wind::radiat(k, R, E0);
       if (!(fabs(k1) > 0.5 / dt)) {
-        a.V[PRS * a.sv + cell] = wind::plain_cell(k, R, p, dt, cand, bad);
+        // the integrator continues from this first rate: a non-stiff cell
+        // costs what the plain kernel's does
+        a.V[PRS * a.sv + cell] = wind::plain_cell_from(k, R, p, dt, E0, k1, cand, bad);
         ncalls = R.nev;
       } else {
         ncalls = R.nev;
@@ -509,20 +507,20 @@ __global__ __launch_bounds__(128) void surr_classify_kernel(CoolBatch b, SurrWor
       }
     }
   }
-  const long long li = wave_append(&w.cnt[0], to_list);
-  const long long fi = wave_append(&w.cnt[1], to_fb);
+  const long long li = wave_append(&w.cnt[0], to_list, &w.cnt[4]);
+  const long long fi = wave_append(&w.cnt[1], to_fb, &w.cnt[5]);
   if (to_list) {
     w.list[li] = q;
     for (int c = 0; c < 4; c++) w.feat[4 * li + c] = f[c];
     w.f0[li] = k1;
   }
   if (to_fb) w.fb[fi] = q;
-  count_calls(&w.cnt[3], ncalls);
+  count_calls(w.cnt + wind::SURR_CALLS, ncalls);
   reduce_dtcool_any(cand, bad, sg);
 }
 
 __device__ __forceinline__ void surr_mlp_group(const Weights& wt, const SurrWork& w, FwdLds& S,
-                                               long long n, long long s0) {
+                                               long long n, long long s0, float* ylds) {
   const int l = threadIdx.x, r = l & 15, kq = (l >> 4) * 4;
   const long long sidx = s0 + r;
   const bool valid = sidx < n;
@@ -545,22 +543,13 @@ __device__ __forceinline__ void surr_mlp_group(const Weights& wt, const SurrWork
   epilogue(acc, wt.P + OFF_B3, nullptr, S.H3, nullptr, l);
   __syncthreads();
   float y = output_unit(S.H3, wt, l);
-  if (l < 16 && valid) w.yhat[sidx] = y * YS;
+  if (l < 16) ylds[r] = y * YS;
   __syncthreads();   // S is reused by the next group
 }
 
-// Network forward pass over the listed cells, 16 per wavefront; the grid
-// strides over the list, whose length is known only on the device.
-__global__ __launch_bounds__(64) void surr_mlp_kernel(Weights wt, SurrWork w) {
-  __shared__ FwdLds S;
-  const long long n = (long long)w.cnt[0];
-  for (long long s0 = (long long)blockIdx.x * 16; s0 < n; s0 += (long long)gridDim.x * 16)
-    surr_mlp_group(wt, w, S, n, s0);
-}
-
-// Newton refinement of the network's guess; failures join the queue.
+// Newton refinement of the network's guess yhat; failures join the queue.
 __device__ __forceinline__ void surr_refine_cell(const CoolBatch& b, const SurrWork& w,
-                                                 long long qi, long long n) {
+                                                 long long qi, long long n, float yhat) {
   double cand = 1.0e300;
   bool fail = false, ok = false;
   long long q = 0;
@@ -579,7 +568,7 @@ __device__ __forceinline__ void surr_refine_cell(const CoolBatch& b, const SurrW
     // guess: the network's, or (ablation) the explicit step limited to one
     // e-fold, which is what a solver without the network starts from
     const double f0 = w.f0[qi];
-    const double Eg = w.guess == 0 ? E0 * pow(10.0, (double)w.yhat[qi])
+    const double Eg = w.guess == 0 ? E0 * pow(10.0, (double)yhat)
                                    : E0 * exp(fmax(-1.0, fmin(1.0, f0 * dt / E0)));
     double Ef;
     if (wind::refine_cell(k, R, E0, f0, dt, Eg, Ef) > 0) {
@@ -590,21 +579,32 @@ __device__ __forceinline__ void surr_refine_cell(const CoolBatch& b, const SurrW
     }
     ncalls = R.nev;
   }
-  const long long fi = wave_append(&w.cnt[1], fail);
+  const long long fi = wave_append(&w.cnt[1], fail, &w.cnt[5]);
   if (fail) w.fb[fi] = q;
-  wave_append(&w.cnt[2], ok);
-  count_calls(&w.cnt[3], ncalls);
+  wave_append(&w.cnt[6], ok);
+  count_calls(w.cnt + wind::SURR_CALLS, ncalls);
   reduce_dtcool_any(cand, 0, sg);
 }
 
-// The grid strides over the list (length known only on the device); the
-// loop bound is uniform over each workgroup, so every wavefront takes part
-// in the wavefront-wide appends and reductions of every pass.
-__global__ __launch_bounds__(128) void surr_refine_kernel(CoolBatch b, SurrWork w) {
+// The listed cells, 64 per wavefront and pass: the network's forward pass
+// (four MFMA groups of 16 cells), then one cell per lane refined from its
+// guess.  The grid strides over the list, whose length is known only on
+// the device; the loop bound is uniform over the (one-wavefront)
+// workgroup, so every lane takes part in the wavefront-wide appends and
+// reductions of every pass.
+__global__ __launch_bounds__(64) void surr_mlp_refine_kernel(Weights wt, CoolBatch b, SurrWork w) {
+  __shared__ FwdLds S;
+  __shared__ float ylds[64];
   const long long n = (long long)w.cnt[0];
-  for (long long base = (long long)blockIdx.x * blockDim.x; base < n;
-       base += (long long)gridDim.x * blockDim.x)
-    surr_refine_cell(b, w, base + threadIdx.x, n);
+  for (long long s0 = (long long)blockIdx.x * 64; s0 < n; s0 += (long long)gridDim.x * 64) {
+    if (w.guess == 0) {
+#pragma unroll 1
+      for (int g = 0; g < 4 && s0 + 16 * g < n; g++)
+        surr_mlp_group(wt, w, S, n, s0 + 16 * g, ylds + 16 * g);
+    }
+    surr_refine_cell(b, w, s0 + threadIdx.x, n, w.guess == 0 ? ylds[threadIdx.x] : 0.f);
+    __syncthreads();   // ylds is rewritten by the next pass
+  }
 }
 
 // The plain integrator for the queued cells.
@@ -624,7 +624,7 @@ __device__ __forceinline__ void surr_fallback_cell(const CoolBatch& b, const Sur
     a.V[PRS * a.sv + cell] = wind::plain_cell(a.k, R, p, a.dtc ? a.dtc[cell] : a.s->dt, cand, bad);
     ncalls = R.nev;
   }
-  count_calls(&w.cnt[3], ncalls);
+  count_calls(w.cnt + wind::SURR_CALLS, ncalls);
   reduce_dtcool_any(cand, bad, sg);
 }
 
@@ -728,13 +728,11 @@ int surrogate_cool(pmx_mlp* m, const wind::CoolBatch& b, const wind::SurrWork& w
   // the list and queue lengths are known only on the device: those kernels
   // stride over them with at most SURR_GRID workgroups
   const unsigned nr = std::min(nb, (unsigned)SURR_GRID);
-  const unsigned nm = (unsigned)std::min((b.total + 15) / 16, (long long)SURR_GRID);
-  hipLaunchKernelGGL(surr_reset_kernel, dim3(1), dim3(64), 0, st, w.cnt);
+  const unsigned nm = (unsigned)std::min((b.total + 63) / 64, (long long)SURR_GRID);
   hipLaunchKernelGGL(surr_classify_kernel, dim3(nb), dim3(128), 0, st, b, w);
-  hipLaunchKernelGGL(surr_mlp_kernel, dim3(nm), dim3(64), 0, st, m->w, w);
-  hipLaunchKernelGGL(surr_refine_kernel, dim3(nr), dim3(128), 0, st, b, w);
+  hipLaunchKernelGGL(surr_mlp_refine_kernel, dim3(nm), dim3(64), 0, st, m->w, b, w);
   hipLaunchKernelGGL(surr_fallback_kernel, dim3(nr), dim3(128), 0, st, b, w);
-  hipLaunchKernelGGL(surr_totals_kernel, dim3(1), dim3(64), 0, st, w.cnt);
+  hipLaunchKernelGGL(surr_end_call_kernel, dim3(1), dim3(64), 0, st, w.cnt);
   PMX_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -943,15 +941,15 @@ int pmx_mlp_cool_cells(pmx_mlp* m, const pmx_wind_desc* d, long long n, const do
   A((void**)&sw.list, sizeof(long long) * n);
   A((void**)&sw.fb, sizeof(long long) * n);
   A((void**)&sw.feat, sizeof(float) * 4 * n);
-  A((void**)&sw.yhat, sizeof(float) * n);
   A((void**)&sw.f0, sizeof(double) * n);
-  A((void**)&sw.cnt, sizeof(unsigned long long) * 8);
+  A((void**)&sw.cnt, sizeof(unsigned long long) * PMX_SURR_COUNTERS);
   wind::CoolArgs* dargs = nullptr;
   long long* doff = nullptr;
   A((void**)&dargs, sizeof(wind::CoolArgs));
   A((void**)&doff, sizeof(long long) * 2);
   wind::Scalars s0;
   memset(&s0, 0, sizeof(s0));
+  wind::red_clear(s0);
   s0.tstop = HUGE_VAL;
   s0.t = 5.0;    // g_time > 3: Radiat takes xi and T_r from the per-cell arrays
   s0.dt = 0.0;
@@ -965,7 +963,7 @@ int pmx_mlp_cool_cells(pmx_mlp* m, const pmx_wind_desc* d, long long n, const do
   if (e == hipSuccess) e = hipMemcpy(ddt, dt, sizeof(double) * n, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dx1, x1.data(), sizeof(double) * 4, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(ds, &s0, sizeof(s0), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(sw.cnt, 0, sizeof(unsigned long long) * 8);
+  if (e == hipSuccess) e = hipMemset(sw.cnt, 0, sizeof(unsigned long long) * PMX_SURR_COUNTERS);
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
   if (e == hipSuccess) {
@@ -983,7 +981,7 @@ int pmx_mlp_cool_cells(pmx_mlp* m, const pmx_wind_desc* d, long long n, const do
     if (e == hipSuccess) {
       if (mode == 0)
         hipLaunchKernelGGL(plain_cells_kernel, dim3((unsigned)((n + 127) / 128)), dim3(128), 0,
-                           m->stream, b, sw.cnt + 7);
+                           m->stream, b, sw.cnt + wind::SURR_CALLS);
       else if (surrogate_cool(m, b, sw, m->stream))
         e = hipErrorUnknown;
     }
@@ -994,10 +992,10 @@ int pmx_mlp_cool_cells(pmx_mlp* m, const pmx_wind_desc* d, long long n, const do
   float fms = 0.f;
   if (e == hipSuccess) e = hipEventElapsedTime(&fms, e0, e1);
   if (e == hipSuccess) e = hipMemcpy(V.data(), dV, sizeof(double) * 6 * n, hipMemcpyDeviceToHost);
-  unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long c[PMX_SURR_COUNTERS] = {};
   if (e == hipSuccess) e = hipMemcpy(c, sw.cnt, sizeof(c), hipMemcpyDeviceToHost);
   hipFree(dV); hipFree(dxi); hipFree(dtr); hipFree(ddt); hipFree(dx1); hipFree(ds);
-  hipFree(sw.list); hipFree(sw.fb); hipFree(sw.feat); hipFree(sw.yhat); hipFree(sw.f0);
+  hipFree(sw.list); hipFree(sw.fb); hipFree(sw.feat); hipFree(sw.f0);
   hipFree(sw.cnt); hipFree(dargs); hipFree(doff);
   if (e0) hipEventDestroy(e0);
   if (e1) hipEventDestroy(e1);
@@ -1005,7 +1003,7 @@ int pmx_mlp_cool_cells(pmx_mlp* m, const pmx_wind_desc* d, long long n, const do
   for (long long q = 0; q < n; q++) p_out[q] = V[PRS * n + q];
   if (stats) {
     stats[0] = (long long)c[4]; stats[1] = (long long)c[6]; stats[2] = (long long)c[5];
-    stats[3] = (long long)c[7];
+    stats[3] = (long long)wind::surr_calls(c);
   }
   if (ms) *ms = fms;
   return 0;

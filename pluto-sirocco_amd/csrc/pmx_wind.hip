@@ -676,7 +676,8 @@ __device__ __forceinline__ bool block_reduce_stage(Scalars* s, double cdt, int f
     int f = sflag[0];
     for (int q = 1; q < NT / 64; q++) { m = dmax(m, smax[q]); f |= sflag[q]; }
     if (STAGE == 1 && m > 0.0)
-      atomicMax(&s->cdt_bits, (unsigned long long)__double_as_longlong(m));
+      atomicMax(&s->cdt_red[red_index(blockIdx.x + blockIdx.y * gridDim.x)],
+                (unsigned long long)__double_as_longlong(m));
     if (f) atomicOr(&s->err, f);
   }
   __syncthreads();   // the shared slots are reused by the next call
@@ -735,10 +736,9 @@ __device__ __forceinline__ void stage_grid(const StageArgs& a) {
       // update_stage.c:389-394: invDt = MAX(invDt, C_dt) over the domain,
       // then / DIMENSIONS (stage-1 values, complete when stage 2 runs)
       double inv = a.s->inv_dt_hyp;
-      inv = dmax(inv, __longlong_as_double((long long)a.s->cdt_bits));
+      inv = dmax(inv, take_cdt(a.s));
       inv /= (double)2;
       a.s->inv_dt_hyp = inv;
-      a.s->cdt_bits = 0ull;
     }
   }
 }
@@ -790,20 +790,29 @@ __global__ __launch_bounds__(128) void wind_cool_kernel(CoolArgs a, unsigned lon
     a.V[PRS * a.sv + cell] = pn;
   }
   if (MODE == PMX_COOL_PLAIN) {
+    // one device atomic per workgroup and reduction, each into the
+    // workgroup's spread slot
     __shared__ double smin[2];
     __shared__ int sbad[2];
+    __shared__ long long scalls[2];
     for (int off = 32; off > 0; off >>= 1) {
       cand = dmin(cand, __shfl_xor(cand, off));
       bad |= __shfl_xor(bad, off);
       ncalls += __shfl_xor(ncalls, off);
     }
-    if ((threadIdx.x & 63) == 0 && ncalls) atomicAdd(calls, (unsigned long long)ncalls);
-    if ((threadIdx.x & 63) == 0) { smin[threadIdx.x >> 6] = cand; sbad[threadIdx.x >> 6] = bad; }
+    if ((threadIdx.x & 63) == 0) {
+      smin[threadIdx.x >> 6] = cand;
+      sbad[threadIdx.x >> 6] = bad;
+      scalls[threadIdx.x >> 6] = ncalls;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
+      const long long nc = scalls[0] + scalls[1];
+      if (nc) atomicAdd(calls + red_index(blockIdx.x), (unsigned long long)nc);
       double m = dmin(smin[0], smin[1]);
       if (m >= 0.0)
-        atomicMin(&a.s->dtcool_bits, (unsigned long long)__double_as_longlong(m));
+        atomicMin(&a.s->dtcool_red[red_index(blockIdx.x)],
+                  (unsigned long long)__double_as_longlong(m));
       if (sbad[0] | sbad[1]) atomicOr(&a.s->err, PMX_FLAG_COOL_SUBSTEPS);
     }
   }
@@ -820,6 +829,7 @@ __global__ void wind_end_kernel(Scalars* s, int next, double cfl, double cfl_max
   if (s->halt) return;
   double dt = s->dt;
   s->t += dt;
+  fold_dtcool(s);
   double dt_cool = __longlong_as_double((long long)s->dtcool_bits);
   s->dt_cool = dt_cool;
   if (next) {
@@ -1164,10 +1174,10 @@ int cool(pmx_wind* w, bool loop) {
     if (mlp::surrogate_cool(w->mlp, b, w->sw, w->stream)) return -1;
   } else if (w->cooling == PMX_COOL_BLONDIN)
     hipLaunchKernelGGL(wind_cool_kernel<PMX_COOL_BLONDIN>, grid, block, 0, w->stream, a,
-                       w->sw.cnt + 7);
+                       w->sw.cnt + SURR_CALLS);
   else
     hipLaunchKernelGGL(wind_cool_kernel<PMX_COOL_PLAIN>, grid, block, 0, w->stream, a,
-                       w->sw.cnt + 7);
+                       w->sw.cnt + SURR_CALLS);
   PMX_HIP_CHECK(hipGetLastError());
   return timed_end(w, e, true);
 }
@@ -1257,7 +1267,7 @@ int pmx_wind_destroy(pmx_wind* w) {
   for (auto& p : w->pend_cool) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
   hipFree(w->V[0]); hipFree(w->V[1]); hipFree(w->U); hipFree(w->xi_s); hipFree(w->tr_s);
   hipFree(w->pre); hipFree(w->fm); hipFree(w->tab); hipFree(w->lf); hipFree(w->s); hipFree(w->shock); hipFree(w->Ue);
-  hipFree(w->sw.list); hipFree(w->sw.fb); hipFree(w->sw.feat); hipFree(w->sw.yhat);
+  hipFree(w->sw.list); hipFree(w->sw.fb); hipFree(w->sw.feat);
   hipFree(w->sw.f0); hipFree(w->sw.cnt); hipFree(w->cargs); hipFree(w->coff);
   if (w->copy_stream) {
     hipStreamSynchronize(w->copy_stream);
@@ -1497,8 +1507,8 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
   }
   w->T.x1l0 = x1l[0];
   w->T.sin_x2l0 = fabs(sin(x2l[0]));
-  CK(hipMalloc(&w->sw.cnt, sizeof(unsigned long long) * 8));
-  CK(hipMemset(w->sw.cnt, 0, sizeof(unsigned long long) * 8));
+  CK(hipMalloc(&w->sw.cnt, sizeof(unsigned long long) * PMX_SURR_COUNTERS));
+  CK(hipMemset(w->sw.cnt, 0, sizeof(unsigned long long) * PMX_SURR_COUNTERS));
   CK(hipMalloc(&w->cargs, sizeof(CoolArgs)));
   CK(hipMalloc(&w->coff, sizeof(long long) * 2));
   CK(hipMemcpy(w->tab, packed.data(), sizeof(double) * packed.size(), hipMemcpyHostToDevice));
@@ -1518,6 +1528,7 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
   }
   Scalars s0;
   memset(&s0, 0, sizeof(s0));
+  red_clear(s0);
   s0.tstop = HUGE_VAL;
   s0.dt_cool = 1.e38;
   s0.dtcool_bits = (unsigned long long)0;
@@ -1692,6 +1703,7 @@ int pmx_wind_set_dts(pmx_wind* w, double inv_dt_hyp, double dt_cool) {
   s0.inv_dt_hyp = inv_dt_hyp;
   s0.dt_cool = dt_cool;
   memcpy(&s0.dtcool_bits, &dt_cool, sizeof(double));
+  red_clear(s0);
   PMX_HIP_CHECK(hipMemcpyAsync(w->s, &s0, sizeof(Scalars), hipMemcpyHostToDevice, w->stream));
   PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
   return 0;
@@ -1709,6 +1721,7 @@ int pmx_wind_get_time(pmx_wind* w, double* t, double* dt, long* step, double* in
   if (inv_dt_hyp) *inv_dt_hyp = s0.inv_dt_hyp;
   if (dt_cool) {
     double c;
+    fold_dtcool(&s0);
     memcpy(&c, &s0.dtcool_bits, sizeof(double));
     *dt_cool = c;
   }
@@ -1798,7 +1811,6 @@ int pmx_wind_set_surrogate(pmx_wind* w, pmx_mlp* m, int guess) {
     PMX_HIP_CHECK(hipMalloc(&w->sw.list, sizeof(long long) * n));
     PMX_HIP_CHECK(hipMalloc(&w->sw.fb, sizeof(long long) * n));
     PMX_HIP_CHECK(hipMalloc(&w->sw.feat, sizeof(float) * 4 * n));
-    PMX_HIP_CHECK(hipMalloc(&w->sw.yhat, sizeof(float) * n));
     PMX_HIP_CHECK(hipMalloc(&w->sw.f0, sizeof(double) * n));
   }
   w->mlp = m;
@@ -1813,16 +1825,17 @@ int pmx_wind_set_surrogate(pmx_wind* w, pmx_mlp* m, int guess) {
 int pmx_wind_surrogate_stats(pmx_wind* w, long long* listed, long long* accepted,
                              long long* fallback, long long* radiat_calls, int reset) {
   if (!w) { set_error("null argument"); return -1; }
-  unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  unsigned long long c[PMX_SURR_COUNTERS];
   {
     PMX_HIP_CHECK(hipMemcpyAsync(c, w->sw.cnt, sizeof(c), hipMemcpyDeviceToHost, w->stream));
     PMX_HIP_CHECK(hipStreamSynchronize(w->stream));
-    if (reset) PMX_HIP_CHECK(hipMemset(w->sw.cnt + 4, 0, sizeof(unsigned long long) * 4));
+    if (reset)
+      PMX_HIP_CHECK(hipMemset(w->sw.cnt + 4, 0, sizeof(unsigned long long) * (PMX_SURR_COUNTERS - 4)));
   }
   if (listed) *listed = (long long)c[4];
   if (accepted) *accepted = (long long)c[6];
   if (fallback) *fallback = (long long)c[5];
-  if (radiat_calls) *radiat_calls = (long long)c[7];
+  if (radiat_calls) *radiat_calls = (long long)surr_calls(c);
   return 0;
 }
 
@@ -1892,12 +1905,12 @@ __global__ void wind_batch_end_kernel(const StageArgs* args, int G, int next, co
   Scalars* s = args[g].s;
   if (s->halt) return;
   double inv = s->inv_dt_hyp;
-  inv = dmax(inv, __longlong_as_double((long long)s->cdt_bits));
+  inv = dmax(inv, take_cdt(s));
   inv /= (double)2;
   s->inv_dt_hyp = inv;
-  s->cdt_bits = 0ull;
   const double dt = s->dt;
   s->t += dt;
+  fold_dtcool(s);
   const double dt_cool = __longlong_as_double((long long)s->dtcool_bits);
   s->dt_cool = dt_cool;
   if (next) {
@@ -2075,7 +2088,7 @@ int pmx_wind_batch_destroy(pmx_wind_batch* b) {
   for (auto w : b->grids) pmx_wind_destroy(w);
   hipFree(b->args1); hipFree(b->args2); hipFree(b->tile_off); hipFree(b->cfl);
   hipFree(b->cfl_max_var); hipFree(b->cargs); hipFree(b->cell_off); hipFree(b->calls);
-  hipFree(b->sw.list); hipFree(b->sw.fb); hipFree(b->sw.feat); hipFree(b->sw.yhat);
+  hipFree(b->sw.list); hipFree(b->sw.fb); hipFree(b->sw.feat);
   hipFree(b->sw.f0); hipFree(b->sw.cnt);
   if (b->stream) hipStreamDestroy(b->stream);
   delete b;
@@ -2130,8 +2143,8 @@ int pmx_wind_batch_create(const pmx_wind_desc* descs, int ngrids, pmx_wind_batch
   CK(hipMalloc(&b->cfl_max_var, sizeof(double) * G));
   CK(hipMalloc(&b->cargs, sizeof(CoolArgs) * G));
   CK(hipMalloc(&b->cell_off, sizeof(long long) * (G + 1)));
-  CK(hipMalloc(&b->calls, sizeof(unsigned long long)));
-  CK(hipMemset(b->calls, 0, sizeof(unsigned long long)));
+  CK(hipMalloc(&b->calls, sizeof(unsigned long long) * RED_SLOTS * RED_STRIDE));
+  CK(hipMemset(b->calls, 0, sizeof(unsigned long long) * RED_SLOTS * RED_STRIDE));
   CK(hipMemcpy(b->args1, a1.data(), sizeof(StageArgs) * G, hipMemcpyHostToDevice));
   CK(hipMemcpy(b->args2, a2.data(), sizeof(StageArgs) * G, hipMemcpyHostToDevice));
   CK(hipMemcpy(b->cfl, cfl.data(), sizeof(double) * G, hipMemcpyHostToDevice));
@@ -2164,10 +2177,9 @@ int pmx_wind_batch_set_surrogate(pmx_wind_batch* b, pmx_mlp* m, int guess) {
     PMX_HIP_CHECK(hipMalloc(&b->sw.list, sizeof(long long) * n));
     PMX_HIP_CHECK(hipMalloc(&b->sw.fb, sizeof(long long) * n));
     PMX_HIP_CHECK(hipMalloc(&b->sw.feat, sizeof(float) * 4 * n));
-    PMX_HIP_CHECK(hipMalloc(&b->sw.yhat, sizeof(float) * n));
     PMX_HIP_CHECK(hipMalloc(&b->sw.f0, sizeof(double) * n));
-    PMX_HIP_CHECK(hipMalloc(&b->sw.cnt, sizeof(unsigned long long) * 8));
-    PMX_HIP_CHECK(hipMemset(b->sw.cnt, 0, sizeof(unsigned long long) * 8));
+    PMX_HIP_CHECK(hipMalloc(&b->sw.cnt, sizeof(unsigned long long) * PMX_SURR_COUNTERS));
+    PMX_HIP_CHECK(hipMemset(b->sw.cnt, 0, sizeof(unsigned long long) * PMX_SURR_COUNTERS));
   }
   b->mlp = m;
   b->sw.guess = guess;

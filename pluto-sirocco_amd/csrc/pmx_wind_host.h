@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "../../include/plutomx.h"
+#include "pmx_common.h"
 #include "pmx_wind_math.h"
 
 namespace pmx {
@@ -20,7 +21,46 @@ struct Scalars {
   int halt;                        // PMX_FLAG_DT_TOO_SMALL / _COOL_SUBSTEPS / _TSTOP once set
                                    // (sticky): later main-loop steps are no-ops
   int last;                        // the next step is the last one (dt cut to tstop - t)
+  // spread slots of the two reductions (words red_index(k)): the stage-1
+  // C_dt maxima (0 = none) and the cooling dt minima (~0 = none), folded
+  // into cdt_bits / dtcool_bits by take_cdt / fold_dtcool
+  unsigned long long cdt_red[RED_SLOTS * RED_STRIDE];
+  unsigned long long dtcool_red[RED_SLOTS * RED_STRIDE];
 };
+
+// Empty spread slots.
+__host__ __device__ inline void red_clear(Scalars& s) {
+  for (int k = 0; k < RED_SLOTS; k++) {
+    s.cdt_red[k * RED_STRIDE] = 0ull;
+    s.dtcool_red[k * RED_STRIDE] = ~0ull;
+  }
+}
+
+// The stage-1 maximum of C_dt (non-negative doubles order like their bit
+// patterns); clears the words for the next step.
+__device__ inline double take_cdt(Scalars* s) {
+  unsigned long long m = s->cdt_bits;
+#pragma unroll
+  for (int k = 0; k < RED_SLOTS; k++) {
+    const unsigned long long v = s->cdt_red[k * RED_STRIDE];
+    m = v > m ? v : m;
+  }
+  s->cdt_bits = 0ull;
+#pragma unroll
+  for (int k = 0; k < RED_SLOTS; k++) s->cdt_red[k * RED_STRIDE] = 0ull;
+  return __longlong_as_double((long long)m);
+}
+
+// Folds the cooling dt candidates of the spread slots into dtcool_bits.
+__host__ __device__ inline void fold_dtcool(Scalars* s) {
+  unsigned long long m = s->dtcool_bits;
+  for (int k = 0; k < RED_SLOTS; k++) {
+    const unsigned long long v = s->dtcool_red[k * RED_STRIDE];
+    m = v < m ? v : m;
+    s->dtcool_red[k * RED_STRIDE] = ~0ull;
+  }
+  s->dtcool_bits = m;
+}
 
 // Src/main.c:226-229: the step that would reach tstop (to 1e-8 relative)
 // is the last one, and its dt is cut to land on tstop.
@@ -88,15 +128,27 @@ __device__ __forceinline__ int find_grid(const long long* off, int G, long long 
   return lo;
 }
 
+constexpr int SURR_CALLS = 8;
+#define PMX_SURR_COUNTERS (::pmx::wind::SURR_CALLS + ::pmx::RED_SLOTS * ::pmx::RED_STRIDE)
+
+// Sum of the Radiat-call slots of a host copy of the counters.
+inline unsigned long long surr_calls(const unsigned long long* c) {
+  unsigned long long n = 0;
+  for (int k = 0; k < RED_SLOTS; k++) n += c[SURR_CALLS + red_index(k)];
+  return n;
+}
+
 // Work buffers of the surrogate-cooled SplitSource (one entry per cell).
 struct SurrWork {
   long long* list;       // stiff cells (global index) that take the network + refinement
   long long* fb;         // cells that take the plain integrator
   float* feat;           // [cell-list][4] network inputs
-  float* yhat;
   double* f0;            // initial rate of each listed cell
-  unsigned long long* cnt;   // [0] listed, [1] fallback, [2] accepted, [3] Radiat calls;
-                             // [4..7] running totals of the same
+  unsigned long long* cnt;   // PMX_SURR_COUNTERS entries, zeroed at creation: [0], [1]
+                             // the tails of list and fb during a call (zero between
+                             // calls); running totals [4] listed, [5] fallback,
+                             // [6] accepted; Radiat calls in the spread slots
+                             // cnt[SURR_CALLS + red_index(k)] (summed by the reader)
   int guess;                 // 0: the network's guess; 1: no network (ablation)
 };
 

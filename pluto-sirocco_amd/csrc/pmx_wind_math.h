@@ -653,11 +653,12 @@ PMX_HD double finish_cell(const Const& k, const RadCell& R, double p, double E1,
 
 // One cell of CoolingSource: returns the new pressure and the cell's
 // dt * maxCoolingRate / |dp/p| candidate for dt_cool.
-PMX_HD double plain_cell(const Const& k, const RadCell& R, double p,
-                                             double dt, double& dt_cool_cell, int& bad) {
+// plain_cell_from: the same from E0 and k1 = Radiat(E0) already formed
+// (E0 as Radiat left it), for a caller that has evaluated the first rate.
+PMX_HD double plain_cell_from(const Const& k, const RadCell& R, double p, double dt, double E0,
+                              double k1, double& dt_cool_cell, int& bad) {
   const double min_tol = 2.e-5;
-  double E0 = p / (k.gamma - 1.0), E1 = E0;
-  double k1 = radiat(k, R, E0);
+  double E1 = E0;
   int stiff = (fabs(k1) > 0.5 / dt ? 1 : 0);
   double err = 0.0;
   if (!stiff) {
@@ -681,6 +682,13 @@ PMX_HD double plain_cell(const Const& k, const RadCell& R, double p,
     } while (!done);
   }
   return finish_cell(k, R, p, E1, dt, dt_cool_cell);
+}
+
+PMX_HD double plain_cell(const Const& k, const RadCell& R, double p, double dt,
+                         double& dt_cool_cell, int& bad) {
+  double E0 = p / (k.gamma - 1.0);
+  const double k1 = radiat(k, R, E0);
+  return plain_cell_from(k, R, p, dt, E0, k1, dt_cool_cell, bad);
 }
 
 // ---------------------------------------------------------------------

@@ -90,10 +90,10 @@ def test_snapshot_is_taken_when_queued(pmx):
         with pytest.raises(pmx.PmxError, match="still referenced"):
             w.snapshot(user_vars=False)
         del V
+        V8 = w.get_prim()
         w.snapshot(user_vars=False)
         w.run(1)
         s3 = w.snapshot_wait(copy=False)
-        V8 = w.get_prim()
     # ... and closing the context leaves the buffer to the view
     assert np.array_equal(s3["V"], V8)
     del s3
