@@ -143,8 +143,8 @@ int pmx_hydro_advance_step_noalloc(const pmx_hydro_desc *desc, double *V, double
 int pmx_hydro_run(pmx_hydro *h, int nsteps);
 int pmx_hydro_sync(pmx_hydro *h);
 
-/* Same as pmx_hydro_run, but replays a captured HIP graph of one step
- * (built on first use). */
+/* Same as pmx_hydro_run, but replays a captured HIP graph of 8 steps
+ * (built on first use); a remainder of fewer steps runs eagerly. */
 int pmx_hydro_run_graph(pmx_hydro *h, int nsteps);
 
 /* Per-kernel timing with HIP events recorded on the context's stream
@@ -317,10 +317,11 @@ int pmx_wind_split_source(pmx_wind *w);
  * invDt / dt_cool reset and NextTimeStep every second step, all on the
  * device.  Asynchronous; pmx_wind_sync() waits. */
 int pmx_wind_run(pmx_wind *w, int nsteps);
-/* Same as pmx_wind_run, but replays a captured HIP graph of a step pair
- * (an even step, then an odd one; built on first use, rebuilt after
- * pmx_wind_set_surrogate).  Bit-identical to pmx_wind_run; with timing
- * enabled it runs eagerly. */
+/* Same as pmx_wind_run, but replays a captured HIP graph of 8 steps
+ * (from an even step; built on first use, rebuilt after
+ * pmx_wind_set_surrogate); an odd first step and a remainder of fewer
+ * steps run eagerly.  Bit-identical to pmx_wind_run; with timing enabled
+ * it runs eagerly. */
 int pmx_wind_run_graph(pmx_wind *w, int nsteps);
 int pmx_wind_sync(pmx_wind *w);
 /* Per-kernel timing (HIP events on the context's stream) of the stage and

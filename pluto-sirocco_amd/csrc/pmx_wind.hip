@@ -1017,7 +1017,7 @@ struct pmx_wind {
   bool snap_pending;
   int snap_nvar;
   long snap_step;
-  // pmx_wind_run_graph: a captured (even, odd) step pair
+  // pmx_wind_run_graph: GRAPH_STEPS captured steps from an even one
   hipGraphExec_t graph;
   bool have_graph;
 };
@@ -1763,6 +1763,12 @@ int pmx_wind_run(pmx_wind* w, int nsteps) {
   return 0;
 }
 
+// Steps per captured graph (an even number: the graph starts on an even
+// step, whose split cooling comes after the hydro step): a graph launch
+// costs more than the launches of one step pair save, so a graph holds
+// several pairs.  Steps after the run halts are no-ops on the device.
+constexpr int GRAPH_STEPS = 8;
+
 int pmx_wind_run_graph(pmx_wind* w, int nsteps) {
   if (!w || nsteps < 0) { set_error("bad argument"); return -1; }
   if (w->timing) return pmx_wind_run(w, nsteps);
@@ -1772,19 +1778,20 @@ int pmx_wind_run_graph(pmx_wind* w, int nsteps) {
     w->step++;
     n++;
   }
-  if (nsteps - n >= 2 && !w->have_graph) {
+  if (nsteps - n >= GRAPH_STEPS && !w->have_graph) {
     hipGraph_t gph;
     PMX_HIP_CHECK(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = wind_step(w, 0) | wind_step(w, 1);
+    int rc = 0;
+    for (int q = 0; q < GRAPH_STEPS; q++) rc |= wind_step(w, q);
     const hipError_t e = hipStreamEndCapture(w->stream, &gph);
     if (rc || e != hipSuccess) { set_error("graph capture failed"); return -1; }
     PMX_HIP_CHECK(hipGraphInstantiate(&w->graph, gph, nullptr, nullptr, 0));
     PMX_HIP_CHECK(hipGraphDestroy(gph));
     w->have_graph = true;
   }
-  for (; nsteps - n >= 2; n += 2) {
+  for (; nsteps - n >= GRAPH_STEPS; n += GRAPH_STEPS) {
     PMX_HIP_CHECK(hipGraphLaunch(w->graph, w->stream));
-    w->step += 2;
+    w->step += GRAPH_STEPS;
   }
   for (; n < nsteps; n++) {
     if (wind_step(w, w->step)) return -1;

@@ -144,15 +144,18 @@ def test_ghost_download_follows_boundaries():
 
 
 def test_graph_replay_matches_eager():
+    """The captured graph holds 8 steps: three replays separated by host
+    work and syncs, each followed by steps that run eagerly (fewer than a
+    graph's worth), match the eager run bit for bit."""
     cfg = ODD[1]
     say("[gpu] graph replay")
     V0 = _ic(cfg, 11)
-    ref = run_gpu(cfg, V0, 4)
+    ref = run_gpu(cfg, V0, 3 * 11)
     with plutomx.Hydro(cfg) as h:
         h.set_prim(V0)
         h.set_time(0.0, cfg.first_dt)
-        for _ in range(4):           # replays separated by host work and syncs
-            h.run(1, graph=True)
+        for _ in range(3):
+            h.run(11, graph=True)    # 8 replayed + 3 eager
             h.sync()
             _ = h.get_time()
         assert_bitwise(h.get_prim(), ref["V"], "V after graph replays")

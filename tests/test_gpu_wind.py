@@ -366,9 +366,10 @@ def test_brem_only_cooling_closed_forms_on_device(pmx, dt):
 
 @pytest.mark.parametrize("kind", ["plain", "production"])
 def test_wind_graph_replay_matches_eager(pmx, kind):
-    """pmx_wind_run_graph replays a captured (even, odd) step pair: every
+    """pmx_wind_run_graph replays 8 captured steps from an even step: every
     replay, with host reads and synchronisations between them, matches the
-    eager run bit for bit, from an odd start step too."""
+    eager run bit for bit, from an odd start step (one eager step first)
+    and with a remainder that runs eagerly."""
     import dataclasses
     if kind == "plain":
         cfg = setups.WindConfig(nx=(64, 32), first_dt=1e-3)
@@ -382,16 +383,16 @@ def test_wind_graph_replay_matches_eager(pmx, kind):
         w.set_prim(V0)
         w.set_time(0.0, cfg.first_dt, 0)
         w.run(1)
-        for _ in range(4):
-            w.run(2)
+        for _ in range(3):
+            w.run(9)
             w.sync()
             eager.append((w.get_prim(), w.get_time()))
     with pmx.Wind(cfg) as w:
         w.set_prim(V0)
         w.set_time(0.0, cfg.first_dt, 0)
         w.run(1, graph=True)
-        for q in range(4):
-            w.run(2, graph=True)
+        for q in range(3):
+            w.run(9, graph=True)     # steps 1..9: 1 eager + 8; 10..18: 8 + 1; 19..27: 1 + 8
             w.sync()
             V, tm = w.get_prim(), w.get_time()
             assert np.array_equal(V, eager[q][0]), q
