@@ -44,6 +44,23 @@ enum { MODE_FIRST = 0, MODE_WEIGHTED = 1, MODE_THIRD = 2 };
 #define PMX_PREFETCH_U 0   // load U and V0 of the cell at the top of the plane loop
 #endif
 
+#ifndef PMX_MAX_ZCHUNKS
+#define PMX_MAX_ZCHUNKS 48
+#endif
+
+// Tile of the x-y plane per workgroup: every per-plane phase (x-states,
+// x-faces, y-states, y-faces) has at most 256 items, so each is a single
+// pass of the 256-thread workgroup.
+#ifndef PMX_TILE_X
+#define PMX_TILE_X 16
+#endif
+#ifndef PMX_TILE_Y
+#define PMX_TILE_Y 14
+#endif
+constexpr int TILE_X = PMX_TILE_X;
+constexpr int TILE_Y = PMX_TILE_Y;
+constexpr int NTHREADS = 256;
+
 struct StageArgs {
   const double* Vin;   // stage input, halo source
   const double* V0;    // V at the start of the step (U0 = P2C(V0)); may alias Vout
@@ -56,8 +73,14 @@ struct StageArgs {
   const double* dtdx[3];    // dt/dx per cell and direction (this step)
   const double* inv_dl[3];  // 1/dx per cell and direction
   double w0, wc;            // RK weights for MODE_WEIGHTED
-  int cz;                   // z planes per workgroup
+  int cz;                   // z planes per workgroup (uniform chunks, nzc == 0)
   int tiles_x, tiles_y, tiles_z;
+  // z chunks of unequal length (3-D, nzc > 0): chunk c is planes
+  // [zc0[c], zc0[c+1]); workgroup b takes chunk b / (tiles_x tiles_y) of
+  // tile b % (tiles_x tiles_y), so the long chunks are dispatched first
+  // and the short ones fill the end of the launch
+  int nzc;
+  int zc0[PMX_MAX_ZCHUNKS + 1];
   int write_u;              // store U (always needed except after the last stage)
   unsigned long long* cdt_bits;  // MODE_FIRST: max over cells of C_dt (bit pattern), spread
                                  // over the words red_index(k) by workgroup
@@ -175,15 +198,26 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
   const bool owner = tid < NC;         // thread owns tile cell (tx, ty)
   const int tx = tid % TX, ty = tid / TX;
   const int nblocks = a.tiles_x * a.tiles_y * a.tiles_z;
-  const int lb = xcd_remap(blockIdx.x, nblocks);
+  const bool zsched = DIMS == 3 && a.nzc > 0;
+  const int lb = __builtin_amdgcn_readfirstlane(zsched ? (int)blockIdx.x : xcd_remap(blockIdx.x, nblocks));
   const int bx = lb % a.tiles_x;
   const int by = (lb / a.tiles_x) % a.tiles_y;
   const int bz = lb / (a.tiles_x * a.tiles_y);
   const int i0 = bx * TX, j0 = by * TY;
   const int i = i0 + tx, j = j0 + ty;
   const bool valid = owner && (i < a.nx) && (j < a.ny);
-  const int k0 = (DIMS == 3) ? bz * a.cz : 0;
-  const int k1 = (DIMS == 3) ? min(k0 + a.cz, a.nz) : 1;
+  int k0 = 0, k1 = 1;
+  if (DIMS == 3) {
+    if (zsched) {
+      k0 = a.zc0[bz];
+      k1 = a.zc0[bz + 1];
+    } else {
+      k0 = bz * a.cz;
+      k1 = min(k0 + a.cz, a.nz);
+    }
+    k0 = __builtin_amdgcn_readfirstlane(k0);
+    k1 = __builtin_amdgcn_readfirstlane(k1);
+  }
   const double gamma = a.gamma, gmm1 = a.gmm1;
 
   double cdt_max = 0.0;
@@ -491,5 +525,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
   }
   if (fail) atomicOr(a.err, fail);
 }
+
+// Launches stage_kernel<DIMS, REC, SOLVER, MODE, BCK_ANY, ...> (a domain
+// whose sides do not all share one rule) on `stream`; those instantiations
+// live in their own translation unit, pmx_hydro_any.hip.
+void launch_stage_any(int dims, int rec, int solver, int mode, const StageArgs& a, int nblocks,
+                      hipStream_t stream);
 
 }  // namespace pmx

@@ -51,14 +51,17 @@ __device__ inline double take_cdt(Scalars* s) {
   return __longlong_as_double((long long)m);
 }
 
-// Folds the cooling dt candidates of the spread slots into dtcool_bits.
+// Folds the cooling dt candidates of the spread slots into dtcool_bits
+// (all slots are read before any is cleared: one memory latency).
 __host__ __device__ inline void fold_dtcool(Scalars* s) {
   unsigned long long m = s->dtcool_bits;
+#pragma unroll
   for (int k = 0; k < RED_SLOTS; k++) {
     const unsigned long long v = s->dtcool_red[k * RED_STRIDE];
     m = v < m ? v : m;
-    s->dtcool_red[k * RED_STRIDE] = ~0ull;
   }
+#pragma unroll
+  for (int k = 0; k < RED_SLOTS; k++) s->dtcool_red[k * RED_STRIDE] = ~0ull;
   s->dtcool_bits = m;
 }
 
