@@ -212,7 +212,10 @@ int stage(pmx_hydro* h, const double* Vin, const double* V0, double* Vout, int m
   a.cdt_bits = h->s->cdt_red;
   a.err = &h->s->err;
   a.halt = loop ? &h->s->halt : nullptr;
-  int nblocks = a.tiles_x * a.tiles_y * a.tiles_z;
+  // with the z schedule, 8 ceil(tiles / 8) workgroups per chunk (a whole
+  // number per XCD; see stage_kernel)
+  int nblocks = a.nzc ? 8 * ((a.tiles_x * a.tiles_y + 7) / 8) * a.nzc
+                      : a.tiles_x * a.tiles_y * a.tiles_z;
 
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (h->timing) {

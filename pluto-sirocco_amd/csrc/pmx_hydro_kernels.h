@@ -76,9 +76,9 @@ struct StageArgs {
   int cz;                   // z planes per workgroup (uniform chunks, nzc == 0)
   int tiles_x, tiles_y, tiles_z;
   // z chunks of unequal length (3-D, nzc > 0): chunk c is planes
-  // [zc0[c], zc0[c+1]); workgroup b takes chunk b / (tiles_x tiles_y) of
-  // tile b % (tiles_x tiles_y), so the long chunks are dispatched first
-  // and the short ones fill the end of the launch
+  // [zc0[c], zc0[c+1]); each XCD's workgroups take its tiles chunk by
+  // chunk, so the long chunks are dispatched first and the short ones
+  // fill the end of the launch
   int nzc;
   int zc0[PMX_MAX_ZCHUNKS + 1];
   int write_u;              // store U (always needed except after the last stage)
@@ -197,12 +197,28 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
   const int tid = threadIdx.x;
   const bool owner = tid < NC;         // thread owns tile cell (tx, ty)
   const int tx = tid % TX, ty = tid / TX;
-  const int nblocks = a.tiles_x * a.tiles_y * a.tiles_z;
+  const int ntile = a.tiles_x * a.tiles_y;
   const bool zsched = DIMS == 3 && a.nzc > 0;
-  const int lb = __builtin_amdgcn_readfirstlane(zsched ? (int)blockIdx.x : xcd_remap(blockIdx.x, nblocks));
-  const int bx = lb % a.tiles_x;
-  const int by = (lb / a.tiles_x) % a.tiles_y;
-  const int bz = lb / (a.tiles_x * a.tiles_y);
+  int tile, bz;
+  if (zsched) {
+    // Workgroup b runs on XCD b % 8 (round-robin dispatch): each XCD owns
+    // a contiguous eighth of the tiles, so neighbouring tiles share its L2,
+    // and walks them chunk by chunk, long chunks first.  The launch has
+    // 8 ceil(ntile / 8) nzc workgroups; the few past an XCD's share end here.
+    const int x = blockIdx.x % 8, r = blockIdx.x / 8;
+    const int t0 = x * ntile / 8, nt = (x + 1) * ntile / 8 - t0;
+    if (r >= nt * a.nzc) return;
+    bz = r / nt;
+    tile = t0 + r % nt;
+  } else {
+    const int lb = xcd_remap(blockIdx.x, ntile * a.tiles_z);
+    bz = lb / ntile;
+    tile = lb % ntile;
+  }
+  tile = __builtin_amdgcn_readfirstlane(tile);
+  bz = __builtin_amdgcn_readfirstlane(bz);
+  const int bx = tile % a.tiles_x;
+  const int by = tile / a.tiles_x;
   const int i0 = bx * TX, j0 = by * TY;
   const int i = i0 + tx, j = j0 + ty;
   const bool valid = owner && (i < a.nx) && (j < a.ny);
