@@ -40,7 +40,7 @@ HBM_PEAK_GBS = 8000.0
 # (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a plain copy of known
 # size reads on this device), mean over the three RK3 stage launches; the
 # commands and per-launch values are in profiles/stage_kernel_sedov256.md.
-TRAFFIC_BYTES_PER_LAUNCH_256 = 3.8369e9
+TRAFFIC_BYTES_PER_LAUNCH_256 = 3.768e9
 TRAFFIC_SOURCE = ("profiles/stage_kernel_sedov256.md: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE "
                   "of this kernel (a profile value, not measured in this run)")
 
