@@ -543,6 +543,10 @@ int pmx_wind_surrogate_stats(pmx_wind *w, long long *listed, long long *accepted
  * fresh device buffers; reports the mean milliseconds per copy.  Used to
  * calibrate HBM byte counters and as the achievable-bandwidth reference. */
 int pmx_copy_probe(long long nbytes, int reps, double *ms_per_copy);
+/* Measurement helper (no counterpart in the reference): the same copy by
+ * the HIP runtime's own device-to-device hipMemcpyAsync, as a second
+ * reference for the achievable bandwidth. */
+int pmx_copy_probe_runtime(long long nbytes, int reps, double *ms_per_copy);
 
 /* Arithmetic check (no counterpart in the reference): the line force's
  * divisions by per-cell and per-run constants (Src/LineDriven/

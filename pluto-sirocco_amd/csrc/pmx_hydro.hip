@@ -746,6 +746,32 @@ int pmx_copy_probe(long long nbytes, int reps, double* ms_per_copy) {
   return 0;
 }
 
+int pmx_copy_probe_runtime(long long nbytes, int reps, double* ms_per_copy) {
+  if (nbytes <= 0 || reps <= 0) { set_error("bad argument"); return -1; }
+  void *a = nullptr, *b = nullptr;
+  hipEvent_t e0, e1;
+  PMX_HIP_CHECK(hipMalloc(&a, nbytes));
+  if (hipMalloc(&b, nbytes) != hipSuccess) { hipFree(a); set_error("hipMalloc"); return -1; }
+  hipMemset(a, 0, nbytes);
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipError_t err = hipMemcpyAsync(b, a, nbytes, hipMemcpyDeviceToDevice, 0);
+  if (err == hipSuccess) err = hipEventRecord(e0, 0);
+  for (int r = 0; r < reps && err == hipSuccess; r++)
+    err = hipMemcpyAsync(b, a, nbytes, hipMemcpyDeviceToDevice, 0);
+  if (err == hipSuccess) err = hipEventRecord(e1, 0);
+  if (err == hipSuccess) err = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  if (err != hipSuccess) { set_error(hipGetErrorString(err)); return -1; }
+  if (ms_per_copy) *ms_per_copy = ms / reps;
+  return 0;
+}
+
 int pmx_hydro_device_state(pmx_hydro* h, double** V, double** U) {
   if (!h) { set_error("null argument"); return -1; }
   if (V) *V = h->V[0];

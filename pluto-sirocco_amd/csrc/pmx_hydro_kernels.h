@@ -40,6 +40,9 @@ enum { MODE_FIRST = 0, MODE_WEIGHTED = 1, MODE_THIRD = 2 };
 #ifndef PMX_ZUNROLL
 #define PMX_ZUNROLL 1   // unroll of the z-plane loop
 #endif
+#ifndef PMX_LDS_PAD
+#define PMX_LDS_PAD 0   // extra doubles per LDS plane row
+#endif
 #ifndef PMX_PREFETCH_U
 #define PMX_PREFETCH_U 0   // load U and V0 of the cell at the top of the plane loop
 #endif
@@ -165,7 +168,8 @@ template <int DIMS, int REC, int SOLVER, int MODE, int BCK, int TX, int TY, int 
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PER_EU, PMX_WAVES_PER_EU))) void stage_kernel(StageArgs a) {
   constexpr int H = RecTraits<REC>::H;
   constexpr int NC = TX * TY;          // cells of a tile plane
-  constexpr int LW = TX + 2 * H;
+  constexpr int LWR = TX + 2 * H;                // plane row width
+  constexpr int LW = LWR + PMX_LDS_PAD;           // row stride in LDS
   constexpr int LH = TY + 2 * H;
   constexpr int NSX = (TX + 2) * TY;   // x-states per tile plane
   constexpr int NSY = TX * (TY + 2);   // y-states
@@ -341,7 +345,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
 #pragma unroll
       for (int q = 0; q < 5; q++) {
         int cc = H + c - 2 + q;
-        cc = cc < 0 ? 0 : (cc >= LW ? LW - 1 : cc);  // PLM (H=2) never reads the outer pair
+        cc = cc < 0 ? 0 : (cc >= LWR ? LWR - 1 : cc);  // PLM (H=2) never reads the outer pair
 #pragma unroll
         for (int nv = 0; nv < NV; nv++) s[q][nv] = sPlane[(nv * LH + H + r) * LW + cc];
       }

@@ -33,7 +33,7 @@ EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro
            "pmx_hydro_get_prim", "pmx_hydro_get_cons", "pmx_hydro_set_time",
            "pmx_hydro_get_time", "pmx_hydro_advance_step", "pmx_hydro_run",
            "pmx_hydro_sync", "pmx_hydro_run_graph", "pmx_hydro_stage_timing",
-           "pmx_hydro_device_state", "pmx_copy_probe", "pmx_check_div_rc",
+           "pmx_hydro_device_state", "pmx_copy_probe", "pmx_copy_probe_runtime", "pmx_check_div_rc",
            "pmx_wind_create", "pmx_wind_destroy", "pmx_wind_set_prim", "pmx_wind_get_prim",
            "pmx_wind_get_cons", "pmx_wind_set_heatcool", "pmx_wind_set_time",
            "pmx_wind_get_time", "pmx_wind_set_dts", "pmx_wind_advance_step", "pmx_wind_split_source",
@@ -137,6 +137,7 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     lib.pmx_hydro_stage_timing.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, ip]
     lib.pmx_hydro_device_state.argtypes = [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]
     lib.pmx_copy_probe.argtypes = [ctypes.c_longlong, ctypes.c_int, dp]
+    lib.pmx_copy_probe_runtime.argtypes = [ctypes.c_longlong, ctypes.c_int, dp]
     lib.pmx_check_div_rc.argtypes = [dp, ctypes.c_longlong, ctypes.c_double,
                                       ctypes.POINTER(ctypes.c_longlong)]
     lp = ctypes.POINTER(ctypes.c_long)
@@ -241,10 +242,13 @@ def check_div_rc(x: np.ndarray, c: float) -> int:
     return bad.value
 
 
-def copy_probe(nbytes: int, reps: int = 10) -> float:
-    """Mean ms of a plain device copy of nbytes (bandwidth reference)."""
+def copy_probe(nbytes: int, reps: int = 10, runtime: bool = False) -> float:
+    """Mean ms of a plain device copy of nbytes (bandwidth reference): the
+    library's streaming copy kernel, or (runtime) the HIP runtime's own
+    device-to-device hipMemcpyAsync."""
     ms = ctypes.c_double()
-    _check(load_library().pmx_copy_probe(nbytes, reps, ctypes.byref(ms)), "copy_probe")
+    fn = load_library().pmx_copy_probe_runtime if runtime else load_library().pmx_copy_probe
+    _check(fn(nbytes, reps, ctypes.byref(ms)), "copy_probe")
     return ms.value
 
 
