@@ -160,12 +160,13 @@ def wind_cpu_baseline(nr: int = 256, nt: int = 128, steps: int = 4) -> dict | No
                       f"1 process ({wall:.2f} s)"}
 
 
-def wind_legs(steps: int = 200, cpu: bool = True) -> dict:
+def wind_legs(steps: int = 200, cpu: bool = True, c3_only: bool = False) -> dict:
     """The wind workloads: C3 (512 x 256 spherical wind, plain sub-cycled
     cooling), C4 (the same with the MLP surrogate, trained here on the GPU
     on 2e6 samples of the plain integrator), the cooling step alone on
     cells drawn over the surrogate's training hull (where cooling is stiff),
-    and C5 (64 uneven grids advanced as one batch)."""
+    and C5 (64 uneven grids advanced as one batch).  c3_only: the C3 leg
+    alone (for counter passes, which serialise every kernel)."""
     import plutomx
     out = {}
     cfg = setups.c3_wind(512, 256, setups.COOL_PLAIN)
@@ -204,6 +205,8 @@ def wind_legs(steps: int = 200, cpu: bool = True) -> dict:
                                             f"cooling, {steps} steps, graph replay")
     if cpu:
         out["c3_plain"]["cpu_baseline"] = wind_cpu_baseline()
+    if c3_only:
+        return out
     # the same wind with the production problem's set-up (cv_idl boundaries
     # and floors, MULTID shock flattening, characteristic limiting, entropy
     # switch, HLL), 20 steps: from this start state the set-up reaches a zone
@@ -337,6 +340,8 @@ def main() -> int:
     ap.add_argument("--wind-only", action="store_true",
                     help="only the wind workloads (for profiling them)")
     ap.add_argument("--wind-steps", type=int, default=200)
+    ap.add_argument("--wind-c3-only", action="store_true",
+                    help="with --wind-only: the C3 leg alone (for counter passes)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB megabytes (counter calibration)")
     args = ap.parse_args()
@@ -348,7 +353,8 @@ def main() -> int:
 
     import plutomx
     if args.wind_only:
-        print(json.dumps({"wind": wind_legs(args.wind_steps, cpu=not args.no_cpu)}), flush=True)
+        print(json.dumps({"wind": wind_legs(args.wind_steps, cpu=not args.no_cpu,
+                                            c3_only=args.wind_c3_only)}), flush=True)
         return 0
     if args.copy_probe:
         nbytes = args.copy_probe * 1 << 20

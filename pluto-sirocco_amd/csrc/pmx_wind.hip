@@ -1905,19 +1905,37 @@ void wind_batch_stage_kernel(const StageArgs* __restrict__ args, const int* __re
 
 // Per grid: fold the stage-1 C_dt into invDt (update_stage.c:389-394),
 // t += dt, and NextTimeStep + reset when `next` (see wind_end_kernel).
-__global__ void wind_batch_end_kernel(const StageArgs* args, int G, int next, const double* cfl,
-                                      const double* cfl_max_var) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void wind_batch_end_kernel(const StageArgs* args, int G, int next,
+                                                            const double* cfl,
+                                                            const double* cfl_max_var) {
+  static_assert(RED_SLOTS == 32, "one lane per spread slot and reduction");
+  const int g = blockIdx.x;          // one workgroup per grid
   if (g >= G) return;
   Scalars* s = args[g].s;
-  if (s->halt) return;
+  if (s->halt) return;               // uniform over the workgroup
+  // lanes 0-31 fold the stage-1 C_dt slots (max), lanes 32-63 the cooling
+  // dt slots (min), each clearing its slot; the halves reduce separately
+  const int l = threadIdx.x;
+  const bool cdt_half = l < RED_SLOTS;
+  const int w = red_index(cdt_half ? l : l - RED_SLOTS);
+  unsigned long long v = cdt_half ? s->cdt_red[w] : s->dtcool_red[w];
+  if (cdt_half) s->cdt_red[w] = 0ull;
+  else s->dtcool_red[w] = ~0ull;
+  for (int off = RED_SLOTS / 2; off > 0; off >>= 1) {
+    const unsigned long long u = __shfl_xor(v, off);
+    v = cdt_half ? (u > v ? u : v) : (u < v ? u : v);
+  }
+  const unsigned long long cmax = __shfl(v, 0), cmin = __shfl(v, RED_SLOTS);
+  if (l != 0) return;
+  const unsigned long long cb = s->cdt_bits > cmax ? s->cdt_bits : cmax;
+  s->cdt_bits = 0ull;
   double inv = s->inv_dt_hyp;
-  inv = dmax(inv, take_cdt(s));
+  inv = dmax(inv, __longlong_as_double((long long)cb));
   inv /= (double)2;
   s->inv_dt_hyp = inv;
   const double dt = s->dt;
   s->t += dt;
-  fold_dtcool(s);
+  if (cmin < s->dtcool_bits) s->dtcool_bits = cmin;
   const double dt_cool = __longlong_as_double((long long)s->dtcool_bits);
   s->dt_cool = dt_cool;
   if (next) {
@@ -2075,7 +2093,7 @@ int batch_step(pmx_wind_batch* b, long st) {
     if (batch_cool(b) || batch_advance(b)) return -1;
   }
   const int next = (b->cooling == PMX_COOL_NONE) || (st % 2 == 1);
-  hipLaunchKernelGGL(wind_batch_end_kernel, dim3((b->G + 63) / 64), dim3(64), 0, b->stream,
+  hipLaunchKernelGGL(wind_batch_end_kernel, dim3(b->G), dim3(64), 0, b->stream,
                      b->args1, b->G, next, b->cfl, b->cfl_max_var);
   PMX_HIP_CHECK(hipGetLastError());
   return 0;
