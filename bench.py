@@ -160,6 +160,34 @@ def wind_cpu_baseline(nr: int = 256, nt: int = 128, steps: int = 4) -> dict | No
                       f"1 process ({wall:.2f} s)"}
 
 
+def c5_leg(steps: int, mlp=None) -> dict:
+    """64 uneven wind grids advanced as one batch (graph replay, autotuned
+    tiles), with the surrogate cooling when a trained network is given."""
+    import plutomx
+    cfgs = setups.c5_batch()
+    with plutomx.WindBatch(cfgs) as b:
+        for w, c in zip(b.grids, cfgs):
+            w.set_prim(setups.wind_ic(c))
+            w.set_time(0.0, c.first_dt, 0)
+        b.set_step(0)
+        if mlp is not None:
+            b.set_surrogate(mlp)
+        tune = b.autotune()
+        b.run(2, graph=True)
+        b.sync()
+        t0 = time.perf_counter()
+        b.run(steps, graph=True)
+        b.sync()
+        el = time.perf_counter() - t0
+        cells = b.cells
+    cooling = "surrogate cooling with plain fallback" if mlp is not None else "plain cooling"
+    return {"grids": len(cfgs), "cells": cells,
+            "Mcell_updates_per_s": round(cells * steps / el / 1e6, 2),
+            "ms_per_step": round(1e3 * el / steps, 3), "autotune": tune,
+            "workload": "64 wind grids 128..768 x 64..384, L_x 1e36..1e38, "
+                        f"k 0.1..0.4, {cooling}, graph replay, {steps} steps"}
+
+
 def wind_legs(steps: int = 200, cpu: bool = True, c3_only: bool = False) -> dict:
     """The wind workloads: C3 (512 x 256 spherical wind, plain sub-cycled
     cooling), C4 (the same with the MLP surrogate, trained here on the GPU
@@ -302,28 +330,8 @@ def wind_legs(steps: int = 200, cpu: bool = True, c3_only: bool = False) -> dict
     # C5: batched parameter sweep, with the surrogate cooling (the network
     # trained above; the hull does not depend on L_x or k) and with the
     # plain integrator for comparison
-    cfgs = setups.c5_batch()
     for name, mlp in (("c5_batch", m), ("c5_batch_plain", None)):
-        with plutomx.WindBatch(cfgs) as b:
-            for w, c in zip(b.grids, cfgs):
-                w.set_prim(setups.wind_ic(c))
-                w.set_time(0.0, c.first_dt, 0)
-            b.set_step(0)
-            if mlp is not None:
-                b.set_surrogate(mlp)
-            tune = b.autotune()
-            b.run(2, graph=True)
-            b.sync()
-            t0 = time.perf_counter()
-            b.run(steps, graph=True)
-            b.sync()
-            el = time.perf_counter() - t0
-            cooling = "surrogate cooling with plain fallback" if mlp is not None else "plain cooling"
-            out[name] = {"grids": len(cfgs), "cells": b.cells,
-                         "Mcell_updates_per_s": round(b.cells * steps / el / 1e6, 2),
-                         "ms_per_step": round(1e3 * el / steps, 3), "autotune": tune,
-                         "workload": "64 wind grids 128..768 x 64..384, L_x 1e36..1e38, "
-                                     f"k 0.1..0.4, {cooling}, graph replay, {steps} steps"}
+        out[name] = c5_leg(steps, mlp)
     m.close()
     return out
 
@@ -342,6 +350,9 @@ def main() -> int:
     ap.add_argument("--wind-steps", type=int, default=200)
     ap.add_argument("--wind-c3-only", action="store_true",
                     help="with --wind-only: the C3 leg alone (for counter passes)")
+    ap.add_argument("--wind-c5-only", action="store_true",
+                    help="with --wind-only: the C5 batch with plain cooling alone "
+                         "(for counter passes)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MB",
                     help="only run a plain device copy of MB megabytes (counter calibration)")
     args = ap.parse_args()
@@ -352,6 +363,9 @@ def main() -> int:
         return 2
 
     import plutomx
+    if args.wind_only and args.wind_c5_only:
+        print(json.dumps({"wind": {"c5_batch_plain": c5_leg(args.wind_steps)}}), flush=True)
+        return 0
     if args.wind_only:
         print(json.dumps({"wind": wind_legs(args.wind_steps, cpu=not args.no_cpu,
                                             c3_only=args.wind_c3_only)}), flush=True)
