@@ -820,7 +820,7 @@ class Wind:
 
     def run(self, nsteps: int, graph: bool = False) -> None:
         """nsteps iterations of the main loop; graph=True replays a captured
-        HIP graph of a step pair (bit-identical)."""
+        HIP graph of 8 steps (bit-identical; a remainder runs eagerly)."""
         if graph:
             _check(self._lib.pmx_wind_run_graph(self._h, nsteps), "run_graph")
         else:
