@@ -22,7 +22,8 @@ struct DevScalars {
   double inv_dt_hyp;             // Dts->invDt_hyp of the last step
   double tstop;                  // runtime.tstop
   double dt_uncut;               // the dt a cut to tstop replaced
-  int err;                       // cons->prim flags of the last step
+  int err;                       // cons->prim flags of the step under way
+  int step_err;                  // cons->prim flags of the last completed step
   int halt;                      // PMX_FLAG_DT_TOO_SMALL / PMX_FLAG_TSTOP once set (sticky):
                                  // later main-loop steps are no-ops
   int last;                      // the next step is the last one (dt clamped to tstop - t)
@@ -60,10 +61,11 @@ __global__ void prep_kernel(DevScalars* s, const double* dx, double* dtdx, int n
 // main loop (loop != 0) a halted run does nothing; NextTimeStep's "dt is
 // too small" quit (main.c:672-683) halts it, and so does the end of the
 // step that reached tstop; otherwise the next step's dt is cut to land on
-// tstop when it would pass it (main.c:226-229).
-__global__ void finish_kernel(DevScalars* s, int ndim, double cfl, double cfl_max_var,
-                              double first_dt, int loop, const double* inv_in) {
-  if (loop && s->halt) return;
+// tstop when it would pass it (main.c:226-229).  One thread.  Returns
+// whether the run goes on.
+__device__ bool finish_step(DevScalars* s, int ndim, double cfl, double cfl_max_var,
+                            double first_dt, int loop, const double* inv_in) {
+  if (loop && s->halt) return false;
   double inv = inv_in ? *inv_in : 0.0;   // Dts->invDt_hyp the stage-1 maximum folds into
   unsigned long long m = 0ull;   // non-negative doubles order like their bit patterns
 #pragma unroll
@@ -75,6 +77,8 @@ __global__ void finish_kernel(DevScalars* s, int ndim, double cfl, double cfl_ma
   inv = dmax(inv, c);
   inv /= (double)ndim;
   s->inv_dt_hyp = inv;
+  s->step_err = s->err;
+  s->err = 0;
   double dt = s->dt;
   s->t += dt;
   double dt_hyp = 1.0 / inv;
@@ -82,7 +86,7 @@ __global__ void finish_kernel(DevScalars* s, int ndim, double cfl, double cfl_ma
   double dtnext = dt_hyp;
   dtnext = dmin(dtnext, cfl_max_var * dt);
   if (cfl_max_var != 1.0) s->dt = dtnext;
-  if (!loop) return;
+  if (!loop) return false;
   if (dtnext < first_dt * 1.e-9) s->halt |= PMX_FLAG_DT_TOO_SMALL;   // main.c:672-683
   if (s->last) {
     s->halt |= PMX_FLAG_TSTOP;
@@ -94,6 +98,26 @@ __global__ void finish_kernel(DevScalars* s, int ndim, double cfl, double cfl_ma
     s->dt_uncut = q.dt_uncut;
     s->last = q.last;
   }
+  return !s->halt;
+}
+
+// finish_step, then, in the main loop while the run goes on, what
+// prep_kernel would do for the next step: dt/dx with the next dt and the
+// cleared C_dt slots (the next step's own prep launch is then left out).
+__global__ __launch_bounds__(256) void finish_kernel(DevScalars* s, int ndim, double cfl,
+                                                     double cfl_max_var, double first_dt,
+                                                     int loop, const double* inv_in,
+                                                     const double* dx, double* dtdx, int ntot) {
+  __shared__ int go;
+  __shared__ double dt_next;
+  if (threadIdx.x == 0) {
+    go = finish_step(s, ndim, cfl, cfl_max_var, first_dt, loop, inv_in) ? 1 : 0;
+    dt_next = s->dt;
+  }
+  __syncthreads();
+  if (!go) return;
+  for (int q = threadIdx.x; q < ntot; q += blockDim.x) dtdx[q] = dt_next / dx[q];
+  if (threadIdx.x < RED_SLOTS) s->cdt_red[red_index(threadIdx.x)] = 0ull;
 }
 
 // Plain streaming copy, used to calibrate the HBM byte counters and to
@@ -241,13 +265,17 @@ int stage(pmx_hydro* h, const double* Vin, const double* V0, double* Vout, int m
 }
 
 // One AdvanceStep: Src/Time_Stepping/rk_step.c:117-330.  V[0] holds V^n on
-// entry and V^{n+1} on exit.
-int advance(pmx_hydro* h, const double* dt_override) {
+// entry and V^{n+1} on exit.  In the main loop the previous step's
+// finish_kernel has prepared this one (dt/dx, cleared slots) unless the
+// run is just starting (with_prep).
+int advance(pmx_hydro* h, const double* dt_override, bool with_prep) {
   int ntot = h->nx + h->ny + h->nz;
-  hipLaunchKernelGGL(prep_kernel, dim3((ntot + 255) / 256), dim3(256), 0, h->stream, h->s,
-                     h->dx, h->dtdx, ntot, dt_override);
-  PMX_HIP_CHECK(hipGetLastError());
   const bool loop = dt_override == nullptr;
+  if (with_prep || !loop) {
+    hipLaunchKernelGGL(prep_kernel, dim3((ntot + 255) / 256), dim3(256), 0, h->stream, h->s,
+                       h->dx, h->dtdx, ntot, dt_override);
+    PMX_HIP_CHECK(hipGetLastError());
+  }
   if (h->d.rk == 3) {
     if (stage(h, h->V[0], h->V[0], h->V[1], MODE_FIRST, 0, 0, 1, loop)) return -1;
     if (stage(h, h->V[1], h->V[0], h->V[2], MODE_WEIGHTED, 0.75, 0.25, 1, loop)) return -1;
@@ -256,9 +284,9 @@ int advance(pmx_hydro* h, const double* dt_override) {
     if (stage(h, h->V[0], h->V[0], h->V[1], MODE_FIRST, 0, 0, 1, loop)) return -1;
     if (stage(h, h->V[1], h->V[0], h->V[0], MODE_WEIGHTED, 0.5, 0.5, 1, loop)) return -1;
   }
-  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(1), 0, h->stream, h->s, h->d.dims,
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, h->stream, h->s, h->d.dims,
                      h->d.cfl, h->d.cfl_max_var, h->d.first_dt, loop ? 1 : 0,
-                     loop ? nullptr : dt_override + 1);
+                     loop ? nullptr : dt_override + 1, h->dx, h->dtdx, ntot);
   PMX_HIP_CHECK(hipGetLastError());
   return 0;
 }
@@ -544,7 +572,7 @@ int pmx_hydro_get_time(pmx_hydro* h, double* t, double* dt, double* inv_dt_hyp, 
   if (t) *t = s0.t;
   if (dt) *dt = s0.dt;
   if (inv_dt_hyp) *inv_dt_hyp = s0.inv_dt_hyp;
-  if (flags) *flags = s0.err | s0.halt;
+  if (flags) *flags = s0.step_err | s0.err | s0.halt;
   return 0;
 }
 
@@ -552,7 +580,7 @@ int pmx_hydro_advance_step(pmx_hydro* h, double dt, double* inv_dt_hyp) {
   if (!h) { set_error("null argument"); return -1; }
   const double in[2] = {dt, inv_dt_hyp ? *inv_dt_hyp : 0.0};
   PMX_HIP_CHECK(hipMemcpyAsync(h->dt_in, in, sizeof(in), hipMemcpyHostToDevice, h->stream));
-  if (advance(h, h->dt_in)) return -1;
+  if (advance(h, h->dt_in, true)) return -1;
   if (h->timing) collect_timing(h);
   DevScalars s0;
   PMX_HIP_CHECK(hipMemcpyAsync(&s0, h->s, sizeof(s0), hipMemcpyDeviceToHost, h->stream));
@@ -564,7 +592,7 @@ int pmx_hydro_advance_step(pmx_hydro* h, double dt, double* inv_dt_hyp) {
 int pmx_hydro_run(pmx_hydro* h, int nsteps) {
   if (!h || nsteps < 0) { set_error("bad argument"); return -1; }
   for (int n = 0; n < nsteps; n++)
-    if (advance(h, nullptr)) return -1;
+    if (advance(h, nullptr, n == 0)) return -1;
   return 0;
 }
 
@@ -581,7 +609,7 @@ int pmx_hydro_run_graph(pmx_hydro* h, int nsteps) {
     hipGraph_t g;
     PMX_HIP_CHECK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
-    for (int q = 0; q < GRAPH_STEPS && !rc; q++) rc = advance(h, nullptr);
+    for (int q = 0; q < GRAPH_STEPS && !rc; q++) rc = advance(h, nullptr, q == 0);
     hipError_t e = hipStreamEndCapture(h->stream, &g);
     if (rc) return -1;
     if (e != hipSuccess) { set_error(std::string("capture: ") + hipGetErrorString(e)); return -1; }
@@ -592,8 +620,8 @@ int pmx_hydro_run_graph(pmx_hydro* h, int nsteps) {
   int n = 0;
   for (; nsteps - n >= GRAPH_STEPS; n += GRAPH_STEPS)
     PMX_HIP_CHECK(hipGraphLaunch(h->graph, h->stream));
-  for (; n < nsteps; n++)
-    if (advance(h, nullptr)) return -1;
+  for (const int n0 = n; n < nsteps; n++)
+    if (advance(h, nullptr, n == n0)) return -1;
   return 0;
 }
 
@@ -709,7 +737,7 @@ int pmx_hydro_advance_step_noalloc(const pmx_hydro_desc* desc, double* V, double
                                hipMemcpyHostToDevice, 0));
   PMX_HIP_CHECK(hipMemcpyAsync(h.s, &s0, sizeof(s0), hipMemcpyHostToDevice, 0));
   PMX_HIP_CHECK(hipMemcpyAsync(h.dt_in, in, sizeof(in), hipMemcpyHostToDevice, 0));
-  if (advance(&h, h.dt_in)) return -1;
+  if (advance(&h, h.dt_in, true)) return -1;
   PMX_HIP_CHECK(hipMemcpyAsync(&s0, h.s, sizeof(s0), hipMemcpyDeviceToHost, 0));
   PMX_HIP_CHECK(hipStreamSynchronize(0));
   if (inv_dt_hyp) *inv_dt_hyp = s0.inv_dt_hyp;
