@@ -35,7 +35,7 @@ enum { MODE_FIRST = 0, MODE_WEIGHTED = 1, MODE_THIRD = 2 };
                              // without spills (~215 VGPRs) and measured 3% faster than 3
 #endif
 #ifndef PMX_MERGE_XY
-#define PMX_MERGE_XY 1     // 3-D: x and y sweeps share phases, 4 barriers per plane instead of 6
+#define PMX_MERGE_XY 1     // x and y sweeps share phases, 4 barriers per plane instead of 6
 #endif
 #ifndef PMX_ZUNROLL
 #define PMX_ZUNROLL 1   // unroll of the z-plane loop
@@ -187,9 +187,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PMX_WAVES_PE
   // consecutive doubles (no bank conflicts).
   __shared__ double sPlane[NPLANE];    // V plane [nv][LH][LW]; later y-fluxes [7][NFY]
   // states [vp(5) vm(5)][item]; x and y each have their own when the two
-  // directions' phases share barriers (3-D: 72 KB, still 2 workgroups per
-  // CU; the 2-D kernel keeps the smaller footprint)
-  constexpr bool MERGE = PMX_MERGE_XY && DIMS == 3;
+  // directions' phases share barriers (72 KB, still 2 workgroups per CU;
+  // in 2-D, KH 256^2, 2% faster than separate phases with less LDS)
+  constexpr bool MERGE = PMX_MERGE_XY;
   __shared__ double sSt[(MERGE ? 2 : 1) * NST * 10];
   double* const sStX = sSt;
   double* const sStY = sSt + (MERGE ? NST * 10 : 0);
