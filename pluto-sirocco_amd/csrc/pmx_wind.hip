@@ -78,6 +78,9 @@ struct StageArgs {
   // ghost layers, [nt+4][nr+4]; null without flattening
   const unsigned char* shock;
   double* Ue;          // ENTROPY_SWITCH: Uc[ENTR] = rho s per zone; null without it
+  // the stage's line-force accelerations per zone, g_r [sv] then g_theta
+  // [sv]: written by the line-force kernel, read by the stage kernel
+  double* glf;
   int loop;            // a main-loop step: nothing to do once s->halt is set
 };
 
@@ -381,6 +384,87 @@ __device__ __forceinline__ void plm_zone(const StageArgs& a, int j, int i, const
 // a feature still check it.
 enum { FEAT_CHAR = 1, FEAT_ENT = 2, FEAT_TAB = 4, FEAT_ROE = 8 };
 
+// VGradCalc + LineForce of both sweeps for zone (i, j) of the stage input
+// (line_connect.c:628-738, :815-903): the x1 call takes the zone's PLM
+// face average (vp + vm)/2, the x2 call the zone centre; g_r and g_theta
+// go to a.glf.  A kernel of its own: the 36-bin loop with its two powers
+// per bin needs few registers beside the accumulators, while inside the
+// stage it would sit on top of the whole stencil (and drop the stage to
+// one wave per SIMD).
+template <int FEAT>
+__device__ __forceinline__ void lforce_cell(const StageArgs& a, int i, int j) {
+  const Const& k = a.k;
+  const Tables& T = a.T;
+  const int I = i + NG;
+  const long long cell = (long long)j * a.nr + i;
+  double vc[NVW], vc1[NVW];
+  {
+    double w1[NVW], e1[NVW], c[6], vpi[NVW], vmi[NVW];
+    ld6(a, a.Vin, j, i, vc);
+    ld6(a, j, i - 1, w1);
+    ld6(a, j, i + 1, e1);
+    ldc(T.pr, a.nr, I, c);
+    plm_zone<0, FEAT & FEAT_CHAR>(a, j, i, w1, vc, e1, c, vpi, vmi);
+#pragma unroll
+    for (int nv = 0; nv < NVW; nv++) vc1[nv] = 0.5 * (vpi[nv] + vmi[nv]);
+  }
+  VGrad vg;
+  vgrad_cell(a, j, i, vg);
+  // thermal speeds of the two LineForce calls (line_connect.c:848-866)
+  const double rho1 = vc1[RHO] * k.UD, rho2 = vc[RHO] * k.UD;
+  const double T1 = vc1[PRS] / vc1[RHO] * k.KELVIN * k.mu;
+  const double T2 = vc[PRS] / vc[RHO] * k.KELVIN * k.mu;
+  const double vth1 = sqrt((2.0 * C_kB * T1) / C_mp);
+  const double vth2 = sqrt((2.0 * C_kB * T2) / C_mp);
+  const bool tab = (FEAT & FEAT_TAB) && k.fm_points > 0;
+  // factored power: A = (sigma_e rho v_th)^alpha per call; the reference's
+  // form: log2 of num = sigma_e rho v_th per call, once per zone
+  double A1 = 0.0, A2 = 0.0;
+  PowNum P1{}, P2{};
+  if (!tab && k.fm_fast) {
+    A1 = force_factor(k, rho1, vth1);
+    A2 = force_factor(k, rho2, vth2);
+  } else if (!tab) {
+    P1 = pow_num(k.sigma_e * rho1 * vth1);
+    P2 = pow_num(k.sigma_e * rho2 * vth2);
+  }
+  const long long sa = a.sv * a.na;
+  const double* lf = T.lf + cell;
+  const double rc_c = 1.0 / C_c, r_ua = 1.0 / k.UA;
+  double g1r = 0.0, g2t = 0.0;
+  // the bin loop in two copies, chosen once per run (k.fast_div): its
+  // quotients through div_rc or through the division
+  auto bins = [&](auto fast) {
+    constexpr bool FAST = decltype(fast)::value;
+// two bins at a time give the scheduler two independent chains
+#ifndef PMX_ANG_UNROLL
+#define PMX_ANG_UNROLL 2
+#endif
+#pragma unroll PMX_ANG_UNROLL
+    for (int ang = 0; ang < a.na; ang++) {
+      const double* q = lf + (long long)ang * a.sv;
+      const double dvds = vgrad_dvds<FAST>(a, vg, q, sa, ang);
+      const double fr = q[LF_FR * sa], ft = q[LF_FT * sa];
+      double fm1, fm2;
+      if (tab) {   // both calls read the fit of this cell (line_connect.c:858-862)
+        fm1 = force_mult_tab(k, rho1, vth1, dvds, T.fm_lt, T.fm_lm + cell, a.sv);
+        fm2 = force_mult_tab(k, rho2, vth2, dvds, T.fm_lt, T.fm_lm + cell, a.sv);
+      } else if (k.fm_fast) {
+        force_mult_pair(k, A1, A2, dvds, fm1, fm2);
+      } else {
+        force_mult_shared(k, P1, P2, dvds, fm1, fm2);
+      }
+      // ((fm sigma_e f) / c) / UA, each quotient bit-identical
+      g1r += qdiv<FAST>(qdiv<FAST>(fm1 * k.sigma_e * fr, C_c, rc_c), k.UA, r_ua);
+      g2t += qdiv<FAST>(qdiv<FAST>(fm2 * k.sigma_e * ft, C_c, rc_c), k.UA, r_ua);
+    }
+  };
+  if (k.fast_div) bins(std::true_type{});
+  else bins(std::false_type{});
+  a.glf[cell] = g1r;
+  a.glf[a.sv + cell] = g2t;
+}
+
 template <int STAGE, int FEAT>
 __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, double& cdt,
                                            int& flags) {
@@ -409,63 +493,8 @@ __device__ __forceinline__ void cell_stage(const StageArgs& a, int i, int j, dou
     ldc(T.pr, a.nr, I, c);
     plm_zone<0, FEAT & FEAT_CHAR>(a, j, i, w1, vc, e1, c, vpi, vmi);
 
-    // ---- VGradCalc + LineForce for both sweeps (line_connect.c:628-738,
-    //      :815-903): x1 uses vc = (vp + vm)/2, x2 the cell centre ------
-    double g1r = 0.0, g2t = 0.0;
-    {
-      VGrad vg;
-      vgrad_cell(a, j, i, vg);
-      // thermal speeds of the two LineForce calls (line_connect.c:848-866)
-      double vc1[NVW];
-#pragma unroll
-      for (int nv = 0; nv < NVW; nv++) vc1[nv] = 0.5 * (vpi[nv] + vmi[nv]);
-      double rho1 = vc1[RHO] * k.UD, rho2 = vc[RHO] * k.UD;
-      double T1 = vc1[PRS] / vc1[RHO] * k.KELVIN * k.mu;
-      double T2 = vc[PRS] / vc[RHO] * k.KELVIN * k.mu;
-      double vth1 = sqrt((2.0 * C_kB * T1) / C_mp);
-      double vth2 = sqrt((2.0 * C_kB * T2) / C_mp);
-      const bool tab = (FEAT & FEAT_TAB) && k.fm_points > 0;
-      // factored power: A = (sigma_e rho v_th)^alpha per call; the
-      // reference's form: num = sigma_e rho v_th per call
-      double A1 = 0.0, A2 = 0.0;
-      if (!tab && k.fm_fast) { A1 = force_factor(k, rho1, vth1); A2 = force_factor(k, rho2, vth2); }
-      else if (!tab) { A1 = k.sigma_e * rho1 * vth1; A2 = k.sigma_e * rho2 * vth2; }
-      const long long sa = a.sv * a.na;
-      const double* lf = T.lf + cell;
-      const double rc_c = 1.0 / C_c, r_ua = 1.0 / k.UA;
-      // the bin loop in two copies, chosen once per run (k.fast_div): its
-      // quotients through div_rc or through the division
-      auto bins = [&](auto fast) {
-        constexpr bool FAST = decltype(fast)::value;
-// two bins at a time give the scheduler two independent chains (C3: one
-// wave per slot, nothing else hides latency); 0.097 -> 0.093 ms per stage,
-// C5 14.2 -> 13.6 ms per step; four cost the feature kernels registers.
-// The sums keep their order.
-#ifndef PMX_ANG_UNROLL
-#define PMX_ANG_UNROLL 2
-#endif
-#pragma unroll PMX_ANG_UNROLL
-      for (int ang = 0; ang < a.na; ang++) {
-        const double* q = lf + (long long)ang * a.sv;
-        const double dvds = vgrad_dvds<FAST>(a, vg, q, sa, ang);
-        double fr = q[LF_FR * sa], ft = q[LF_FT * sa];
-        double fm1, fm2;
-        if (tab) {   // both calls read the fit of this cell (line_connect.c:858-862)
-          fm1 = force_mult_tab(k, rho1, vth1, dvds, T.fm_lt, T.fm_lm + cell, a.sv);
-          fm2 = force_mult_tab(k, rho2, vth2, dvds, T.fm_lt, T.fm_lm + cell, a.sv);
-        } else if (k.fm_fast) {
-          force_mult_pair(k, A1, A2, dvds, fm1, fm2);
-        } else {
-          force_mult_ref(k, A1, A2, dvds, fm1, fm2);
-        }
-        // ((fm sigma_e f) / c) / UA, each quotient bit-identical
-        g1r += qdiv<FAST>(qdiv<FAST>(fm1 * k.sigma_e * fr, C_c, rc_c), k.UA, r_ua);
-        g2t += qdiv<FAST>(qdiv<FAST>(fm2 * k.sigma_e * ft, C_c, rc_c), k.UA, r_ua);
-      }
-      };
-      if (k.fast_div) bins(std::true_type{});
-      else bins(std::false_type{});
-    }
+    // ---- the line force of both sweeps, from the line-force kernel
+    const double g1r = a.glf[cell], g2t = a.glf[a.sv + cell];
 
     // ---- x1 faces i-1/2 and i+1/2 --------------------------------------
     double rhs1[NVW], Fm[NVW], Fp[NVW], pm, pp, cm1, cp1;
@@ -758,6 +787,16 @@ void wind_stage_kernel_w(StageArgs a) {
   stage_grid<STAGE, FEAT>(a);
 }
 
+// The stage's line force (lforce_cell) over the interior, before the stage
+// kernel.  FEAT: FEAT_CHAR and FEAT_TAB as for the stage.
+template <int FEAT>
+__global__ __launch_bounds__(BX* BY) void wind_lforce_kernel(StageArgs a) {
+  if (a.loop && a.s->halt) return;     // uniform over the launch
+  const int i = blockIdx.x * BX + threadIdx.x;
+  const int j = blockIdx.y * BY + threadIdx.y;
+  if (i < a.nr && j < a.nt) lforce_cell<FEAT>(a, i, j);
+}
+
 template <int MODE>
 __global__ __launch_bounds__(128) void wind_cool_kernel(CoolArgs a, unsigned long long* calls) {
   if (a.loop && a.s->halt) return;     // uniform over the launch
@@ -995,6 +1034,7 @@ struct pmx_wind {
   double* lf;
   unsigned char* shock;  // SHOCK_FLATTENING MULTID zone flags, or null
   double* Ue;            // ENTROPY_SWITCH: Uc[ENTR], or null
+  double* glf;           // the stage's line force, [2][sv]
   Tables T;
   Scalars* s;
   long step;
@@ -1080,6 +1120,7 @@ StageArgs stage_args(pmx_wind* w, const double* Vin, double* Vout, bool loop = f
   a.s = w->s;
   a.shock = w->shock;
   a.Ue = w->Ue;
+  a.glf = w->glf;
   a.loop = loop ? 1 : 0;
   return a;
 }
@@ -1115,6 +1156,14 @@ void launch_feat(pmx_wind* w, dim3 grid, dim3 block, const StageArgs& a) {
 
 template <int STAGE>
 void launch_stage(pmx_wind* w, dim3 grid, dim3 block, const StageArgs& a) {
+  switch (features(w) & (FEAT_CHAR | FEAT_TAB)) {
+    case 0: hipLaunchKernelGGL((wind_lforce_kernel<0>), grid, block, 0, w->stream, a); break;
+    case FEAT_CHAR:
+      hipLaunchKernelGGL((wind_lforce_kernel<FEAT_CHAR>), grid, block, 0, w->stream, a);
+      break;
+    default:
+      hipLaunchKernelGGL((wind_lforce_kernel<FEAT_CHAR | FEAT_TAB>), grid, block, 0, w->stream, a);
+  }
   switch (features(w)) {
     case 0: hipLaunchKernelGGL((wind_stage_kernel<STAGE, 0>), grid, block, 0, w->stream, a); break;
     case 1: launch_feat<STAGE, 1>(w, grid, block, a); break;
@@ -1267,6 +1316,7 @@ int pmx_wind_destroy(pmx_wind* w) {
   for (auto& p : w->pend_cool) { hipEventDestroy(p.first); hipEventDestroy(p.second); }
   hipFree(w->V[0]); hipFree(w->V[1]); hipFree(w->U); hipFree(w->xi_s); hipFree(w->tr_s);
   hipFree(w->pre); hipFree(w->fm); hipFree(w->tab); hipFree(w->lf); hipFree(w->s); hipFree(w->shock); hipFree(w->Ue);
+  hipFree(w->glf);
   hipFree(w->sw.list); hipFree(w->sw.fb); hipFree(w->sw.feat);
   hipFree(w->sw.f0); hipFree(w->sw.cnt); hipFree(w->cargs); hipFree(w->coff);
   if (w->copy_stream) {
@@ -1488,6 +1538,7 @@ int pmx_wind_create(const pmx_wind_desc* d, pmx_wind** out) {
   CK(hipMalloc(&w->s, sizeof(Scalars)));
   if (d->shock_flattening) CK(hipMalloc(&w->shock, (size_t)(w->nr + 4) * (w->nt + 4)));
   if (d->entropy_switch) CK(hipMalloc(&w->Ue, sizeof(double) * w->sv));
+  CK(hipMalloc(&w->glf, sizeof(double) * 2 * w->sv));
   CK(hipMalloc(&w->pre, sizeof(double) * 6 * w->sv));
   {
     std::vector<double> ones((size_t)6 * w->sv, 1.0);
@@ -1903,6 +1954,22 @@ void wind_batch_stage_kernel(const StageArgs* __restrict__ args, const int* __re
   }
 }
 
+// The stages' line force (lforce_cell) over the same (grid, tile) items.
+template <int FEAT, int TBX, int TBY>
+__global__ __launch_bounds__(TBX* TBY) void wind_batch_lforce_kernel(
+    const StageArgs* __restrict__ args, const int* __restrict__ tile_off, int G, int total) {
+  for (int item = blockIdx.x; item < total; item += gridDim.x) {
+    const int g = __builtin_amdgcn_readfirstlane(find_item(tile_off, G, item));
+    const StageArgs& a = args[g];
+    if (a.s->halt) continue;           // a halted grid (uniform over the workgroup)
+    const int tx_n = (a.nr + TBX - 1) / TBX;
+    const int local = item - tile_off[g];
+    const int ty = local / tx_n, tx = local - ty * tx_n;
+    const int i = tx * TBX + threadIdx.x, j = ty * TBY + threadIdx.y;
+    if (i < a.nr && j < a.nt) lforce_cell<FEAT>(a, i, j);
+  }
+}
+
 // Per grid: fold the stage-1 C_dt into invDt (update_stage.c:389-394),
 // t += dt, and NextTimeStep + reset when `next` (see wind_end_kernel).
 __global__ __launch_bounds__(64) void wind_batch_end_kernel(const StageArgs* args, int G, int next,
@@ -1999,6 +2066,12 @@ void launch_batch_stage(pmx_wind_batch* b) {
   const dim3 grid(b->blocks);
 #define PMX_SHAPE(q, X, Y)                                                                   \
   case q:                                                                                    \
+    if (b->feat_any)                                                                         \
+      hipLaunchKernelGGL((wind_batch_lforce_kernel<FEAT_CHAR | FEAT_TAB, X, Y>), grid,       \
+                         dim3(X, Y), 0, b->stream, args, b->tile_off, b->G, b->total_tiles); \
+    else                                                                                     \
+      hipLaunchKernelGGL((wind_batch_lforce_kernel<0, X, Y>), grid, dim3(X, Y), 0,           \
+                         b->stream, args, b->tile_off, b->G, b->total_tiles);                \
     if (b->feat_any)                                                                         \
       hipLaunchKernelGGL((wind_batch_stage_kernel<STAGE, 7, X, Y>), grid, dim3(X, Y), 0,     \
                          b->stream, args, b->tile_off, b->G, b->total_tiles);                \

@@ -364,6 +364,144 @@ PMX_HD void force_mult_ref(const Const& k, double num1, double num2, double dvds
   f2 = 1.0 + M2;
 }
 
+// ---------------------------------------------------------------------
+// The reference's power M = krad pow(t, alpha), t = num / dvds rounded as
+// the reference rounds it, with the logarithms shared between the calls:
+//   log2 t = log2 num - log2 dvds + log2(1 + eps),
+// where eps = (t dvds - num) / num is the quotient's relative rounding
+// error (the residual is exact in an FMA) and log2(1 + eps) = eps log2 e
+// to 2^-106.  A cell takes log2 num once per call and a bin log2 dvds once
+// for both calls, each in double-double (~2^-60), then 2^(alpha log2 t)
+// from a double-double argument: within about 0.6 ulp of the exact power
+// (the device pow and glibc's are within an ulp of it too).  About a third
+// of the two library powers' work per bin.
+// ---------------------------------------------------------------------
+struct DD {
+  double hi, lo;
+};
+
+PMX_HD DD dd_two_sum(double a, double b) {   // a + b = hi + lo exactly
+  const double s = a + b;
+  const double bb = s - a;
+  return {s, (a - (s - bb)) + (b - bb)};
+}
+
+constexpr double LOG2E_HI = 1.4426950408889634, LOG2E_LO = 2.0355273740931033e-17;
+constexpr double LN2_HI = 0.6931471805599453, LN2_LO = 2.3190468138462996e-17;
+
+// log2 x for finite x > 0, double-double.  x = m 2^e with m in
+// [sqrt(1/2), sqrt(2)); ln m = 2 atanh(s), s = (m - 1) / (m + 1), |s| <=
+// 0.1716: 2 s + 2 s^3 (1/3 + s^2/5 + ... + s^20/23), truncation < 2^-66.
+PMX_HD DD log2_dd(double x) {
+  int e;
+  double m = frexp(x, &e);                 // m in [0.5, 1)
+  if (m < 0.70710678118654752440) {
+    m = m * 2.0;
+    e -= 1;
+  }
+  const double a = m - 1.0;                // exact
+  const DD b = dd_two_sum(1.0, m);
+  const double sh = a / b.hi;
+  const double r = fma(-sh, b.hi, a) - sh * b.lo;   // a - sh (b.hi + b.lo)
+  const double sl = r / b.hi;
+  const double z = sh * sh;
+  double p = 1.0 / 23.0;
+  p = fma(p, z, 1.0 / 21.0);
+  p = fma(p, z, 1.0 / 19.0);
+  p = fma(p, z, 1.0 / 17.0);
+  p = fma(p, z, 1.0 / 15.0);
+  p = fma(p, z, 1.0 / 13.0);
+  p = fma(p, z, 1.0 / 11.0);
+  p = fma(p, z, 1.0 / 9.0);
+  p = fma(p, z, 1.0 / 7.0);
+  p = fma(p, z, 1.0 / 5.0);
+  p = fma(p, z, 1.0 / 3.0);
+  const DD lnm = dd_two_sum(2.0 * sh, 2.0 * sl + 2.0 * sh * z * p);
+  const double ph = lnm.hi * LOG2E_HI;
+  const double pl = fma(lnm.hi, LOG2E_HI, -ph) + (lnm.hi * LOG2E_LO + lnm.lo * LOG2E_HI);
+  const DD q = dd_two_sum((double)e, ph);
+  return dd_two_sum(q.hi, q.lo + pl);
+}
+
+// 2^y for a double-double y, one rounding at the end (|y| below 1030; the
+// far ranges overflow or underflow).  y = n + f, |f| <= 1/2; e^g with g =
+// f ln 2 = gh + gl: 1 + gh + gh^2 (1/2! + gh/3! + ... + gh^12/14!) (Taylor,
+// truncation < 2^-62) times (1 + gl).
+PMX_HD double exp2_dd(double yh, double yl) {
+  if (!(yh < 1030.0)) return yh + yh;      // +inf, or NaN as it came
+  if (yh < -1100.0) return 0.0;
+  const double n = rint(yh);
+  const double f = yh - n;                 // exact
+  const double gh = f * LN2_HI;
+  const double gl = fma(f, LN2_HI, -gh) + (f * LN2_LO + yl * LN2_HI);
+  double q = 1.0 / 87178291200.0;          // 1/14!
+  q = fma(q, gh, 1.0 / 6227020800.0);
+  q = fma(q, gh, 1.0 / 479001600.0);
+  q = fma(q, gh, 1.0 / 39916800.0);
+  q = fma(q, gh, 1.0 / 3628800.0);
+  q = fma(q, gh, 1.0 / 362880.0);
+  q = fma(q, gh, 1.0 / 40320.0);
+  q = fma(q, gh, 1.0 / 5040.0);
+  q = fma(q, gh, 1.0 / 720.0);
+  q = fma(q, gh, 1.0 / 120.0);
+  q = fma(q, gh, 1.0 / 24.0);
+  q = fma(q, gh, 1.0 / 6.0);
+  q = fma(q, gh, 0.5);
+  const double u = gh * gh * q;            // e^gh - 1 - gh
+  const DD one = dd_two_sum(1.0, gh);
+  const double r = one.hi + (one.lo + (u + gl * (1.0 + gh + u)));
+  return ldexp(r, (int)n);
+}
+
+// log2 of one LineForce call's sigma_e rho v_th, and its reciprocal for the
+// quotient's rounding error (any few correct bits do).
+struct PowNum {
+  double num, rn;
+  DD L;
+};
+
+PMX_HD PowNum pow_num(double num) {
+  PowNum p;
+  p.num = num;
+  p.rn = 1.0 / num;
+  p.L = (num >= 2.2250738585072014e-308 && num < HUGE_VAL) ? log2_dd(num) : DD{0.0, 0.0};
+  return p;
+}
+
+// krad pow(t, alpha) for t = num / dvds, dvds > 0, given D = log2 dvds.
+// A t or num that is not a normal positive number (0, subnormal, inf,
+// NaN: where the quotient's residual need not be exact) takes the library
+// power.
+PMX_HD double force_power_shared(const Const& k, const PowNum& a, double dvds, DD D) {
+  const double t = a.num / dvds;           // the reference's t (line_connect.c:880)
+  const double tiny = 2.2250738585072014e-308;   // smallest normal double
+  if (!(t >= tiny && t < HUGE_VAL && a.num >= tiny && a.num < HUGE_VAL))
+    return k.krad * pow(t, k.alpharad);
+  const double eps = fma(t, dvds, -a.num) * a.rn;
+  const DD d = dd_two_sum(a.L.hi, -D.hi);
+  const DD dl = dd_two_sum(d.hi, d.lo + ((a.L.lo - D.lo) + eps * LOG2E_HI));
+  const double yh = k.alpharad * dl.hi;
+  const double yl = fma(k.alpharad, dl.hi, -yh) + k.alpharad * dl.lo;
+  return k.krad * exp2_dd(yh, yl);
+}
+
+// Both calls' multipliers for one bin in the reference's form (as
+// force_mult_ref), the powers through force_power_shared.
+PMX_HD void force_mult_shared(const Const& k, const PowNum& a1, const PowNum& a2, double dvds,
+                              double& f1, double& f2) {
+  const double M_max = 4400.;
+  double M1 = 0.0, M2 = 0.0;
+  if (dvds > 0.0) {
+    const DD D = log2_dd(dvds);
+    M1 = force_power_shared(k, a1, dvds, D);
+    M2 = force_power_shared(k, a2, dvds, D);
+    if (M1 > M_max) M1 = M_max;
+    if (M2 > M_max) M2 = M_max;
+  }
+  f1 = 1.0 + M1;
+  f2 = 1.0 + M2;
+}
+
 PMX_HD void force_mult_pair(const Const& k, double A1, double A2, double dvds,
                             double& f1, double& f2) {
   const double M_max = 4400.;
