@@ -557,6 +557,16 @@ int pmx_copy_probe_runtime(long long nbytes, int reps, double *ms_per_copy);
  * reports how many of the quotients differ in any bit (0 expected). */
 int pmx_check_div_rc(const double *x, long long n, double c, long long *mismatches);
 
+/* Arithmetic check (no counterpart in the reference): the line force's
+ * force-multiplier power M = krad pow(t, alpha), t = num / dvds
+ * (Src/LineDriven/line_connect.c:880-881), as the device evaluates it:
+ * log2 num per LineForce call and log2 dvds per bin, shared, in
+ * double-double, and 2^(alpha log2 t) with the quotient's rounding error
+ * folded in.  out[i] = krad pow(num[i] / dvds[i], alpha) for dvds[i] > 0,
+ * else 0 (host arrays of n values). */
+int pmx_check_force_power(const double *num, const double *dvds, long long n, double krad,
+                          double alpha, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -233,12 +233,11 @@ __device__ __forceinline__ void vgrad_cell(const StageArgs& a, int j, int i, VGr
   g.z = T.x1[I] * cj * UL;
 }
 
-// q: the cell's line-force table entries of bin `ang`; sa: table stride.
+// g2, so, co: the cell's LF_F2, LF_SIN and LF_COS entries of bin `ang`.
 template <bool FAST = false>
-__device__ __forceinline__ double vgrad_dvds(const StageArgs& a, const VGrad& g,
-                                             const double* q, long long sa, int ang) {
+__device__ __forceinline__ double vgrad_dvds(const StageArgs& a, const VGrad& g, double g2,
+                                             double so, double co, int ang) {
   const Const& k = a.k;
-  double g2 = q[LF_F2 * sa];
   if (g2 == -999) return -999;
   const double ddx1 = g.maxds * a.T.sta[ang], ddx2 = g.maxds * a.T.cta[ang];
   const double ds = sqrt(ddx1 * ddx1 + ddx2 * ddx2);
@@ -250,13 +249,19 @@ __device__ __forceinline__ double vgrad_dvds(const StageArgs& a, const VGrad& g,
   t1 = (1.0 - g1) * g.v11_1 + g1 * g.v21_1;
   t2 = (1.0 - g1) * g.v12_1 + g1 * g.v22_1;
   double b1 = (1.0 - g2) * t1 + g2 * t2;
-  double so = q[LF_SIN * sa], co = q[LF_COS * sa];
   double vx2 = (b0 * k.UV * so + b1 * k.UV * co);
   double vz2 = (b0 * k.UV * co - b1 * k.UV * so);
   double st = a.T.sta[ang], ctt = a.T.cta[ang];
   double u1 = st * g.vx1 + ctt * g.vz1;
   double u2 = st * vx2 + ctt * vz2;
   return fabs((u2 - u1) / ds);
+}
+
+// q: the cell's line-force table entries of bin `ang`; sa: table stride.
+template <bool FAST = false>
+__device__ __forceinline__ double vgrad_dvds(const StageArgs& a, const VGrad& g,
+                                             const double* q, long long sa, int ang) {
+  return vgrad_dvds<FAST>(a, g, q[LF_F2 * sa], q[LF_SIN * sa], q[LF_COS * sa], ang);
 }
 
 __device__ __forceinline__ void ldc(const double* tab, int n, int idx, double* c) {
@@ -440,11 +445,23 @@ __device__ __forceinline__ void lforce_cell(const StageArgs& a, int i, int j) {
 #ifndef PMX_ANG_UNROLL
 #define PMX_ANG_UNROLL 2
 #endif
+    // a bin's five table entries are loaded one bin ahead (the loop is
+    // bound by their latency otherwise: two waves per SIMD hide little)
+    double nx[NLF];
+#pragma unroll
+    for (int f = 0; f < NLF; f++) nx[f] = lf[f * sa];
 #pragma unroll PMX_ANG_UNROLL
     for (int ang = 0; ang < a.na; ang++) {
-      const double* q = lf + (long long)ang * a.sv;
-      const double dvds = vgrad_dvds<FAST>(a, vg, q, sa, ang);
-      const double fr = q[LF_FR * sa], ft = q[LF_FT * sa];
+      double cur[NLF];
+#pragma unroll
+      for (int f = 0; f < NLF; f++) cur[f] = nx[f];
+      {
+        const double* q = lf + (long long)min(ang + 1, a.na - 1) * a.sv;
+#pragma unroll
+        for (int f = 0; f < NLF; f++) nx[f] = q[f * sa];
+      }
+      const double dvds = vgrad_dvds<FAST>(a, vg, cur[LF_F2], cur[LF_SIN], cur[LF_COS], ang);
+      const double fr = cur[LF_FR], ft = cur[LF_FT];
       double fm1, fm2;
       if (tab) {   // both calls read the fit of this cell (line_connect.c:858-862)
         fm1 = force_mult_tab(k, rho1, vth1, dvds, T.fm_lt, T.fm_lm + cell, a.sv);
@@ -1302,6 +1319,18 @@ __global__ void div_check_kernel(const double* __restrict__ x, long long n, doub
     if (__double_as_longlong(q) != __double_as_longlong(r)) mine++;
   }
   if (mine) atomicAdd(bad, mine);
+}
+
+// The line force's reference-form power krad pow(num / dvds, alpha) as the
+// line-force kernel evaluates it (force_power_shared), per element; 0 where
+// dvds <= 0 (a bin the reference leaves at M = 0).
+__global__ void power_check_kernel(const double* __restrict__ num, const double* __restrict__ dvds,
+                                   long long n, Const k, double* __restrict__ out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double d = dvds[i];
+    out[i] = d > 0.0 ? force_power_shared(k, pow_num(num[i]), d, log2_dd(d)) : 0.0;
+  }
 }
 }  // namespace wind
 }  // namespace pmx
@@ -2426,6 +2455,28 @@ int pmx_check_div_rc(const double* x, long long n, double c, long long* mismatch
   hipFree(bad);
   if (e != hipSuccess) { set_error(hipGetErrorString(e)); return -1; }
   *mismatches = (long long)h;
+  return 0;
+}
+
+int pmx_check_force_power(const double* num, const double* dvds, long long n, double krad,
+                          double alpha, double* out) {
+  if (!num || !dvds || !out || n <= 0) { set_error("bad argument"); return -1; }
+  double* d = nullptr;
+  PMX_HIP_CHECK(hipMalloc(&d, sizeof(double) * 3 * n));
+  pmx::wind::Const k{};
+  k.krad = krad;
+  k.alpharad = alpha;
+  hipError_t e = hipMemcpy(d, num, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d + n, dvds, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const long long nb = std::min<long long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(pmx::wind::power_check_kernel, dim3((unsigned)nb), dim3(256), 0, 0, d,
+                       d + n, n, k, d + 2 * n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, d + 2 * n, sizeof(double) * n, hipMemcpyDeviceToHost);
+  hipFree(d);
+  if (e != hipSuccess) { set_error(hipGetErrorString(e)); return -1; }
   return 0;
 }
 

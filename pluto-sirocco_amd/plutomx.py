@@ -34,6 +34,7 @@ EXPORTS = ["pmx_last_error", "pmx_hydro_create", "pmx_hydro_destroy", "pmx_hydro
            "pmx_hydro_get_time", "pmx_hydro_advance_step", "pmx_hydro_run",
            "pmx_hydro_sync", "pmx_hydro_run_graph", "pmx_hydro_stage_timing",
            "pmx_hydro_device_state", "pmx_copy_probe", "pmx_copy_probe_runtime", "pmx_check_div_rc",
+           "pmx_check_force_power",
            "pmx_wind_create", "pmx_wind_destroy", "pmx_wind_set_prim", "pmx_wind_get_prim",
            "pmx_wind_get_cons", "pmx_wind_set_heatcool", "pmx_wind_set_time",
            "pmx_wind_get_time", "pmx_wind_set_dts", "pmx_wind_advance_step", "pmx_wind_split_source",
@@ -140,6 +141,8 @@ def load_library(path: pathlib.Path | str | None = None) -> ctypes.CDLL:
     lib.pmx_copy_probe_runtime.argtypes = [ctypes.c_longlong, ctypes.c_int, dp]
     lib.pmx_check_div_rc.argtypes = [dp, ctypes.c_longlong, ctypes.c_double,
                                       ctypes.POINTER(ctypes.c_longlong)]
+    lib.pmx_check_force_power.argtypes = [dp, dp, ctypes.c_longlong, ctypes.c_double,
+                                          ctypes.c_double, dp]
     lp = ctypes.POINTER(ctypes.c_long)
     lib.pmx_wind_create.argtypes = [ctypes.POINTER(WindDesc), ctypes.POINTER(vp)]
     lib.pmx_wind_destroy.argtypes = [vp]
@@ -240,6 +243,19 @@ def check_div_rc(x: np.ndarray, c: float) -> int:
     _check(load_library().pmx_check_div_rc(_dptr(x), x.size, float(c), ctypes.byref(bad)),
            "check_div_rc")
     return bad.value
+
+
+def check_force_power(num: np.ndarray, dvds: np.ndarray, krad: float, alpha: float) -> np.ndarray:
+    """The device's line-force power krad pow(num / dvds, alpha) per element
+    (0 where dvds <= 0), as the line-force kernel evaluates it."""
+    num = np.ascontiguousarray(num, dtype=np.float64)
+    dvds = np.ascontiguousarray(dvds, dtype=np.float64)
+    if num.shape != dvds.shape:
+        raise ValueError("num and dvds differ in shape")
+    out = np.empty_like(num)
+    _check(load_library().pmx_check_force_power(_dptr(num), _dptr(dvds), num.size, float(krad),
+                                                float(alpha), _dptr(out)), "check_force_power")
+    return out
 
 
 def copy_probe(nbytes: int, reps: int = 10, runtime: bool = False) -> float:

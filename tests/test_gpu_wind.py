@@ -2,9 +2,11 @@
 
 Tolerances.  Everything in the wind step that uses only +, -, *, / and
 sqrt is bit-identical to the reference; the only differences come from
-the device math library's pow / exp / log10 / cosh, which can differ from
-the host libm in the last bit.  They enter through the force multiplier
-M = k t^alpha of the line force and through the cooling rates.  So:
+the force multiplier's power M = k t^alpha of the line force (evaluated
+from shared double-double logarithms, within about 0.6 ulp of the exact
+power where the host libm's is within 0.5: the two differ in the last bit
+for about 1% of arguments) and from the device math library's exp / log10
+/ cosh in the cooling rates.  So:
 
 * with k = 0 (no force multiplier), no cooling and no internal-boundary
   floors (whose pow(cs, 2) the device forms as cs*cs) the GPU step must be
@@ -14,6 +16,8 @@ M = k t^alpha of the line force and through the cooling rates.  So:
   floor for components that pass through zero, such as v_r in a wind that
   starts at rest), and the dt sequence must agree to 1e-12.
 """
+import math
+
 import numpy as np
 import pytest
 
@@ -123,6 +127,63 @@ def test_reciprocal_division_is_bit_identical(pmx):
         xs = np.concatenate([x, mult, np.nextafter(mult, np.inf), np.nextafter(mult, -np.inf),
                              edges])
         assert pmx.check_div_rc(xs, c) == 0, c
+
+
+def test_force_power_against_exact(pmx):
+    """The line force's power pow(num / dvds, alpha) as the device evaluates
+    it (log2 num and log2 dvds in double-double, shared between the calls,
+    the quotient's rounding error folded in) against the exact power of the
+    rounded quotient (mpmath at 160 bits): within 0.75 ulp, within one ulp
+    of the host libm's pow and equal to it in the great majority of cases,
+    over the ranges of the wind (num = sigma_e rho v_th from 1e-17 to 0.1,
+    dv/ds from 1e-12 to 1e4 s^-1, alpha 0.2 to 0.95)."""
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.prec = 160
+    rng = np.random.default_rng(11)
+    n = 4000
+    num = 10.0 ** rng.uniform(-17, -1, n)
+    dvds = 10.0 ** rng.uniform(-12, 4, n)
+    for alpha in (0.6, 0.2 + 0.75 * rng.random()):
+        out = pmx.check_force_power(num, dvds, 1.0, alpha)
+        t = num / dvds                      # the reference's rounded quotient
+        exact = np.array([float(mpmath.power(mpmath.mpf(x), mpmath.mpf(alpha))) for x in t])
+        err = np.array([float((mpmath.mpf(o) - mpmath.power(mpmath.mpf(x), mpmath.mpf(alpha)))
+                              / mpmath.mpf(float(np.spacing(e))))
+                        for o, x, e in zip(out, t, exact)])
+        assert np.abs(err).max() <= 0.75, (alpha, np.abs(err).max())
+        # the host libm's pow (numpy's own power may take a SIMD
+        # implementation that is not the libm's)
+        lib = np.array([math.pow(x, alpha) for x in t])
+        assert np.all(np.abs(out - lib) <= np.spacing(lib)), alpha
+        assert np.mean(out == lib) >= 0.95, (alpha, np.mean(out == lib))
+    # krad scales the rounded power (one more rounding, as the reference's
+    # krad * pow)
+    out = pmx.check_force_power(num, dvds, 0.2, 0.6)
+    lib = 0.2 * np.array([math.pow(x, 0.6) for x in num / dvds])
+    assert np.all(np.abs(out - lib) <= 2 * np.spacing(out))
+
+
+def test_force_power_special_cases(pmx):
+    """Zero, subnormal, overflowing and exact cases of the line force's
+    power, and dv/ds <= 0 (the bin has M = 0)."""
+    tiny = np.finfo(float).tiny
+    num = np.array([0.0, 1e-310, 1e-300, 1.0, 1e-3, 2.0 ** -40, 1e-5, 1e-5, 1.0, np.nan, 1e-300])
+    dvds = np.array([1.0, 1.0, 1e10, 1e-310, 1e300, 2.0 ** -20, 0.0, -3.0, 2.0 ** 10, 1.0,
+                     1e-310])
+    out = pmx.check_force_power(num, dvds, 1.0, 0.5)
+    with np.errstate(over="ignore", divide="ignore", invalid="ignore"):
+        t = num / dvds
+    lib = np.array([math.pow(x, 0.5) if d > 0.0 else 0.0 for x, d in zip(t, dvds)])
+    assert out[0] == 0.0                                   # num = 0
+    assert t[1] < tiny and abs(out[1] - lib[1]) <= np.spacing(lib[1])   # subnormal num
+    assert t[2] < tiny and abs(out[2] - lib[2]) <= np.spacing(lib[2])   # subnormal t
+    assert out[3] == np.inf                                # t overflows to inf
+    assert abs(out[4] - lib[4]) <= np.spacing(lib[4])      # t near the bottom of the normals
+    assert out[5] == 2.0 ** -10                            # exact: (2^-20)^0.5
+    assert out[6] == 0.0 and out[7] == 0.0                 # dv/ds <= 0
+    assert out[8] == 2.0 ** -5                             # exact: (2^-10)^0.5
+    assert np.isnan(out[9])
+    assert abs(out[10] - lib[10]) <= np.spacing(lib[10])   # subnormal dv/ds, t = 1e10
 
 
 def test_reciprocal_division_is_chosen_per_run(pmx):
