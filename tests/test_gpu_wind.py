@@ -5,7 +5,7 @@ sqrt is bit-identical to the reference; the only differences come from
 the force multiplier's power M = k t^alpha of the line force (evaluated
 from shared double-double logarithms, within about 0.6 ulp of the exact
 power where the host libm's is within 0.5: the two differ in the last bit
-for about 1% of arguments) and from the device math library's exp / log10
+for about 1.4% of arguments) and from the device math library's exp / log10
 / cosh in the cooling rates.  So:
 
 * with k = 0 (no force multiplier), no cooling and no internal-boundary
@@ -132,9 +132,10 @@ def test_reciprocal_division_is_bit_identical(pmx):
 def test_force_power_against_exact(pmx):
     """The line force's power pow(num / dvds, alpha) as the device evaluates
     it (log2 num and log2 dvds in double-double, shared between the calls,
-    the quotient's rounding error folded in) against the exact power of the
-    rounded quotient (mpmath at 160 bits): within 0.75 ulp, within one ulp
-    of the host libm's pow and equal to it in the great majority of cases,
+    the quotient's rounding error folded in, one rounding) against the
+    exact power of the rounded quotient (mpmath at 160 bits): within 0.6
+    ulp (0.575 measured), within one ulp of the host libm's pow and equal
+    to it for at least 97% of the arguments (98.6% measured),
     over the ranges of the wind (num = sigma_e rho v_th from 1e-17 to 0.1,
     dv/ds from 1e-12 to 1e4 s^-1, alpha 0.2 to 0.95)."""
     mpmath = pytest.importorskip("mpmath")
@@ -150,12 +151,12 @@ def test_force_power_against_exact(pmx):
         err = np.array([float((mpmath.mpf(o) - mpmath.power(mpmath.mpf(x), mpmath.mpf(alpha)))
                               / mpmath.mpf(float(np.spacing(e))))
                         for o, x, e in zip(out, t, exact)])
-        assert np.abs(err).max() <= 0.75, (alpha, np.abs(err).max())
+        assert np.abs(err).max() <= 0.6, (alpha, np.abs(err).max())
         # the host libm's pow (numpy's own power may take a SIMD
         # implementation that is not the libm's)
         lib = np.array([math.pow(x, alpha) for x in t])
         assert np.all(np.abs(out - lib) <= np.spacing(lib)), alpha
-        assert np.mean(out == lib) >= 0.95, (alpha, np.mean(out == lib))
+        assert np.mean(out == lib) >= 0.97, (alpha, np.mean(out == lib))
     # krad scales the rounded power (one more rounding, as the reference's
     # krad * pow)
     out = pmx.check_force_power(num, dvds, 0.2, 0.6)
