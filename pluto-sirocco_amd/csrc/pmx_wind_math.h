@@ -344,26 +344,6 @@ PMX_HD double force_mult_tab(const Const& k, double rho_cgs, double v_th, double
   return 1.0 + M;
 }
 
-// The reference's form of both calls' multipliers for one bin
-// (line_connect.c:879-893): t = (sigma_e rho v_th) / dvds and M = krad
-// pow(t, alpha) per call, capped at 4400; num = sigma_e rho v_th of the
-// call, in the reference's operation order.  The device pow and glibc's
-// are both within an ulp of the exact power, so M agrees with the
-// reference's to an ulp (and bit for bit on most inputs).
-PMX_HD void force_mult_ref(const Const& k, double num1, double num2, double dvds, double& f1,
-                           double& f2) {
-  const double M_max = 4400.;
-  double M1 = 0.0, M2 = 0.0;
-  if (dvds > 0.0) {
-    M1 = k.krad * pow(num1 / dvds, k.alpharad);
-    M2 = k.krad * pow(num2 / dvds, k.alpharad);
-    if (M1 > M_max) M1 = M_max;
-    if (M2 > M_max) M2 = M_max;
-  }
-  f1 = 1.0 + M1;
-  f2 = 1.0 + M2;
-}
-
 // ---------------------------------------------------------------------
 // The reference's power M = krad pow(t, alpha), t = num / dvds rounded as
 // the reference rounds it, with the logarithms shared between the calls:
@@ -372,9 +352,11 @@ PMX_HD void force_mult_ref(const Const& k, double num1, double num2, double dvds
 // error (the residual is exact in an FMA) and log2(1 + eps) = eps log2 e
 // to 2^-106.  A cell takes log2 num once per call and a bin log2 dvds once
 // for both calls, each in double-double (~2^-60), then 2^(alpha log2 t)
-// from a double-double argument: within about 0.6 ulp of the exact power
-// (the device pow and glibc's are within an ulp of it too).  About a third
-// of the two library powers' work per bin.
+// from a double-double argument, rounded once: within 0.6 ulp of the exact
+// power (0.575 the largest error found over the wind's ranges) and equal
+// to glibc's, which is correctly rounded in practice, for 98.6% of the
+// arguments (within an ulp of it for the rest).  About half the two
+// library powers' work per bin.
 // ---------------------------------------------------------------------
 struct DD {
   double hi, lo;
@@ -485,8 +467,10 @@ PMX_HD double force_power_shared(const Const& k, const PowNum& a, double dvds, D
   return k.krad * exp2_dd(yh, yl);
 }
 
-// Both calls' multipliers for one bin in the reference's form (as
-// force_mult_ref), the powers through force_power_shared.
+// The reference's form of both calls' multipliers for one bin
+// (line_connect.c:879-893): t = (sigma_e rho v_th) / dvds and M = krad
+// pow(t, alpha) per call, capped at 4400, the powers through
+// force_power_shared (log2 dvds once for both calls).
 PMX_HD void force_mult_shared(const Const& k, const PowNum& a1, const PowNum& a2, double dvds,
                               double& f1, double& f2) {
   const double M_max = 4400.;
