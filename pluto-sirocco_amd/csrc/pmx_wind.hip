@@ -339,7 +339,7 @@ __device__ __forceinline__ void roe_hll_face(const double* vL, const double* vR,
 template <int DIR, bool ROE = false>
 __device__ __forceinline__ void face(const double* vL, const double* vR, const Const& k,
                                      double* F, double& p, double& cmax, bool hll = false) {
-  if (ROE) {
+  if (ROE && k.roe) {   // a batch kernel built with Roe also runs HLLC / HLL grids
     if (hll) roe_hll_face<DIR>(vL, vR, k.gamma, k.gmm1, F, p, cmax);
     else roe<DIR, 2>(vL, vR, k.gamma, k.gmm1, F, p, cmax);
   } else if (hll || k.hll) {
@@ -2065,8 +2065,8 @@ struct pmx_wind_batch {
   long long* cell_off;    // device, G + 1 (multiples of 128)
   long long total_cells;  // padded
   long long cells;        // interior cells of all grids
-  int feat_any;           // some grid uses characteristic limiting, the entropy switch or
-                          // the tabulated force multiplier
+  int feat_any;           // OR of the grids' features (characteristic limiting, the entropy
+                          // switch, the tabulated force multiplier, Roe)
   int shape;              // index into the tile-shape table
   int blocks;             // persistent workgroups per stage launch
   hipStream_t stream;
@@ -2101,7 +2101,10 @@ void launch_batch_stage(pmx_wind_batch* b) {
     else                                                                                     \
       hipLaunchKernelGGL((wind_batch_lforce_kernel<0, X, Y>), grid, dim3(X, Y), 0,           \
                          b->stream, args, b->tile_off, b->G, b->total_tiles);                \
-    if (b->feat_any)                                                                         \
+    if (b->feat_any & FEAT_ROE)                                                              \
+      hipLaunchKernelGGL((wind_batch_stage_kernel<STAGE, 15, X, Y>), grid, dim3(X, Y), 0,    \
+                         b->stream, args, b->tile_off, b->G, b->total_tiles);                \
+    else if (b->feat_any)                                                                    \
       hipLaunchKernelGGL((wind_batch_stage_kernel<STAGE, 7, X, Y>), grid, dim3(X, Y), 0,     \
                          b->stream, args, b->tile_off, b->G, b->total_tiles);                \
     else                                                                                     \
@@ -2230,11 +2233,6 @@ int pmx_wind_batch_create(const pmx_wind_desc* descs, int ngrids, pmx_wind_batch
   for (int g = 0; g < ngrids; g++) {
     if (descs[g].cooling != b->cooling || descs[g].cooling == PMX_COOL_BLONDIN) {
       set_error("batched grids share one cooling mode: PMX_COOL_NONE or PMX_COOL_PLAIN");
-      pmx_wind_batch_destroy(b);
-      return -1;
-    }
-    if (descs[g].solver == PMX_SOLVER_ROE) {
-      set_error("the batch runs HLLC and HLL grids (a Roe grid runs on its own)");
       pmx_wind_batch_destroy(b);
       return -1;
     }

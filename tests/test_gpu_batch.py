@@ -158,8 +158,20 @@ def test_batch_surrogate_matches_single_grids(pmx):
             assert tb["dt"] == ts["dt"]
 
 
-def test_batch_refuses_roe_grids(pmx):
-    """A Roe grid runs on its own context; the batch says so."""
-    cfgs = small_batch()[:1] + [dataclasses.replace(small_batch()[1], solver=setups.ROE)]
-    with pytest.raises(pmx.PmxError, match="Roe"):
-        pmx.WindBatch(cfgs)
+def test_batch_with_roe_grids_matches_single_grids(pmx):
+    """Roe grids (one plain, one with the production boundaries and shock
+    flattening, whose flagged zones take Roe's HLL branch) in a batch with
+    two HLLC grids: every grid as on its own, bit for bit, with graph
+    replay (the batch's kernel is built with Roe and runs the HLLC grids
+    through HLLC)."""
+    roe = dataclasses.replace(small_batch()[1], solver=setups.ROE)
+    roe_prod = dataclasses.replace(setups.cvidl_wind(32, 16, dfloor=1.e-15), first_dt=1.e-3,
+                                   shock_flattening=True, solver=setups.ROE)
+    cfgs = [small_batch()[0], roe, small_batch()[2], roe_prod]
+    V0s = [setups.wind_ic_mixed(c, seed=80 + g, rho0_cgs=1.e-11, t_range=(1.e2, 3.e7))
+           for g, c in enumerate(cfgs)]
+    batch = run_batch(pmx, cfgs, V0s, 6, graph=True)
+    for cfg, V0, (Vb, Ub, tb) in zip(cfgs, V0s, batch):
+        Vs, Us, ts = run_single(pmx, cfg, V0, 6)
+        assert np.array_equal(Vb, Vs) and np.array_equal(Ub, Us)
+        assert tb["dt"] == ts["dt"]
