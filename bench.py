@@ -224,7 +224,7 @@ def wind_legs(steps: int = 200, cpu: bool = True, c3_only: bool = False) -> dict
             tm = w.timing(False)
         return {"Mcell_updates_per_s": round(cfg.ncells * steps / el / 1e6, 2),
                 "ms_per_step": round(1e3 * el / steps, 4),
-                "stage_ms_per_launch": round(tm["stage_ms"] / max(tm["stage_launches"], 1), 4),
+                "stage_ms_per_rk_stage": round(tm["stage_ms"] / max(tm["stage_launches"], 1), 4),
                 "cool_ms_per_call": round(tm["cool_ms"] / max(tm["cool_launches"], 1), 4),
                 "surrogate": st}, T
 
@@ -258,7 +258,7 @@ def wind_legs(steps: int = 200, cpu: bool = True, c3_only: bool = False) -> dict
     out["c3_production_setup"] = {
         "Mcell_updates_per_s": round(prod_cfg.ncells * psteps / el / 1e6, 2),
         "ms_per_step": round(1e3 * el / psteps, 4),
-        "stage_ms_per_launch": round(tm["stage_ms"] / max(tm["stage_launches"], 1), 4),
+        "stage_ms_per_rk_stage": round(tm["stage_ms"] / max(tm["stage_launches"], 1), 4),
         "workload": "C3 with cv_idl boundaries and floors, SHOCK_FLATTENING MULTID, "
                     f"CHAR_LIMITING, ENTROPY_SWITCH ALWAYS, HLL, {psteps} steps"}
     # output every 5 steps (V + the 23 user variables, 30 MB, read in place

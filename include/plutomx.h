@@ -324,8 +324,9 @@ int pmx_wind_run(pmx_wind *w, int nsteps);
  * it runs eagerly. */
 int pmx_wind_run_graph(pmx_wind *w, int nsteps);
 int pmx_wind_sync(pmx_wind *w);
-/* Per-kernel timing (HIP events on the context's stream) of the stage and
- * cooling kernels while enabled: summed milliseconds and launch counts. */
+/* Timing (HIP events on the context's stream) while enabled: the RK
+ * stages (each the line-force and the stage kernel) and the cooling calls,
+ * summed milliseconds and counts (stage_launches counts RK stages). */
 int pmx_wind_timing(pmx_wind *w, int enable, int reset, double *stage_ms, int *stage_launches,
                     double *cool_ms, int *cool_launches);
 
