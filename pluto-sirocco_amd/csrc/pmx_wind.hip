@@ -31,11 +31,13 @@ constexpr int BX = 64, BY = 2; // stage-kernel workgroup
 constexpr int NLF = 5;         // line-force tables per (angle, cell)
 constexpr int FM_POINTS_MAX = 4096;   // fit points of a tabulated force multiplier
 
-// Waves per SIMD that the batch stage kernel's register allocation targets.
-// The cell update needs ~230 VGPRs: the one-grid kernel reaches 2 waves per
-// SIMD by itself (231 VGPRs; forcing the attribute on it measured 13%
-// slower), while the batch kernel, left alone, takes 276 registers and 1
-// wave; forced to 2 (256 VGPRs, 20 spilled) it runs 1.5x faster.
+// Waves per SIMD that the register allocation of the batch stage kernels
+// and the one-grid feature kernels targets.  The plain
+// stage kernel fits 2 waves by itself (200-226 VGPRs); kernels built with
+// characteristic limiting, the entropy switch, the tabulated multiplier or
+// Roe would take more than 256 registers and fall to 1 wave, so they are
+// capped and spill.  (With the line force still inside it, the batch
+// kernel took 276 registers and 1 wave; capped at 2 it ran 1.5x faster.)
 #ifndef PMX_WIND_WAVES
 #define PMX_WIND_WAVES 2
 #endif
@@ -790,7 +792,8 @@ __device__ __forceinline__ void stage_grid(const StageArgs& a) {
 }
 
 // The plain stage fits 2 waves per SIMD by itself (forcing the attribute
-// on it costs 13%); with characteristic limiting, the entropy switch or the
+// on it cost 13% while the line force was still part of it); with
+// characteristic limiting, the entropy switch or the
 // tabulated force multiplier the allocation grows past 256 registers
 // (VGPRs + AGPRs) and falls to 1 wave unless capped.
 template <int STAGE, int FEAT>
