@@ -683,7 +683,12 @@ PMX_HD double radiat(const Const& k, const RadCell& R, double& E) {
   double lambda = nH * (ne * comp_heat + nH * xray_heat - ne * comp_cool - ne * line_cool -
                         ne * brem_cool);
   double rhs = (R.trc > 0.01) ? lambda * k.E_cost : 0.0;
-  rhs *= 1.0 - 1.0 / cosh(pow(T / minT, 12));
+  // 1 - 1/cosh(y), y = (T/minT)^12, rounds to 1.0 exactly once 1/cosh(y) <
+  // 2^-54 (y > 38): from T/minT >= 1.4 on (y >= 56.7, 1/cosh(y) < 5e-25,
+  // far beyond any last-bit error of pow or cosh) the factor is 1.0 and
+  // the two calls are skipped, bit for bit the same result
+  const double tr = T / minT;
+  if (!(tr >= 1.4)) rhs *= 1.0 - 1.0 / cosh(pow(tr, 12));
   return rhs;
 }
 
