@@ -176,12 +176,12 @@ void launch_stage(pmx_hydro* h, const StageArgs& a, int nblocks) {
   const dim3 g(nblocks), b(NTHREADS);
   switch (boundary_kind(h->d)) {
     case BCK_OUTFLOW:
-      hipLaunchKernelGGL((stage_kernel<DIMS, REC, SOLVER, MODE, BCK_OUTFLOW, TILE_X, TILE_Y,
-                                       NTHREADS>), g, b, 0, h->stream, a);
+      hipLaunchKernelGGL((stage_kernel<DIMS, REC, SOLVER, MODE, BCK_OUTFLOW, TILE_X,
+                                       tile_y<DIMS>(), NTHREADS>), g, b, 0, h->stream, a);
       break;
     case BCK_PERIODIC:
-      hipLaunchKernelGGL((stage_kernel<DIMS, REC, SOLVER, MODE, BCK_PERIODIC, TILE_X, TILE_Y,
-                                       NTHREADS>), g, b, 0, h->stream, a);
+      hipLaunchKernelGGL((stage_kernel<DIMS, REC, SOLVER, MODE, BCK_PERIODIC, TILE_X,
+                                       tile_y<DIMS>(), NTHREADS>), g, b, 0, h->stream, a);
       break;
     default:   // mixed sides: compiled in pmx_hydro_any.hip
       launch_stage_any(DIMS, REC, SOLVER, MODE, a, nblocks, h->stream);
@@ -229,7 +229,8 @@ int stage(pmx_hydro* h, const double* Vin, const double* V0, double* Vout, int m
   a.w0 = w0; a.wc = wc;
   a.cz = h->cz;
   a.tiles_x = (h->nx + TILE_X - 1) / TILE_X;
-  a.tiles_y = (h->ny + TILE_Y - 1) / TILE_Y;
+  const int ty = h->d.dims == 3 ? tile_y<3>() : tile_y<2>();
+  a.tiles_y = (h->ny + ty - 1) / ty;
   a.nzc = h->d.dims == 3 ? z_schedule(h->nz, a.zc0) : 0;
   a.tiles_z = (h->d.dims == 3) ? (a.nzc ? a.nzc : (h->nz + h->cz - 1) / h->cz) : 1;
   a.write_u = write_u;

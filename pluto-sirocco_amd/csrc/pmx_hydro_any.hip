@@ -14,7 +14,7 @@ namespace {
 
 template <int DIMS, int REC, int SOLVER, int MODE>
 void launch(const StageArgs& a, int nblocks, hipStream_t st) {
-  hipLaunchKernelGGL((stage_kernel<DIMS, REC, SOLVER, MODE, BCK_ANY, TILE_X, TILE_Y, NTHREADS>),
+  hipLaunchKernelGGL((stage_kernel<DIMS, REC, SOLVER, MODE, BCK_ANY, TILE_X, tile_y<DIMS>(), NTHREADS>),
                      dim3(nblocks), dim3(NTHREADS), 0, st, a);
 }
 
